@@ -1,0 +1,434 @@
+"""Schedule compiler and executor for ``kernel_fn`` on the HIP kernels.
+
+A ``serial`` network is compiled into fused *stages*, each one launch per
+tile of pairs:
+
+* the first stage fuses the input covariance with the first Conv (the
+  stencil of the input Gram is the Gram of zero-padded patches) and, when a
+  Relu follows, the arc-cosine map;
+* every later stage is ``[Conv] -> [Relu] -> [AvgPool]`` in one kernel;
+* the head is ``GlobalAvgPool`` or ``Flatten`` followed by the trailing Dense
+  layers folded into one affine map.
+
+Per-image self chains (``x1`` and ``x2`` against themselves, nngp only) run
+first and leave the ReLU normalisers ``q[n, h, w]`` of every stage; pair
+tiles then run the cross chain.  All arithmetic happens in libnta.so; this
+module only plans, allocates (torch caching allocator) and launches.
+
+Reference call path replaced: ``_preprocess_kernel_fn`` /
+``_inputs_to_kernel`` (``stax/requirements.py:641-830, 912-1054``), the
+``serial`` loop (``stax/combinators.py:40-68``) and the per-layer
+``kernel_fn``s listed in each stage; pair tiling follows ``nt.batch``
+(``batching.py:314-503``).
+"""
+import dataclasses
+import math
+from typing import List, Optional, Tuple
+
+import torch
+
+from . import _lib
+
+
+@dataclasses.dataclass
+class Stage:
+  conv: Optional[tuple] = None     # (fh, fw, valid, w_var, b_var)
+  relu: bool = False
+  pool: Optional[tuple] = None     # (ph, pw)
+  first: bool = False
+  # sizes, filled by the planner
+  hin: int = 0
+  win: int = 0
+  hc: int = 0
+  wc: int = 0
+  hout: int = 0
+  wout: int = 0
+
+  def flags(self, ntk: bool) -> int:
+    f = 0
+    if self.conv is not None:
+      f |= _lib.CONV
+      if self.conv[2]:
+        f |= _lib.CONV_VALID
+    if self.relu:
+      f |= _lib.RELU
+    if ntk:
+      f |= _lib.NTK
+    return f
+
+
+@dataclasses.dataclass
+class Plan:
+  stages: List[Stage]
+  head: Optional[str]                   # None | 'gap' | 'flatten'
+  head_coef: Tuple[float, ...]          # (a, b, c, d, e)
+  n_convs: int
+  is_gaussian: bool
+  out_channels: Optional[int]
+  out_spatial: Tuple[int, int]
+
+
+class _Unsupported(NotImplementedError):
+  pass
+
+
+def compile_plan(specs, in_shape) -> Plan:
+  """Groups layer specs into fused stages; validates shapes and options."""
+  from . import stax as S
+  if len(in_shape) != 4:
+    raise NotImplementedError(
+        'The HIP engine runs NHWC image inputs [N, H, W, C]; got input shape '
+        f'{tuple(in_shape)}.')
+  _, H, W, C = in_shape
+  stages: List[Stage] = []
+  cur: Optional[Stage] = None
+  head = None
+  dense = []
+  is_gaussian = False
+  n_convs = 0
+  chans = C
+
+  def close():
+    nonlocal cur
+    if cur is not None:
+      stages.append(cur)
+      cur = None
+
+  for spec in specs:
+    if head is not None:
+      if isinstance(spec, S._DenseSpec):
+        dense.append(spec)
+        is_gaussian = True
+        chans = spec.out_dim
+        continue
+      if isinstance(spec, S._IdentitySpec):
+        continue
+      raise NotImplementedError(
+          f'{spec.name} after a readout is not supported by the HIP engine; '
+          'only Dense layers may follow GlobalAvgPool/Flatten.')
+    if isinstance(spec, S._IdentitySpec):
+      continue
+    if isinstance(spec, S._ConvSpec):
+      if any(s != 1 for s in spec.strides):
+        raise NotImplementedError('Conv strides other than 1 are not '
+                                  'supported by the HIP engine.')
+      if spec.padding not in ('SAME', 'VALID'):
+        raise NotImplementedError(f'Conv padding {spec.padding!r} is not '
+                                  'supported by the HIP engine.')
+      if len(spec.filter_shape) != 2:
+        raise NotImplementedError('Only 2-D convolutions are supported.')
+      if spec.parameterization != 'ntk':
+        raise NotImplementedError('Only the ntk parameterization is '
+                                  'supported by the HIP engine.')
+      close()
+      fh, fw = spec.filter_shape
+      w_var = float(spec.W_std) ** 2
+      b_var = 0.0 if spec.b_std is None else float(spec.b_std) ** 2
+      cur = Stage(conv=(fh, fw, spec.padding == 'VALID', w_var, b_var),
+                  first=not stages and n_convs == 0)
+      if not cur.first and not stages:
+        raise NotImplementedError('The network must start with a Conv.')
+      n_convs += 1
+      is_gaussian = True
+      chans = spec.out_chan
+      continue
+    if isinstance(spec, S._ReluSpec) and not is_gaussian:
+      raise ValueError('The input to the activation function must be '
+                       'Gaussian, i.e. a random affine transform is '
+                       'required before the activation function.')
+    if not stages and cur is None:
+      raise NotImplementedError(
+          f'The HIP engine needs a Conv as the first layer, got {spec.name}.')
+    if isinstance(spec, S._ReluSpec):
+      if cur is None or cur.relu or cur.pool is not None:
+        close()
+        cur = Stage()
+      cur.relu = True
+      is_gaussian = False
+      continue
+    if isinstance(spec, S._AvgPoolSpec):
+      win = tuple(spec.window_shape)
+      st = tuple(spec.strides)
+      if len(win) != 2 or win != st:
+        raise NotImplementedError('AvgPool is supported with a 2-D window '
+                                  'equal to its strides.')
+      if spec.padding not in ('VALID', 'SAME'):
+        raise NotImplementedError(f'AvgPool padding {spec.padding!r}.')
+      if cur is None or cur.pool is not None:
+        close()
+        cur = Stage()
+      cur.pool = (spec.padding,) + win
+      continue
+    if isinstance(spec, (S._GlobalAvgPoolSpec, S._FlattenSpec)):
+      close()
+      head = 'gap' if isinstance(spec, S._GlobalAvgPoolSpec) else 'flatten'
+      if isinstance(spec, S._FlattenSpec):
+        is_gaussian = False
+      continue
+    if isinstance(spec, S._DenseSpec):
+      raise NotImplementedError('Dense before a readout (on spatial '
+                                'kernels) is not supported by the HIP '
+                                'engine.')
+    raise NotImplementedError(f'Layer {spec.name} is not supported by the '
+                              'HIP engine.')
+  close()
+  if not stages:
+    raise NotImplementedError('The network has no Conv layer.')
+
+  # The first stage carries only Conv (+ Relu); its pool becomes a stage.
+  if stages[0].pool is not None:
+    p = stages[0].pool
+    stages[0].pool = None
+    stages.insert(1, Stage(pool=p))
+
+  # Sizes.
+  h, w = H, W
+  for s in stages:
+    s.hin, s.win = h, w
+    if s.conv is not None:
+      fh, fw, valid = s.conv[:3]
+      if valid:
+        if fh > h or fw > w:
+          raise ValueError(f'VALID {fh}x{fw} filter on a {h}x{w} kernel.')
+        h, w = h - fh + 1, w - fw + 1
+    s.hc, s.wc = h, w
+    if s.pool is not None:
+      pad, ph, pw = s.pool
+      if pad == 'SAME' and (h % ph or w % pw):
+        raise NotImplementedError('AvgPool SAME with a window that does not '
+                                  'divide the spatial size.')
+      if ph > h or pw > w:
+        raise ValueError(f'AvgPool window {ph}x{pw} larger than {h}x{w}.')
+      h, w = h // ph, w // pw
+      s.pool = (ph, pw)
+    s.hout, s.wout = h, w
+
+  # Dense layers after the readout fold into nngp = a m + b and
+  # ntk = c m + d mt + e (linear.py:902-926 applied in sequence).
+  a, b, c, d, e = 1.0, 0.0, 1.0, 0.0, 0.0   # nngp = m, ntk = mt
+  ntk_m, ntk_t, ntk_1 = 0.0, 1.0, 0.0
+  for spec in dense:
+    wv = float(spec.W_std) ** 2
+    bv = 0.0 if spec.b_std is None else float(spec.b_std) ** 2
+    a, b = wv * a, wv * b + bv
+    ntk_m, ntk_t, ntk_1 = a + wv * ntk_m, wv * ntk_t, b + wv * ntk_1
+  coef = (a, b, ntk_m, ntk_t, ntk_1)
+  return Plan(stages=stages, head=head, head_coef=coef, n_convs=n_convs,
+              is_gaussian=is_gaussian, out_channels=chans,
+              out_spatial=(h, w))
+
+
+# ------------------------------------------------------------------ runner
+
+DEFAULT_TILE_BYTES = 6 << 30
+
+# When a list, every launch appends (entry point, start event, end event,
+# algorithmic HBM bytes of the launch, pairs in the launch).  Used by the
+# benchmark to time each kernel with HIP events on the launch stream.
+TRACE = None
+
+
+def _launch(name, alg_bytes, pairs, *args):
+  if TRACE is None:
+    _lib.call(name, *args)
+    return
+  s = torch.cuda.Event(enable_timing=True)
+  e = torch.cuda.Event(enable_timing=True)
+  s.record()
+  _lib.call(name, *args)
+  e.record()
+  TRACE.append((name, s, e, alg_bytes, pairs))
+
+
+def _state_elems(h, w):
+  return h * h * w * w
+
+
+def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
+               collect_q=None, head_out=None, ld=0):
+  """Runs all stages for one tile.  Self tile when ``n2 == 0`` (then
+  ``collect_q`` receives the normaliser of each ReLU stage, computed from the
+  self state on the way).  Returns the final (nngp, ntk) state when the plan
+  has no head, else writes the head output into ``head_out``."""
+  dev = xa.device
+  T = n1 if n2 == 0 else n1 * n2
+  _, H, W, C = xa.shape
+  nngp = ntk = None
+  fl_ntk = want_ntk and n2 != 0
+  for si, s in enumerate(plan.stages):
+    if s.first:
+      fh, fw, valid, w_var, b_var = s.conv
+      flags = s.flags(fl_ntk)
+      q1 = q2 = None
+      if s.relu:
+        if n2 == 0:
+          q = torch.empty((n1, s.hc, s.wc), dtype=torch.float32, device=dev)
+          _launch('nta_first_diag', 4 * n1 * s.hc * s.wc, 0, _lib.ptr(xa),
+                  n1, H, W, C, fh, fw, flags, w_var, b_var, _lib.ptr(q),
+                  stream)
+          collect_q[si] = q
+          q1 = q
+        else:
+          q1, q2 = qa[si], qb[si]
+      out = torch.empty((T,) + (s.hc, s.hc, s.wc, s.wc), dtype=torch.float32,
+                        device=dev)
+      out_t = torch.empty_like(out) if fl_ntk else None
+      nt = 2 if fl_ntk else 1
+      _launch('nta_first_stage', T * nt * 4 * out[0].numel(), T,
+              _lib.ptr(xa), _lib.ptr(xb), n1, n2, H, W, C, fh, fw, flags,
+              w_var, b_var, _lib.ptr(q1), _lib.ptr(q2), _lib.ptr(out),
+              _lib.ptr(out_t), stream)
+      nngp, ntk = out, out_t
+      continue
+    conv = s.conv if s.conv is not None else (1, 1, False, 1.0, 0.0)
+    fh, fw, valid, w_var, b_var = conv
+    flags = s.flags(fl_ntk)
+    q1 = q2 = None
+    if s.relu:
+      if n2 == 0:
+        q = torch.empty((n1, s.hc, s.wc), dtype=torch.float32, device=dev)
+        _launch('nta_stage_diag', 4 * n1 * s.hc * s.wc, 0, _lib.ptr(nngp),
+                n1, s.hin, s.win, fh, fw, flags, w_var, b_var, _lib.ptr(q),
+                stream)
+        collect_q[si] = q
+        q1 = q
+      else:
+        q1, q2 = qa[si], qb[si]
+    ph, pw = s.pool if s.pool is not None else (1, 1)
+    out = torch.empty((T, s.hout, s.hout, s.wout, s.wout),
+                      dtype=torch.float32, device=dev)
+    out_t = torch.empty_like(out) if fl_ntk else None
+    nt = 2 if fl_ntk else 1
+    _launch('nta_stage', T * nt * 4 * (nngp[0].numel() + out[0].numel()), T,
+            _lib.ptr(nngp), _lib.ptr(ntk), n1, n2, s.hin, s.win, fh, fw, ph,
+            pw, flags, w_var, b_var, _lib.ptr(q1), _lib.ptr(q2),
+            _lib.ptr(out), _lib.ptr(out_t), stream)
+    nngp, ntk = out, out_t
+  if plan.head is None:
+    return nngp, ntk
+  a, b, c, d, e = plan.head_coef
+  hflags = (_lib.FLATTEN if plan.head == 'flatten' else 0) | (
+      _lib.NTK if fl_ntk else 0)
+  last = plan.stages[-1]
+  out_n, out_t = head_out
+  nt = 2 if fl_ntk else 1
+  _launch('nta_head', T * nt * 4 * (nngp[0].numel() + 1), T, _lib.ptr(nngp),
+          _lib.ptr(ntk), n1, n2, last.hout, last.wout, hflags, a, b, c, d, e,
+          _lib.ptr(out_n), _lib.ptr(out_t), ld, stream)
+  return None
+
+
+def _per_pair_bytes(plan: Plan, ntk: bool) -> int:
+  m = 0
+  for s in plan.stages:
+    m = max(m, _state_elems(s.hc, s.wc), _state_elems(s.hout, s.wout))
+  # ping-pong input/output state, nngp (+ ntk)
+  return 4 * m * 2 * (2 if ntk else 1)
+
+
+def _self_chain(plan: Plan, x, want_cov, tile_bytes, stream):
+  """Returns (q list per stage [N, hc, wc], cov) for the images of x."""
+  N = x.shape[0]
+  per = _per_pair_bytes(plan, False)
+  chunk = max(1, min(N, tile_bytes // per))
+  qs = [None] * len(plan.stages)
+  q_parts = [[] for _ in plan.stages]
+  cov = None
+  if plan.head is not None:
+    cov = torch.empty((N,), dtype=torch.float32, device=x.device)
+  full = []
+  for i0 in range(0, N, chunk):
+    n = min(chunk, N - i0)
+    xa = x[i0:i0 + n]
+    collect = [None] * len(plan.stages)
+    if plan.head is not None:
+      _run_chain(plan, xa, None, n, 0, None, None, False, stream,
+                 collect_q=collect, head_out=(cov[i0:i0 + n], None), ld=0)
+    else:
+      st, _ = _run_chain(plan, xa, None, n, 0, None, None, False, stream,
+                         collect_q=collect)
+      if want_cov:
+        full.append(st)
+    for si, q in enumerate(collect):
+      if q is not None:
+        q_parts[si].append(q)
+  for si in range(len(plan.stages)):
+    if q_parts[si]:
+      qs[si] = torch.cat(q_parts[si]) if len(q_parts[si]) > 1 else \
+          q_parts[si][0]
+  if plan.head is None and want_cov:
+    cov = torch.cat(full) if len(full) > 1 else full[0]
+  return qs, cov
+
+
+def run(plan: Plan, x1, x2, want_nngp, want_ntk, want_cov,
+        tile_bytes=DEFAULT_TILE_BYTES, symmetric=True):
+  """Computes nngp/ntk (and cov1/cov2) for inputs x1 [N1,H,W,C], x2 or None.
+
+  Returns dict with 'nngp', 'ntk', 'cov1', 'cov2' tensors (or None)."""
+  stream = _lib.stream_handle(x1.device)
+  q1s, cov1 = _self_chain(plan, x1, want_cov, tile_bytes, stream)
+  if x2 is not None:
+    q2s, cov2 = _self_chain(plan, x2, want_cov, tile_bytes, stream)
+    xb_all = x2
+  else:
+    q2s, cov2 = q1s, None
+    xb_all = x1
+  N1, N2 = x1.shape[0], xb_all.shape[0]
+  dev = x1.device
+  per = _per_pair_bytes(plan, want_ntk)
+  T = max(1, tile_bytes // per)
+  if x2 is None and symmetric and plan.head is not None and T < N1 * N2:
+    ti = tj = max(1, min(N1, math.isqrt(T)))
+  else:
+    tj = min(N2, T)
+    ti = max(1, min(N1, T // tj))
+  out = {'cov1': cov1, 'cov2': cov2, 'nngp': None, 'ntk': None}
+  if plan.head is not None:
+    nngp = torch.empty((N1, N2), dtype=torch.float32, device=dev)
+    ntk = torch.empty((N1, N2), dtype=torch.float32, device=dev) \
+        if want_ntk else None
+  else:
+    last = plan.stages[-1]
+    shp = (N1, N2, last.hout, last.hout, last.wout, last.wout)
+    nngp = torch.empty(shp, dtype=torch.float32, device=dev)
+    ntk = torch.empty(shp, dtype=torch.float32, device=dev) \
+        if want_ntk else None
+  sym = symmetric and x2 is None and plan.head is not None and ti == tj
+  for i0 in range(0, N1, ti):
+    n1 = min(ti, N1 - i0)
+    for j0 in range(0, N2, tj):
+      n2 = min(tj, N2 - j0)
+      if sym and j0 + n2 <= i0:
+        continue           # strictly below the diagonal: mirrored later
+      qa = [None if q is None else q[i0:i0 + n1] for q in q1s]
+      qb = [None if q is None else q[j0:j0 + n2] for q in q2s]
+      xa = x1[i0:i0 + n1]
+      xb = xb_all[j0:j0 + n2]
+      if plan.head is not None:
+        on = nngp[i0:, j0:]
+        ot = ntk[i0:, j0:] if want_ntk else None
+        _run_chain(plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
+                   head_out=(on, ot), ld=N2)
+      else:
+        st_n, st_t = _run_chain(plan, xa, xb, n1, n2, qa, qb, want_ntk,
+                                stream)
+        sh = st_n.shape[1:]
+        nngp[i0:i0 + n1, j0:j0 + n2] = st_n.view((n1, n2) + sh)
+        if want_ntk:
+          ntk[i0:i0 + n1, j0:j0 + n2] = st_t.view((n1, n2) + sh)
+  if x2 is None and plan.head is not None:
+    _symmetrize_from_upper(nngp)
+    if want_ntk:
+      _symmetrize_from_upper(ntk)
+  out['nngp'] = nngp
+  out['ntk'] = ntk
+  return out
+
+
+def _symmetrize_from_upper(m):
+  """K(x, x) is symmetric: keep the upper triangle (which every tile above
+  the block diagonal computed) and mirror it, so the result is exactly
+  symmetric."""
+  up = torch.triu(m)
+  m.copy_(up + torch.triu(m, 1).T)

@@ -1,0 +1,448 @@
+"""Drop-in ``stax`` surface for the analytic CNN-kernel path.
+
+Every layer factory returns the reference's triple ``(init_fn, apply_fn,
+kernel_fn)`` (``utils/typing.py:225-229``):
+
+* ``init_fn(rng, input_shape) -> (output_shape, params)`` and
+  ``apply_fn(params, inputs)`` define the finite-width network (torch, NHWC,
+  ntk parameterization);
+* ``kernel_fn(x1, x2=None, get=None, **kw)`` computes the infinite-width
+  NNGP / NTK on the GPU through libnta.so (``_engine``).  Arguments, return
+  types and errors mirror ``AnalyticKernelFn`` of ``stax/requirements.py``
+  (``kernel_fn_any`` 955-1051): ``get=None`` returns a ``Kernel``, a string
+  returns that tensor, a tuple returns an ``AnalyticKernel`` namedtuple.
+
+Layers: ``Conv``, ``Relu``, ``AvgPool``, ``GlobalAvgPool``, ``Dense``,
+``Flatten``, ``Identity``, combined with ``serial``.  Networks outside what
+the engine fuses raise ``NotImplementedError``; there is no CPU path.
+"""
+import collections
+import dataclasses
+import functools
+import math
+import operator
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+from . import _engine
+from .kernel import Kernel
+
+# ------------------------------------------------------------------ specs
+
+
+@dataclasses.dataclass(frozen=True)
+class _ConvSpec:
+  out_chan: int
+  filter_shape: tuple
+  strides: tuple
+  padding: str
+  W_std: float
+  b_std: Optional[float]
+  parameterization: str = 'ntk'
+  name: str = 'Conv'
+
+
+@dataclasses.dataclass(frozen=True)
+class _ReluSpec:
+  name: str = 'Relu'
+
+
+@dataclasses.dataclass(frozen=True)
+class _AvgPoolSpec:
+  window_shape: tuple
+  strides: tuple
+  padding: str
+  normalize_edges: bool
+  name: str = 'AvgPool'
+
+
+@dataclasses.dataclass(frozen=True)
+class _GlobalAvgPoolSpec:
+  name: str = 'GlobalAvgPool'
+
+
+@dataclasses.dataclass(frozen=True)
+class _FlattenSpec:
+  name: str = 'Flatten'
+
+
+@dataclasses.dataclass(frozen=True)
+class _DenseSpec:
+  out_dim: int
+  W_std: float
+  b_std: Optional[float]
+  parameterization: str = 'ntk'
+  name: str = 'Dense'
+
+
+@dataclasses.dataclass(frozen=True)
+class _IdentitySpec:
+  name: str = 'Identity'
+
+
+# ------------------------------------------------------- finite networks
+
+def _as_generator(rng):
+  if isinstance(rng, torch.Generator):
+    return rng
+  g = torch.Generator()
+  g.manual_seed(int(rng) if rng is not None else 0)
+  return g
+
+
+def _same_pads(n, window, stride):
+  out = -(-n // stride)
+  total = max((out - 1) * stride + window - n, 0)
+  return total // 2, total - total // 2
+
+
+def _conv_layer_fns(spec: _ConvSpec):
+  fh, fw = spec.filter_shape
+
+  def init_fn(rng, input_shape):
+    g = _as_generator(rng)
+    n, h, w, c = input_shape
+    W = torch.randn((fh, fw, c, spec.out_chan), generator=g,
+                    dtype=torch.float64)
+    b = None if spec.b_std is None else torch.randn((spec.out_chan,),
+                                                    generator=g,
+                                                    dtype=torch.float64)
+    if spec.padding == 'SAME':
+      ho, wo = -(-h // spec.strides[0]), -(-w // spec.strides[1])
+    else:
+      ho = (h - fh) // spec.strides[0] + 1
+      wo = (w - fw) // spec.strides[1] + 1
+    return (n, ho, wo, spec.out_chan), (W, b)
+
+  def apply_fn(params, inputs, **kwargs):
+    W, b = params
+    x = inputs.permute(0, 3, 1, 2)
+    W = W.to(x.device, x.dtype)
+    if spec.padding == 'SAME':
+      ph = _same_pads(x.shape[2], fh, spec.strides[0])
+      pw = _same_pads(x.shape[3], fw, spec.strides[1])
+      x = F.pad(x, (pw[0], pw[1], ph[0], ph[1]))
+    fan_in = fh * fw * W.shape[2]
+    y = F.conv2d(x, W.permute(3, 2, 0, 1), stride=tuple(spec.strides))
+    y = y * (spec.W_std / math.sqrt(fan_in))
+    if b is not None:
+      y = y + spec.b_std * b.to(y.device, y.dtype)[None, :, None, None]
+    return y.permute(0, 2, 3, 1)
+  return init_fn, apply_fn
+
+
+def _dense_layer_fns(spec: _DenseSpec):
+  def init_fn(rng, input_shape):
+    g = _as_generator(rng)
+    c = input_shape[-1]
+    W = torch.randn((c, spec.out_dim), generator=g, dtype=torch.float64)
+    b = None if spec.b_std is None else torch.randn((spec.out_dim,),
+                                                    generator=g,
+                                                    dtype=torch.float64)
+    return tuple(input_shape[:-1]) + (spec.out_dim,), (W, b)
+
+  def apply_fn(params, inputs, **kwargs):
+    W, b = params
+    W = W.to(inputs.device, inputs.dtype)
+    y = inputs @ W * (spec.W_std / math.sqrt(W.shape[0]))
+    if b is not None:
+      y = y + spec.b_std * b.to(y.device, y.dtype)
+    return y
+  return init_fn, apply_fn
+
+
+def _relu_layer_fns(spec):
+  return (lambda rng, s: (tuple(s), ())), (lambda p, x, **kw: torch.relu(x))
+
+
+def _avg_pool_layer_fns(spec: _AvgPoolSpec):
+  wh, ww = spec.window_shape
+  sh, sw = spec.strides
+
+  def out_hw(h, w):
+    if spec.padding == 'SAME':
+      return -(-h // sh), -(-w // sw)
+    return (h - wh) // sh + 1, (w - ww) // sw + 1
+
+  def init_fn(rng, input_shape):
+    n, h, w, c = input_shape
+    return (n,) + out_hw(h, w) + (c,), ()
+
+  def apply_fn(params, inputs, **kwargs):
+    x = inputs.permute(0, 3, 1, 2)
+    if spec.padding == 'SAME':
+      ph = _same_pads(x.shape[2], wh, sh)
+      pw = _same_pads(x.shape[3], ww, sw)
+      ones = torch.ones_like(x[:1, :1])
+      x = F.pad(x, (pw[0], pw[1], ph[0], ph[1]))
+      y = F.avg_pool2d(x, (wh, ww), (sh, sw))
+      if spec.normalize_edges:
+        cnt = F.avg_pool2d(F.pad(ones, (pw[0], pw[1], ph[0], ph[1])),
+                           (wh, ww), (sh, sw))
+        y = y / cnt
+    else:
+      y = F.avg_pool2d(x, (wh, ww), (sh, sw))
+    return y.permute(0, 2, 3, 1)
+  return init_fn, apply_fn
+
+
+def _gap_layer_fns(spec):
+  def init_fn(rng, input_shape):
+    return (input_shape[0], input_shape[-1]), ()
+  return init_fn, (lambda p, x, **kw: x.mean(dim=tuple(range(1, x.ndim - 1))))
+
+
+def _flatten_layer_fns(spec):
+  def init_fn(rng, input_shape):
+    return (input_shape[0], int(np.prod(input_shape[1:]))), ()
+  return init_fn, (lambda p, x, **kw: x.reshape(x.shape[0], -1))
+
+
+def _identity_layer_fns(spec):
+  return (lambda rng, s: (tuple(s), ())), (lambda p, x, **kw: x)
+
+
+_FNS = {
+    _ConvSpec: _conv_layer_fns,
+    _DenseSpec: _dense_layer_fns,
+    _ReluSpec: _relu_layer_fns,
+    _AvgPoolSpec: _avg_pool_layer_fns,
+    _GlobalAvgPoolSpec: _gap_layer_fns,
+    _FlattenSpec: _flatten_layer_fns,
+    _IdentitySpec: _identity_layer_fns,
+}
+
+
+# ----------------------------------------------------------- kernel_fn
+
+def _canonicalize_get(get):
+  """``utils.py:139-194`` (``canonicalize_get``)."""
+  if get is None:
+    return True, ('nngp', 'ntk', 'cov1', 'cov2')
+  if isinstance(get, str):
+    return False, (get,)
+  get = tuple(get)
+  if len(set(get)) != len(get):
+    raise ValueError(f'Repeated entries in `get`: {get}.')
+  return False, get
+
+
+def _to_device_input(x, name):
+  if x is None:
+    return None
+  if isinstance(x, np.ndarray):
+    x = torch.from_numpy(np.ascontiguousarray(x))
+  if not isinstance(x, torch.Tensor):
+    raise TypeError(f'Wrong input types given. Found `{name}` of type '
+                    f'{type(x)}, need a torch.Tensor or numpy.ndarray.')
+  if x.ndim < 2:
+    raise ValueError('Inputs must be at least 2D (a batch dimension and a '
+                     f'channel/feature dimension), got {x.ndim}.')
+  if not torch.cuda.is_available():
+    raise RuntimeError('kernel_fn runs on the GPU through libnta.so and no '
+                       'GPU is visible.')
+  dev = x.device if x.is_cuda else torch.device('cuda', 0)
+  return x.to(device=dev, dtype=torch.float32).contiguous()
+
+
+class AnalyticKernelFn:
+  """``kernel_fn(x1, x2=None, get=None, **kw)`` of a ``serial`` network."""
+
+  def __init__(self, specs, init_fn=None):
+    self.specs = tuple(specs)
+    self._init_fn = init_fn
+    self._plans = {}
+    self.__name__ = 'kernel_fn'
+
+  def plan(self, in_shape):
+    key = tuple(in_shape[1:])
+    if key not in self._plans:
+      self._plans[key] = _engine.compile_plan(self.specs, (1,) + key)
+    return self._plans[key]
+
+  def __call__(self, x1_or_kernel, x2=None, get=None, *, pattern=None,
+               mask_constant=None, diagonal_batch=None,
+               diagonal_spatial=None, tile_bytes=None, **kwargs):
+    if isinstance(x1_or_kernel, Kernel):
+      raise NotImplementedError('Applying kernel_fn to a Kernel is not '
+                                'supported by the HIP engine.')
+    if pattern is not None or mask_constant is not None:
+      raise NotImplementedError('pattern / mask_constant are not supported '
+                                'by the HIP engine.')
+    if diagonal_batch is False:
+      raise NotImplementedError('diagonal_batch=False is not supported by '
+                                'the HIP engine.')
+    if diagonal_spatial:
+      raise NotImplementedError('diagonal_spatial=True is not supported by '
+                                'the HIP engine.')
+    return_kernel, get_t = _canonicalize_get(get)
+    for g in get_t:
+      if g not in Kernel.__dataclass_fields__:
+        raise ValueError(f'Unknown `get` entry {g!r}.')
+    x1 = _to_device_input(x1_or_kernel, 'x1')
+    x2t = _to_device_input(x2, 'x2')
+    if x2t is not None and tuple(x2t.shape[1:]) != tuple(x1.shape[1:]):
+      raise ValueError(f'x1 and x2 must agree apart from the batch axis, got '
+                       f'{tuple(x1.shape)} and {tuple(x2t.shape)}.')
+    plan = self.plan(x1.shape)
+    want_nngp = True
+    want_ntk = 'ntk' in get_t
+    want_cov = return_kernel or 'cov1' in get_t or 'cov2' in get_t
+    res = _engine.run(plan, x1, x2t, want_nngp, want_ntk, want_cov,
+                      tile_bytes=tile_bytes or _engine.DEFAULT_TILE_BYTES)
+    is_rev = plan.head is None and plan.n_convs % 2 == 1
+
+    def order(t, bnd):
+      if t is None or not is_rev:
+        return t
+      # natural [.., h, h', w, w'] -> reference [.., w, w', h, h']
+      return t.permute(tuple(range(bnd)) + (bnd + 2, bnd + 3, bnd, bnd + 1))
+
+    if x2t is None:
+      x1_is_x2 = True
+    else:
+      x1_is_x2 = (x1.shape == x2t.shape and
+                  bool(torch.all(torch.abs(x1 - x2t) < 1e-12)))
+    shape1 = self._out_shape(tuple(x1.shape))
+    shape2 = self._out_shape(tuple((x1 if x2t is None else x2t).shape))
+    k = Kernel(nngp=order(res['nngp'], 2), ntk=order(res['ntk'], 2),
+               cov1=order(res['cov1'], 1), cov2=order(res['cov2'], 1),
+               x1_is_x2=x1_is_x2, is_gaussian=plan.is_gaussian,
+               is_reversed=is_rev, is_input=False, diagonal_batch=True,
+               diagonal_spatial=False, shape1=shape1, shape2=shape2,
+               batch_axis=0, channel_axis=len(shape1) - 1)
+    if return_kernel:
+      return k
+    if len(get_t) == 1:
+      return getattr(k, get_t[0])
+    return collections.namedtuple('AnalyticKernel', get_t)(
+        *[getattr(k, g) for g in get_t])
+
+  def _out_shape(self, in_shape):
+    if self._init_fn is None:
+      return in_shape
+    out_shape, _ = self._init_fn(0, in_shape)
+    return tuple(out_shape)
+
+
+def _layer(spec):
+  init_fn, apply_fn = _FNS[type(spec)](spec)
+  kernel_fn = AnalyticKernelFn([spec], init_fn)
+  init_fn.__name__ = apply_fn.__name__ = spec.name
+  return init_fn, apply_fn, kernel_fn
+
+
+# ---------------------------------------------------------------- layers
+
+def Conv(out_chan: int, filter_shape: Sequence[int],
+         strides: Optional[Sequence[int]] = None, padding: str = 'VALID',
+         W_std: float = 1.0, b_std: Optional[float] = None,
+         dimension_numbers=None, parameterization: str = 'ntk',
+         s=(1, 1)):
+  """``Conv`` (``linear.py:936-991``)."""
+  if dimension_numbers not in (None, ('NHWC', 'HWIO', 'NHWC')):
+    raise NotImplementedError('Only NHWC/HWIO/NHWC dimension numbers.')
+  filter_shape = tuple(int(f) for f in filter_shape)
+  strides = tuple(strides) if strides is not None else (1,) * len(
+      filter_shape)
+  padding = padding.upper()
+  if padding not in ('SAME', 'VALID', 'CIRCULAR'):
+    raise ValueError(f'Unknown padding {padding!r}.')
+  return _layer(_ConvSpec(out_chan, filter_shape, strides, padding,
+                          float(W_std), None if b_std is None else
+                          float(b_std), parameterization))
+
+
+def Relu(do_stabilize: bool = False):
+  """``Relu`` (``elementwise.py:482-491``)."""
+  if do_stabilize:
+    raise NotImplementedError('do_stabilize=True is not supported.')
+  return _layer(_ReluSpec())
+
+
+def AvgPool(window_shape: Sequence[int],
+            strides: Optional[Sequence[int]] = None, padding: str = 'VALID',
+            normalize_edges: bool = False, batch_axis: int = 0,
+            channel_axis: int = -1):
+  """``AvgPool`` (``linear.py:1459-1500``)."""
+  if batch_axis != 0 or channel_axis not in (-1, 3):
+    raise NotImplementedError('Only NHWC pooling is supported.')
+  window_shape = tuple(int(w) for w in window_shape)
+  strides = tuple(strides) if strides is not None else (1,) * len(
+      window_shape)
+  return _layer(_AvgPoolSpec(window_shape, strides, padding.upper(),
+                             bool(normalize_edges)))
+
+
+def GlobalAvgPool(batch_axis: int = 0, channel_axis: int = -1):
+  """``GlobalAvgPool`` (``linear.py:1701-1721``)."""
+  if batch_axis != 0 or channel_axis not in (-1, 3):
+    raise NotImplementedError('Only NHWC global pooling is supported.')
+  return _layer(_GlobalAvgPoolSpec())
+
+
+def Dense(out_dim: int, W_std: float = 1., b_std: Optional[float] = None,
+          batch_axis: int = 0, channel_axis: int = -1,
+          parameterization: str = 'ntk', s=(1, 1)):
+  """``Dense`` (``linear.py:762-931``)."""
+  return _layer(_DenseSpec(int(out_dim), float(W_std),
+                           None if b_std is None else float(b_std),
+                           parameterization))
+
+
+def Flatten(batch_axis: int = 0, batch_axis_out: int = 0):
+  """``Flatten`` (``linear.py:1815-1906``)."""
+  if batch_axis != 0 or batch_axis_out != 0:
+    raise NotImplementedError('Only batch_axis=0 Flatten is supported.')
+  return _layer(_FlattenSpec())
+
+
+def Identity():
+  """``Identity`` (``linear.py``)."""
+  return _layer(_IdentitySpec())
+
+
+def serial(*layers):
+  """``serial`` (``combinators.py:40-68``): composes layer triples."""
+  init_fns, apply_fns, kernel_fns = zip(*layers)
+  specs = []
+  for kf in kernel_fns:
+    if not isinstance(kf, AnalyticKernelFn):
+      raise NotImplementedError(f'{kf!r} is not a layer of this package.')
+    specs.extend(kf.specs)
+
+  def init_fn(rng, input_shape):
+    g = _as_generator(rng)
+    params = []
+    shape = tuple(input_shape)
+    for f in init_fns:
+      shape, p = f(g, shape)
+      params.append(p)
+    return shape, tuple(params)
+
+  def apply_fn(params, inputs, **kwargs):
+    for f, p in zip(apply_fns, params):
+      inputs = f(p, inputs, **kwargs)
+    return inputs
+
+  return init_fn, apply_fn, AnalyticKernelFn(specs, init_fn)
+
+
+def myrtle(depth: int = 5, W_std: float = 2 ** 0.5, b_std: float = 0.,
+           readout_W_std: float = 1., readout_b_std: float = 0.,
+           depths: Optional[Sequence[int]] = None):
+  """Myrtle CNN with GlobalAvgPool readout: per stage ``d x [Conv 3x3 SAME,
+  Relu]`` then ``AvgPool 2x2/2``; depths 5: [2,1,1], 7: [2,2,2],
+  10: [3,3,3] (``notebooks/myrtle_kernel_with_neural_tangents.ipynb``)."""
+  if depths is None:
+    depths = {5: [2, 1, 1], 7: [2, 2, 2], 10: [3, 3, 3]}[depth]
+  layers = []
+  for d in depths:
+    layers += [Conv(1, (3, 3), padding='SAME', W_std=W_std, b_std=b_std),
+               Relu()] * d
+    layers += [AvgPool((2, 2), strides=(2, 2))]
+  layers += [GlobalAvgPool(), Dense(1, readout_W_std, readout_b_std)]
+  return serial(*layers)

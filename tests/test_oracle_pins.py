@@ -1,0 +1,307 @@
+"""Pins of the FP64 oracle against independent facts (CPU only).
+
+Each test checks an oracle function against something that does not reuse
+its formula: brute-force loops, the patch-Gram identity, textbook Gaussian
+integrals, the reference's own known answer (AvgPool), relational identities
+of the reference's tests, and library routines (scipy / numpy).
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.integrate
+import scipy.stats
+
+from conftest import rel_err
+from oracle import predict_oracle as po
+from oracle import stax_oracle as so
+
+
+# ------------------------------------------------------------ Conv stencil
+
+def _brute_conv(nngp, fh, fw, padding):
+  """Direct 4-D loop: out[h,h',w,w'] = mean over taps (k, l) of
+  in[h+k+o, h'+k+o, w+l+o, w'+l+o], zero outside; natural axis order."""
+  n1, n2, H, _, W, _ = nngp.shape
+  if padding == 'SAME':
+    oh, ow, Hc, Wc = -((fh - 1) // 2), -((fw - 1) // 2), H, W
+  else:
+    oh, ow, Hc, Wc = 0, 0, H - fh + 1, W - fw + 1
+  out = np.zeros((n1, n2, Hc, Hc, Wc, Wc))
+  for h in range(Hc):
+    for hp in range(Hc):
+      for w in range(Wc):
+        for wp in range(Wc):
+          acc = 0.0
+          for k in range(fh):
+            a, ap = h + k + oh, hp + k + oh
+            if not (0 <= a < H and 0 <= ap < H):
+              continue
+            for l in range(fw):
+              b, bp = w + l + ow, wp + l + ow
+              if not (0 <= b < W and 0 <= bp < W):
+                continue
+              acc = acc + nngp[:, :, a, ap, b, bp]
+          out[:, :, h, hp, w, wp] = acc / (fh * fw)
+  return out
+
+
+@pytest.mark.parametrize('fh,fw,padding', [(3, 3, 'SAME'), (2, 3, 'SAME'),
+                                           (3, 2, 'VALID'), (1, 3, 'SAME')])
+def test_conv_stencil_bruteforce(fh, fw, padding):
+  rng = np.random.default_rng(0)
+  x = rng.standard_normal((2, 3, 5, 5, 4, 4))
+  got = so.conv_kernel_full_spatial_shared(x, (fh, fw), (1, 1), padding, 2)
+  want = _brute_conv(x, fh, fw, padding)
+  # the reference leaves the spatial order reversed: [.., w, w', h, h']
+  np.testing.assert_allclose(got, np.transpose(want, (0, 1, 4, 5, 2, 3)),
+                             rtol=1e-13, atol=1e-13)
+
+
+def test_patch_gram_identity():
+  """stencil(input Gram) = Gram of zero-padded 3x3xC patches / (9C)."""
+  rng = np.random.default_rng(1)
+  x1 = rng.standard_normal((2, 5, 6, 3))
+  x2 = rng.standard_normal((3, 5, 6, 3))
+  k = so.inputs_to_kernel(x1, x2)
+  got = so.conv_kernel_full_spatial_shared(k.nngp, (3, 3), (1, 1), 'SAME', 2)
+
+  def patches(x):
+    xp = np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)))
+    p = np.stack([xp[:, a:a + 5, e:e + 6, :] for a in range(3)
+                  for e in range(3)], axis=3)
+    return p.reshape(x.shape[0], 5, 6, -1)
+
+  p1, p2 = patches(x1), patches(x2)
+  want = np.einsum('ihwk,jgvk->ijhgwv', p1, p2) / 27.0
+  np.testing.assert_allclose(got, np.transpose(want, (0, 1, 4, 5, 2, 3)),
+                             rtol=1e-12, atol=1e-12)
+
+
+def test_input_kernel_einsum_and_shapes():
+  """requirements.py:668-695 docstring shapes and the closed-form einsum."""
+  x = np.ones((10, 32, 16, 3))
+  k = so.inputs_to_kernel(x, None)
+  assert k.cov1.shape == (10, 32, 32, 16, 16)
+  assert k.nngp.shape == (10, 10, 32, 32, 16, 16)
+  rng = np.random.default_rng(2)
+  x1 = rng.standard_normal((2, 3, 4, 5))
+  x2 = rng.standard_normal((3, 3, 4, 5))
+  k = so.inputs_to_kernel(x1, x2)
+  for i in range(2):
+    for j in range(3):
+      for h in range(3):
+        for hp in range(3):
+          for w in range(4):
+            for wp in range(4):
+              v = np.dot(x1[i, h, w], x2[j, hp, wp]) / 5
+              assert abs(k.nngp[i, j, h, hp, w, wp] - v) < 1e-12
+  np.testing.assert_allclose(k.cov1[1, 2, 0, 3, 1],
+                             np.dot(x1[1, 2, 3], x1[1, 0, 1]) / 5)
+
+
+# ------------------------------------------------------------------ ReLU
+
+def _gauss_phi_expectations(q1, q2, c, a, b):
+  """E[phi(u) phi(v)] and E[phi'(u) phi'(v)] for phi = a min(x,0) +
+  b max(x,0), (u, v) ~ N(0, [[q1, c], [c, q2]]), by 1-D quadrature of the
+  textbook conditional moments of v | u."""
+  beta = c / q1
+  sig = math.sqrt(max(q2 - c * c / q1, 0.0))
+
+  def e_relu(mu):
+    if sig == 0:
+      return max(mu, 0.0)
+    z = mu / sig
+    return mu * scipy.stats.norm.cdf(z) + sig * scipy.stats.norm.pdf(z)
+
+  def e_step(mu):
+    if sig == 0:
+      return 1.0 if mu > 0 else 0.0
+    return scipy.stats.norm.cdf(mu / sig)
+
+  def phi(u):
+    return a * min(u, 0.0) + b * max(u, 0.0)
+
+  su = math.sqrt(q1)
+
+  def dens(u):
+    return scipy.stats.norm.pdf(u / su) / su
+
+  def f_nngp(u):
+    mu = beta * u
+    return phi(u) * ((b - a) * e_relu(mu) + a * mu) * dens(u)
+
+  def f_ntk(u):
+    mu = beta * u
+    du = a if u < 0 else b
+    return du * (a + (b - a) * e_step(mu)) * dens(u)
+
+  lim = 12 * su
+  kw = dict(limit=400, epsabs=1e-13, epsrel=1e-12)
+  en = sum(scipy.integrate.quad(f_nngp, lo, hi, **kw)[0]
+           for lo, hi in ((-lim, 0.0), (0.0, lim)))
+  et = sum(scipy.integrate.quad(f_ntk, lo, hi, **kw)[0]
+           for lo, hi in ((-lim, 0.0), (0.0, lim)))
+  return en, et
+
+
+@pytest.mark.parametrize('a,b', [(0.0, 1.0), (-1.0, 1.0), (0.2, 1.0),
+                                 (0.5, -0.7)])
+@pytest.mark.parametrize('rho', [-0.95, -0.4, 0.0, 0.3, 0.8, 0.999])
+def test_abrelu_matches_gaussian_integrals(a, b, rho):
+  q1, q2 = 1.7, 0.6
+  c = rho * math.sqrt(q1 * q2)
+  nngp = np.array([c])
+  prod = np.array([q1 * q2])
+  got_n, got_d = so.relu_nngp_ntk(nngp, prod, np.array([1.0]), a, b)
+  want_n, want_d = _gauss_phi_expectations(q1, q2, c, a, b)
+  assert abs(got_n[0] - want_n) <= 1e-8 * max(1.0, abs(want_n))
+  assert abs(got_d[0] - want_d) <= 1e-8
+
+
+def test_relu_closed_forms_and_special_points():
+  """Arc-cosine kernel of degree 1 (Cho & Saul 2009) at rho = 1, 0, -1 and
+  the fill_zero=pi/2 convention (elementwise.py:446, 1293-1298)."""
+  P = 2.5
+  sq = math.sqrt(P)
+  c = np.array([sq, 0.0, -sq, 0.0])
+  prod = np.array([P, P, P, 0.0])
+  n, d = so.relu_nngp_ntk(c, prod, np.ones(4))
+  np.testing.assert_allclose(n[:3], [sq / 2, sq / (2 * math.pi), 0.0],
+                             atol=1e-15)
+  np.testing.assert_allclose(d[:3], [0.5, 0.25, 0.0], atol=1e-15)
+  assert n[3] == 0.0 and d[3] == 0.25          # theta = pi/2 fill
+  # prod < nngp^2 (clamped by max(., 0)): s = 0, theta = 0 or pi.
+  n, d = so.relu_nngp_ntk(np.array([2.0, -2.0]), np.array([3.0, 3.0]),
+                          np.ones(2))
+  np.testing.assert_allclose(n, [1.0, 0.0])
+  np.testing.assert_allclose(d, [0.5, 0.0])
+
+
+def test_abrelu_identities():
+  """elementwise_test.py:537-637: ABRelu(a, a) == a^2 * Identity on the
+  nngp and ABRelu(0, 1) == Relu, on a Conv network."""
+  rng = np.random.default_rng(3)
+  x1 = rng.standard_normal((2, 4, 4, 3))
+  x2 = rng.standard_normal((3, 4, 4, 3))
+  conv = so.conv_layer((3, 3), (1, 1), 'SAME', 1.3, 0.2)
+  base = so.kernel_fn([conv], x1, x2, ('nngp', 'ntk', 'cov1'))
+  lin = so.kernel_fn([conv, so.abrelu_layer(0.7, 0.7)], x1, x2,
+                     ('nngp', 'ntk', 'cov1'))
+  np.testing.assert_allclose(lin['nngp'], 0.49 * base['nngp'], rtol=1e-12)
+  np.testing.assert_allclose(lin['cov1'], 0.49 * base['cov1'], rtol=1e-12)
+  np.testing.assert_allclose(lin['ntk'], 0.49 * base['ntk'], rtol=1e-12)
+  r1 = so.kernel_fn([conv, so.relu_layer()], x1, x2, 'nngp')
+  r2 = so.kernel_fn([conv, so.abrelu_layer(0.0, 1.0)], x1, x2, 'nngp')
+  np.testing.assert_array_equal(r1['nngp'], r2['nngp'])
+
+
+def test_relu_requires_gaussian_input():
+  x = np.ones((1, 3, 3, 2))
+  with pytest.raises(ValueError):
+    so.kernel_fn([so.relu_layer()], x, None)
+
+
+# ------------------------------------------------------------------ pools
+
+def test_avg_pool_known_answer():
+  """stax_test.py:494-539 (AvgPool SAME 2x2/1 on all-ones inputs)."""
+  X1 = np.ones((4, 2, 3, 2))
+  X2 = np.ones((3, 2, 3, 2))
+  ker = so.kernel_fn([so.avg_pool_layer((2, 2), (1, 1), 'SAME', False)], X1,
+                     X2, None)
+  ker_norm = so.kernel_fn([so.avg_pool_layer((2, 2), (1, 1), 'SAME', True)],
+                          X1, X2, None)
+  np.testing.assert_allclose(ker_norm.nngp, np.ones_like(ker_norm.nngp))
+  np.testing.assert_allclose(ker_norm.cov1, np.ones_like(ker_norm.cov1))
+  np.testing.assert_allclose(ker_norm.cov2, np.ones_like(ker_norm.cov2))
+  out_unnorm = np.array([[1., 1., 0.5], [0.5, 0.5, 0.25]])
+  ker_unnorm = np.outer(out_unnorm, out_unnorm).reshape((2, 3, 2, 3))
+  ker_unnorm = np.transpose(ker_unnorm, axes=(0, 2, 1, 3))
+  np.testing.assert_allclose(
+      ker.nngp, np.broadcast_to(ker_unnorm[None, None], ker.nngp.shape))
+  np.testing.assert_allclose(
+      ker.cov1, np.broadcast_to(ker_unnorm[None], ker.cov1.shape))
+  np.testing.assert_allclose(
+      ker.cov2, np.broadcast_to(ker_unnorm[None], ker.cov2.shape))
+
+
+def test_valid_avg_pool_is_block_mean():
+  rng = np.random.default_rng(4)
+  x = rng.standard_normal((2, 3, 4, 4, 6, 6))
+  got = so.pool_kernel(x, (2, 2, 2, 2), (2, 2, 2, 2), 'VALID', False, 2)
+  want = x.reshape(2, 3, 2, 2, 2, 2, 3, 2, 3, 2).mean(axis=(3, 5, 7, 9))
+  np.testing.assert_allclose(got, want, rtol=1e-13)
+
+
+def test_gap_flatten_readout_equivalence():
+  """Notebook readout [AvgPool x3 -> 1x1, Flatten, Dense(sqrt 2)] equals
+  [AvgPool -> 4x4, GlobalAvgPool, Dense(1)] times W^2 = 2 (both are the
+  mean over the 8x8 stage-3 grid)."""
+  rng = np.random.default_rng(5)
+  x1 = rng.standard_normal((2, 8, 8, 3))
+  x2 = rng.standard_normal((2, 8, 8, 3))
+  body = [so.conv_layer((3, 3), (1, 1), 'SAME', 2 ** 0.5, 0.0),
+          so.relu_layer()]
+  nb = body + [so.avg_pool_layer((2, 2), (2, 2), 'VALID')] * 3 + [
+      so.flatten_layer(), so.dense_layer(2 ** 0.5, 0.0)]
+  cf = body + [so.avg_pool_layer((2, 2), (2, 2), 'VALID'),
+               so.global_avg_pool_layer(), so.dense_layer(1.0, 0.0)]
+  a = so.kernel_fn(nb, x1, x2)
+  b = so.kernel_fn(cf, x1, x2)
+  np.testing.assert_allclose(a['nngp'], 2 * b['nngp'], rtol=1e-12)
+  np.testing.assert_allclose(a['ntk'], 2 * b['ntk'], rtol=1e-12)
+
+
+def test_full_vs_diagonal_spatial():
+  """requirements_test.py:87-112: the diagonal-spatial path equals the
+  spatial diagonal ('...iijj->...ij') of the full-spatial nngp."""
+  rng = np.random.default_rng(6)
+  x1 = rng.standard_normal((2, 5, 6, 3))
+  x2 = rng.standard_normal((3, 5, 6, 3))
+  layers = [so.conv_layer((2, 3), (1, 1), 'SAME', 1.1, 0.3), so.relu_layer(),
+            so.conv_layer((2, 3), (1, 1), 'SAME', 0.9, 0.1)]
+  full = so.kernel_fn(layers, x1, x2, ('nngp', 'ntk'))
+  diag = so.kernel_fn(layers, x1, x2, ('nngp', 'ntk'), diagonal_spatial=True)
+  for g in ('nngp', 'ntk'):
+    # two convs: spatial order back to (h, h', w, w')
+    np.testing.assert_allclose(diag[g],
+                               np.einsum('...iijj->...ij', full[g]),
+                               rtol=1e-12, atol=1e-14)
+
+
+# ----------------------------------------------------------- predict
+
+def test_cho_solve_matches_library_solve():
+  rng = np.random.default_rng(7)
+  A = rng.standard_normal((30, 30))
+  K = A @ A.T / 30
+  y = rng.standard_normal((30, 4))
+  Kt = rng.standard_normal((5, 30))
+  mean, alpha = po.gp_inference_mean(K, y, Kt, diag_reg=1e-3)
+  lam = 1e-3 * np.trace(K) / 30
+  want = np.linalg.solve(K + lam * np.eye(30), y)
+  np.testing.assert_allclose(alpha, want, rtol=1e-10)
+  np.testing.assert_allclose(mean, Kt @ want, rtol=1e-10)
+  mean_abs, _ = po.gp_inference_mean(K, y, Kt, diag_reg=0.05,
+                                     diag_reg_absolute_scale=True)
+  np.testing.assert_allclose(
+      mean_abs, Kt @ np.linalg.solve(K + 0.05 * np.eye(30), y), rtol=1e-10)
+
+
+def test_finite_t_limits():
+  """predict_test.py:388-484: t -> inf equals t=None; t=0 gives zero
+  test mean (zero-mean prior) and zero train change."""
+  rng = np.random.default_rng(8)
+  A = rng.standard_normal((20, 20))
+  K = A @ A.T / 20 + np.eye(20)
+  y = rng.standard_normal((20, 3))
+  Kt = rng.standard_normal((4, 20))
+  inf, _ = po.gp_inference_mean(K, y, Kt, diag_reg=1e-2)
+  big = po.ensemble_mean_finite_t(K, y, Kt, t=1e9, diag_reg=1e-2)
+  np.testing.assert_allclose(big, inf, rtol=1e-7, atol=1e-9)
+  zero = po.ensemble_mean_finite_t(K, y, Kt, t=0.0, diag_reg=1e-2)
+  np.testing.assert_allclose(zero, 0.0, atol=1e-14)
+  train_big = po.ensemble_mean_finite_t(K, y, None, t=1e9)
+  np.testing.assert_allclose(train_big, y, rtol=1e-7, atol=1e-9)

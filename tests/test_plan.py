@@ -1,0 +1,58 @@
+"""Schedule compilation on the host (no GPU)."""
+import pytest
+
+from neural_tangents_amd import _engine, stax
+
+
+def _plan(*layers, shape=(1, 32, 32, 3)):
+  return _engine.compile_plan(stax.serial(*layers)[2].specs, shape)
+
+
+def test_myrtle5_stages():
+  p = _engine.compile_plan(stax.myrtle(5)[2].specs, (1, 32, 32, 3))
+  assert [(s.first, s.conv is not None, s.relu, s.pool, s.hin, s.hout)
+          for s in p.stages] == [
+              (True, True, True, None, 32, 32),
+              (False, True, True, (2, 2), 32, 16),
+              (False, True, True, (2, 2), 16, 8),
+              (False, True, True, (2, 2), 8, 4)]
+  assert p.head == 'gap'
+  assert p.head_coef == (1.0, 0.0, 1.0, 1.0, 0.0)
+
+
+def test_myrtle10_stages():
+  p = _engine.compile_plan(stax.myrtle(10)[2].specs, (1, 32, 32, 3))
+  assert len(p.stages) == 9
+  assert [s.hout for s in p.stages] == [32, 32, 16, 16, 16, 8, 8, 8, 4]
+
+
+def test_first_stage_pool_split():
+  p = _plan(stax.Conv(1, (3, 3), padding='SAME'), stax.Relu(),
+            stax.AvgPool((2, 2), (2, 2)), stax.GlobalAvgPool())
+  assert len(p.stages) == 2 and p.stages[1].conv is None
+  assert p.stages[1].pool == (2, 2) and not p.stages[1].relu
+
+
+def test_dense_chain_coefficients():
+  p = _plan(stax.Conv(1, (3, 3), padding='SAME'), stax.GlobalAvgPool(),
+            stax.Dense(1, 2.0, 1.0), stax.Dense(1, 3.0, 0.5))
+  a, b, c, d, e = p.head_coef
+  # nngp1 = 4m + 1; ntk1 = nngp1 + 4mt
+  # nngp2 = 9 nngp1 + .25 = 36m + 9.25; ntk2 = nngp2 + 9 ntk1
+  assert (a, b) == (36.0, 9.25)
+  assert (c, d, e) == (36.0 + 36.0, 36.0, 9.25 + 9.0)
+
+
+@pytest.mark.parametrize('layers,exc', [
+    ((stax.Relu(),), ValueError),
+    ((stax.AvgPool((2, 2), (2, 2)),), NotImplementedError),
+    ((stax.Conv(1, (3, 3), (2, 2), 'SAME'),), NotImplementedError),
+    ((stax.Conv(1, (3, 3), padding='CIRCULAR'),), NotImplementedError),
+    ((stax.Conv(1, (3, 3), padding='SAME'), stax.AvgPool((3, 3), (2, 2))),
+     NotImplementedError),
+    ((stax.Conv(1, (3, 3), padding='SAME'), stax.GlobalAvgPool(),
+      stax.Relu()), NotImplementedError),
+])
+def test_rejections(layers, exc):
+  with pytest.raises(exc):
+    _plan(*layers)
