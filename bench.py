@@ -80,7 +80,7 @@ def main():
   ap.add_argument('--workload', default='myrtle5_32x32_256x256',
                   choices=sorted(WORKLOADS))
   ap.add_argument('--no-cpu', action='store_true')
-  ap.add_argument('--cpu-sample', type=int, default=6)
+  ap.add_argument('--cpu-sample', type=int, default=14)
   args = ap.parse_args()
   if args.gpus != 1:
     print('bench.py runs on one GPU only: this engine has no multi-GPU path '

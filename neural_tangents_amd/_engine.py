@@ -229,13 +229,16 @@ TRACE = None
 
 
 def _launch(name, alg_bytes, pairs, *args):
+  """Calls entry point ``name`` (a trace label may follow it in
+  brackets)."""
+  entry = name.split('[')[0]
   if TRACE is None:
-    _lib.call(name, *args)
+    _lib.call(entry, *args)
     return
   s = torch.cuda.Event(enable_timing=True)
   e = torch.cuda.Event(enable_timing=True)
   s.record()
-  _lib.call(name, *args)
+  _lib.call(entry, *args)
   e.record()
   TRACE.append((name, s, e, alg_bytes, pairs))
 
@@ -299,7 +302,8 @@ def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
                       dtype=torch.float32, device=dev)
     out_t = torch.empty_like(out) if fl_ntk else None
     nt = 2 if fl_ntk else 1
-    _launch('nta_stage', T * nt * 4 * (nngp[0].numel() + out[0].numel()), T,
+    _launch(f'nta_stage[{s.hin}->{s.hout}]',
+            T * nt * 4 * (nngp[0].numel() + out[0].numel()), T,
             _lib.ptr(nngp), _lib.ptr(ntk), n1, n2, s.hin, s.win, fh, fw, ph,
             pw, flags, w_var, b_var, _lib.ptr(q1), _lib.ptr(q2),
             _lib.ptr(out), _lib.ptr(out_t), stream)

@@ -1,0 +1,73 @@
+"""Cholesky solves of the regularised train kernel for ``predict``.
+
+``_get_cho_solve`` / ``_add_diagonal_regularizer`` (``predict.py:1224-1258``)
+on the GPU.  ``mode='fp64'`` is the plain FP64 Cholesky solve;
+``mode='mixed'`` factors in FP32 with the hand-written MFMA Cholesky of
+libnta.so and refines the solution with FP64 residuals (``nta_linalg``).
+"""
+import torch
+
+from . import _lib
+
+
+def add_diagonal_regularizer(A: torch.Tensor, diag_reg: float,
+                             diag_reg_absolute_scale: bool) -> torch.Tensor:
+  """``predict.py:1224-1232`` (returns a new tensor)."""
+  n = A.shape[0]
+  if not diag_reg_absolute_scale:
+    diag_reg = diag_reg * torch.trace(A.to(torch.float64)) / n
+  out = A.clone()
+  out.diagonal().add_(diag_reg)
+  return out
+
+
+def regularizer_value(A: torch.Tensor, diag_reg: float,
+                      diag_reg_absolute_scale: bool) -> float:
+  """The FP64 shift lambda added to the diagonal."""
+  if diag_reg_absolute_scale:
+    return float(diag_reg)
+  return float(diag_reg) * float(torch.trace(A.to(torch.float64))) / A.shape[0]
+
+
+class CholeskySolver:
+  """Factor once, then ``solve(b)`` for ``b`` of shape ``[N, K]``."""
+
+  def __init__(self, A: torch.Tensor, diag_reg: float,
+               diag_reg_absolute_scale: bool, mode: str = 'mixed',
+               tol: float = 1e-12, max_iter: int = 200):
+    if A.ndim != 2 or A.shape[0] != A.shape[1]:
+      raise ValueError(f'k_train_train must be square, got {tuple(A.shape)}')
+    if mode not in ('mixed', 'fp64'):
+      raise ValueError(f'Unknown solver {mode!r}; use "mixed" or "fp64".')
+    self.mode = mode
+    self.device = A.device
+    self.n = A.shape[0]
+    self.lam = regularizer_value(A, diag_reg, diag_reg_absolute_scale)
+    self.iterations = 0
+    self.last_residual = None
+    if mode == 'fp64':
+      A64 = A.to(torch.float64).clone()
+      A64.diagonal().add_(self.lam)
+      L, info = torch.linalg.cholesky_ex(A64)
+      if int(info) != 0:
+        raise ValueError(
+            f'Cholesky factorisation failed (leading minor {int(info)} is '
+            'not positive definite); increase diag_reg.')
+      self.L64 = L
+    else:
+      from . import _mixed
+      self.impl = _mixed.MixedCholesky(A, self.lam, tol=tol,
+                                       max_iter=max_iter)
+
+  def solve(self, b: torch.Tensor) -> torch.Tensor:
+    b = b.to(self.device, torch.float64)
+    squeeze = b.ndim == 1
+    if squeeze:
+      b = b[:, None]
+    if self.mode == 'fp64':
+      x = torch.cholesky_solve(b, self.L64)
+    else:
+      x = self.impl.solve(b)
+      self.iterations = self.impl.iterations
+      self.last_residual = self.impl.last_residual
+    return x[:, 0] if squeeze else x

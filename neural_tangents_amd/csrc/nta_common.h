@@ -15,6 +15,29 @@ constexpr float kInv2Pi = 0.15915494309189533577f;
 int set_error(int code, const char* fmt, ...);
 int check_launch(const char* what);
 
+// Shape-specialised launchers (nta_fused.hip).  They return kNotHandled when
+// the shape is outside their specialisation; the caller then launches the
+// generic kernel.
+constexpr int kNotHandled = 1 << 30;
+int launch_stage_fast(const float* in_nngp, const float* in_ntk, int n1,
+                      int n2, int S, int pool, int flags, float w_var,
+                      float b_var, const float* q1, const float* q2,
+                      float* out_nngp, float* out_ntk, hipStream_t stream);
+int launch_first_fast(const float* x1, const float* x2, int n1, int n2, int S,
+                      int C, int flags, float w_var, float b_var,
+                      const float* q1, const float* q2, float* nngp,
+                      float* ntk, hipStream_t stream);
+
+// One-instruction square root on the device (v_sqrt_f32, ~1 ulp); the host
+// pass uses the C library so the same code can be unit-tested on the CPU.
+__host__ __device__ __forceinline__ float fsqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
+
 // q1*q2 - c*c with one rounding (Kahan's product-difference), so that pairs
 // with q1*q2 == c*c in exact arithmetic give exactly 0.
 __device__ __forceinline__ float prod_minus_sq(float q1, float q2, float c) {
@@ -35,6 +58,41 @@ __device__ __forceinline__ float relu_map(float c, float q1, float q2,
   float pm = kPi - theta;
   *dot_sigma = pm * kInv2Pi;
   return fmaf(pm, c, s) * kInv2Pi;
+}
+
+// Normalised form of the same map, used by the fused kernels.  With
+// sq = sqrt(q) and rq = 1/sqrt(q) (0 when q == 0) per position,
+//   rho = c rq1 rq2, theta = arccos(rho) (rho clamped to [-1, 1]),
+//   nngp' = sq1 sq2 (sqrt(1 - rho^2) + (pi - theta) rho) / (2 pi),
+//   dot_sigma = (pi - theta) / (2 pi).
+// arccos(a), a in [0, 1], is Abramowitz & Stegun 4.4.46:
+// sqrt(1 - a) * (a0 + a1 a + ... + a7 a^7), |error| <= 2e-8 (u = 1 - a).
+// q == 0 gives rho = 0, theta = pi/2 and nngp' = 0, the reference's
+// fill_zero convention (elementwise.py:446).
+__host__ __device__ __forceinline__ float acos_pos(float a, float u) {
+  float r = fsqrt(u);
+  float p = -0.0012624911f;
+  p = fmaf(p, a, 0.0066700901f);
+  p = fmaf(p, a, -0.0170881256f);
+  p = fmaf(p, a, 0.0308918810f);
+  p = fmaf(p, a, -0.0501743046f);
+  p = fmaf(p, a, 0.0889789874f);
+  p = fmaf(p, a, -0.2145988016f);
+  p = fmaf(p, a, 1.5707963050f);
+  return r * p;
+}
+
+__host__ __device__ __forceinline__ float relu_map_norm(float c, float sqp,
+                                                        float rqp,
+                                                        float* dot_sigma) {
+  float rho = c * rqp;
+  float a = fminf(fabsf(rho), 1.0f);
+  float u = 1.0f - a;
+  float th = acos_pos(a, u);                  // arccos(|rho|)
+  float pm = rho < 0.0f ? th : kPi - th;      // pi - arccos(rho)
+  float sn = fsqrt(u * (2.0f - u));           // sqrt(1 - rho^2)
+  *dot_sigma = pm * kInv2Pi;
+  return fmaf(pm, rho, sn) * (sqp * kInv2Pi);
 }
 
 }  // namespace nta

@@ -354,6 +354,12 @@ extern "C" int nta_first_stage(const float* x1, const float* x2, int n1,
       ((flags & NTA_RELU) && (!q1 || (n2 > 0 && !q2))))
     return set_error(NTA_EINVAL, "nta_first_stage: null pointer for a "
                      "requested operand");
+  if ((flags & NTA_RELU) && !(flags & NTA_CONV_VALID) && fh == 3 &&
+      fw == 3 && H == W) {
+    int rc2 = launch_first_fast(x1, x2, n1, n2, H, C, flags, w_var, b_var, q1,
+                                q2, nngp, ntk, (hipStream_t)stream);
+    if (rc2 != kNotHandled) return rc2;
+  }
   StencilGeom g = make_geom(H, W, fh, fw, flags);
   float scale = w_var / (float)(fh * fw * C);
   uint64_t T = n2 == 0 ? (uint64_t)n1 : (uint64_t)n1 * n2;
@@ -408,6 +414,13 @@ extern "C" int nta_stage(const float* in_nngp, const float* in_ntk, int n1,
     return set_error(NTA_EINVAL, "nta_stage: pool window %dx%d larger than "
                      "%dx%d", ph, pw, Hc, Wc);
   int Ho = Hc / ph, Wo = Wc / pw;
+  if ((flags & NTA_CONV) && (flags & NTA_RELU) && !(flags & NTA_CONV_VALID) &&
+      fh == 3 && fw == 3 && H == W && ph == pw) {
+    int rc2 = launch_stage_fast(in_nngp, in_ntk, n1, n2, H, ph, flags, w_var,
+                                b_var, q1, q2, out_nngp, out_ntk,
+                                (hipStream_t)stream);
+    if (rc2 != kNotHandled) return rc2;
+  }
   float scale = 1.0f / (float)(fh * fw);
   uint64_t T = n2 == 0 ? (uint64_t)n1 : (uint64_t)n1 * n2;
   uint64_t total = T * Ho * Ho * Wo * Wo;

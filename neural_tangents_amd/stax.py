@@ -303,9 +303,12 @@ class AnalyticKernelFn:
 
     if x2t is None:
       x1_is_x2 = True
+    elif x1.shape != x2t.shape:
+      x1_is_x2 = False
     else:
-      x1_is_x2 = (x1.shape == x2t.shape and
-                  bool(torch.all(torch.abs(x1 - x2t) < 1e-12)))
+      # a 0-d device tensor, like the reference's jnp bool (utils.py:255-282);
+      # evaluated without a host synchronisation.
+      x1_is_x2 = torch.all(torch.abs(x1 - x2t) < 1e-12)
     shape1 = self._out_shape(tuple(x1.shape))
     shape2 = self._out_shape(tuple((x1 if x2t is None else x2t).shape))
     k = Kernel(nngp=order(res['nngp'], 2), ntk=order(res['ntk'], 2),

@@ -176,3 +176,50 @@ def test_errors_match_reference_types():
   with pytest.raises(ValueError):
     stax.serial(stax.Conv(1, (3, 3), padding='SAME'),
                 stax.GlobalAvgPool())[2](x, x[:, :4], 'nngp')
+
+
+def test_cfg2_block_32x32():
+  """Myrtle-5 32x32 (the benchmark workload's shape): a 2x3 block against
+  the FP64 oracle, plus the same-input kernel of 2 images."""
+  x1, x2 = blurred_images([256, 256], 32, seed=0)
+  kfn, ol = _myrtle_pair((2, 1, 1))
+  a, b = x1[100:102], x2[7:10]
+  k = kfn(a, b, ('nngp', 'ntk'))
+  ref = so.kernel_fn(ol, a, b, ('nngp', 'ntk'))
+  _check(_np(k.nngp), ref['nngp'], name='nngp')
+  _check(_np(k.ntk), ref['ntk'], name='ntk')
+  ks = kfn(a[:2], None, ('nngp', 'ntk'))
+  refs = so.kernel_fn(ol, a[:2], None, ('nngp', 'ntk'))
+  _check(_np(ks.nngp), refs['nngp'], name='nngp self')
+  _check(_np(ks.ntk), refs['ntk'], name='ntk self')
+
+
+def test_myrtle10_32x32_block():
+  x1, x2 = blurred_images([2, 2], 32, seed=11)
+  kfn, ol = _myrtle_pair((3, 3, 3))
+  k = kfn(x1[:1], x2, ('nngp', 'ntk'))
+  ref = so.kernel_fn(ol, x1[:1], x2, ('nngp', 'ntk'))
+  _check(_np(k.nngp), ref['nngp'], name='nngp')
+  _check(_np(k.ntk), ref['ntk'], name='ntk')
+
+
+@pytest.mark.parametrize('side', [8, 16, 32])
+def test_single_stage_full_spatial(side):
+  """One Conv+Relu+Conv+Relu(+pool) network without readout: the
+  full-spatial state of every specialised stage kernel against the oracle,
+  with a bias and an odd number of pairs (tail workgroups)."""
+  x1, x2 = blurred_images([3, 1], side, seed=side + 1)
+  for pool in (False, True):
+    layers = [stax.Conv(1, (3, 3), padding='SAME', W_std=1.2, b_std=0.1),
+              stax.Relu(),
+              stax.Conv(1, (3, 3), padding='SAME', W_std=1.4, b_std=0.2),
+              stax.Relu()]
+    ol = [so.conv_layer((3, 3), (1, 1), 'SAME', 1.2, 0.1), so.relu_layer(),
+          so.conv_layer((3, 3), (1, 1), 'SAME', 1.4, 0.2), so.relu_layer()]
+    if pool:
+      layers.append(stax.AvgPool((2, 2), (2, 2)))
+      ol.append(so.avg_pool_layer((2, 2), (2, 2), 'VALID'))
+    k = stax.serial(*layers)[2](x1, x2, ('nngp', 'ntk', 'cov1', 'cov2'))
+    ref = so.kernel_fn(ol, x1, x2, ('nngp', 'ntk', 'cov1', 'cov2'))
+    for f in ('nngp', 'ntk', 'cov1', 'cov2'):
+      _check(_np(getattr(k, f)), ref[f], name=f'{f} pool={pool}')
