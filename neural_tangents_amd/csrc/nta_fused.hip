@@ -35,57 +35,84 @@ __device__ __forceinline__ int xcd_remap(int b, int nblk) {
   return (b & 7) * (nblk >> 3) + (b >> 3);
 }
 
-// (w, w')-diagonal 3-tap sum of K planes.  Plane k is (hh[k], hp[k]) of the
-// pair state at base[k]; planes outside [0, S)^2 are zero.  Each thread owns
-// the 2x2 block (2W.., 2Wp..).  ws[k][e], e = 2c + d, is element
-// (2W + c, 2Wp + d).  Two barriers; every thread of the block must call.
+// Buffer resource over [base, base + bytes): loads at offsets >= bytes
+// return 0, which zero-fills planes outside the grid without branches.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const float* base,
+                                                            unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), 0,
+                                           (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float2 buf_load2(__amdgpu_buffer_rsrc_t r,
+                                            unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(
+      float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+constexpr unsigned kOutside = 0x40000000u;   // byte offset past any state
+
+// ReLU contribution of one pre-activation element, accumulated for the
+// pool: acc_n += sq1 sq2 (sqrt(1 - rho^2) + (pi - theta) rho) and
+// acc_t += (pi - theta) t; both sums are scaled by 1 / (2 pi) by the caller.
+template <bool NTK>
+__device__ __forceinline__ void relu_acc(float cv, float sqp, float rqp,
+                                         float t, float& acc_n,
+                                         float& acc_t) {
+  const float rho = cv * rqp;
+  const float a = fminf(fabsf(rho), 1.0f);
+  const float u = 1.0f - a;
+  const float th = acos_pos(a, u);
+  const float pm = rho < 0.0f ? th : kPi - th;
+  const float sn = fsqrt(u * (2.0f - u));
+  acc_n = fmaf(fmaf(pm, rho, sn), sqp, acc_n);
+  if constexpr (NTK) acc_t = fmaf(pm, t, acc_t);
+}
+
+// (w, w')-diagonal 3-tap sum of K planes of one tensor.  Plane k is
+// (hh[k], hp[k]); planes outside [0, S)^2 (and every plane of a tail pair,
+// whose resource has size 0) read as zero.  Each thread owns the 2x2 block
+// (2W.., 2Wp..); ws[k][e], e = 2c + d, is element (2W + c, 2Wp + d).  Two
+// barriers; every thread of the block calls it the same number of times.
 template <int S, int K>
-__device__ __forceinline__ void wpass(const float* const* base,
+__device__ __forceinline__ void wpass(__amdgpu_buffer_rsrc_t rs,
                                       const int* hh, const int* hp,
-                                      float* __restrict__ lds, int W, int Wp,
-                                      bool valid, float (&ws)[K][4]) {
+                                      float* __restrict__ lds, unsigned tofs,
+                                      int lofs, float (&ws)[K][4]) {
   constexpr int LS = S + 2;
   constexpr int PL = LS * LS;
-  bool ok[K];
+  float2 r0[K], r1[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    ok[k] = (unsigned)hh[k] < (unsigned)S && (unsigned)hp[k] < (unsigned)S;
-    ws[k][0] = ws[k][1] = ws[k][2] = ws[k][3] = 0.0f;
-    if (ok[k]) {
-      float2 r0 = make_float2(0.0f, 0.0f), r1 = r0;
-      if (valid) {
-        const float* pl = base[k] + ((long long)hh[k] * S + hp[k]) * (S * S);
-        r0 = *reinterpret_cast<const float2*>(pl + (2 * W) * S + 2 * Wp);
-        r1 = *reinterpret_cast<const float2*>(pl + (2 * W + 1) * S + 2 * Wp);
-      }
-      float* d = lds + k * PL + (2 * W + 1) * LS + 2 * Wp + 1;
-      d[0] = r0.x;
-      d[1] = r0.y;
-      d[LS] = r1.x;
-      d[LS + 1] = r1.y;
-      ws[k][0] = r0.x;
-      ws[k][1] = r0.y;
-      ws[k][2] = r1.x;
-      ws[k][3] = r1.y;
-    }
+    const bool ok = (unsigned)hh[k] < (unsigned)S &&
+                    (unsigned)hp[k] < (unsigned)S;
+    const unsigned soff = ok ? (unsigned)((hh[k] * S + hp[k]) * S * S) * 4u
+                             : kOutside;
+    r0[k] = buf_load2(rs, tofs, soff);
+    r1[k] = buf_load2(rs, tofs + S * 4, soff);
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    float* d = lds + k * PL + lofs;
+    d[0] = r0[k].x;
+    d[1] = r0[k].y;
+    d[LS] = r1[k].x;
+    d[LS + 1] = r1[k].y;
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    if (ok[k]) {
-      const float* s0 = lds + k * PL + (2 * W + 1) * LS + 2 * Wp + 1;
-      float v00 = ws[k][0], v01 = ws[k][1], v10 = ws[k][2], v11 = ws[k][3];
-      float a_mm = s0[-LS - 1];       // (2W-1, 2Wp-1)
-      float a_m0 = s0[-LS];           // (2W-1, 2Wp)
-      float a_0m = s0[-1];            // (2W,   2Wp-1)
-      float a_12 = s0[LS + 2];        // (2W+1, 2Wp+2)
-      float a_21 = s0[2 * LS + 1];    // (2W+2, 2Wp+1)
-      float a_22 = s0[2 * LS + 2];    // (2W+2, 2Wp+2)
-      ws[k][0] = a_mm + v00 + v11;
-      ws[k][1] = a_m0 + v01 + a_12;
-      ws[k][2] = a_0m + v10 + a_21;
-      ws[k][3] = v00 + v11 + a_22;
-    }
+    const float* s0 = lds + k * PL + lofs;
+    const float v00 = r0[k].x, v01 = r0[k].y, v10 = r1[k].x, v11 = r1[k].y;
+    const float a_mm = s0[-LS - 1];       // (2W-1, 2Wp-1)
+    const float a_m0 = s0[-LS];           // (2W-1, 2Wp)
+    const float a_0m = s0[-1];            // (2W,   2Wp-1)
+    const float a_12 = s0[LS + 2];        // (2W+1, 2Wp+2)
+    const float a_21 = s0[2 * LS + 1];    // (2W+2, 2Wp+1)
+    const float a_22 = s0[2 * LS + 2];    // (2W+2, 2Wp+2)
+    ws[k][0] = a_mm + v00 + v11;
+    ws[k][1] = a_m0 + v01 + a_12;
+    ws[k][2] = a_0m + v10 + a_21;
+    ws[k][3] = v00 + v11 + a_22;
   }
   __syncthreads();
 }
@@ -102,7 +129,7 @@ __global__ __launch_bounds__(256) void k_stage_fast(
   constexpr int LS = S + 2;
   constexpr int PL = LS * LS;
   constexpr int NT = NTK ? 2 : 1;
-  constexpr int NB = POOL == 2 ? 6 : NT;  // LDS planes per exchange
+  constexpr int NB = POOL == 2 ? 6 : 1;   // LDS planes per exchange
   constexpr int SO = S / POOL;
   constexpr long long PER_IN = (long long)S * S * S * S;
   constexpr long long PER_OUT = (long long)SO * SO * SO * SO;
@@ -138,8 +165,20 @@ __global__ __launch_bounds__(256) void k_stage_fast(
   }
   __syncthreads();
   float* L0 = lds + job * NB * PL;
-  const float* bn = in_n + pc * PER_IN;
-  const float* bt = NTK ? in_t + pc * PER_IN : in_n;
+  // One resource per workgroup over its JPW pairs (uniform, so the loads
+  // take it from SGPRs); pairs past the tile end fall outside its size.
+  const long long p0 = (long long)g * JPW;
+  const long long left = T - p0;
+  const unsigned nbytes =
+      (unsigned)((left < JPW ? left : JPW) * PER_IN * 4);
+  const __amdgpu_buffer_rsrc_t rn = make_rsrc(in_n + p0 * PER_IN, nbytes);
+  const __amdgpu_buffer_rsrc_t rt =
+      make_rsrc((NTK ? in_t : in_n) + p0 * PER_IN, nbytes);
+  const unsigned tofs =
+      (unsigned)(job * PER_IN + (2 * W) * S + 2 * Wp) * 4u;
+  const int lofs = (2 * W + 1) * LS + 2 * Wp + 1;
+  const float4* lq1v = reinterpret_cast<const float4*>(lq1);
+  const float4* lq2v = reinterpret_cast<const float4*>(lq2);
 
   if constexpr (POOL == 2) {
     const int D = di - (H2 - 1);
@@ -153,10 +192,8 @@ __global__ __launch_bounds__(256) void k_stage_fast(
       const int hp[4] = {h0 - 1 - d0, h0 - d0, h0 - dp, h0 - 1 - dm};
 #pragma unroll
       for (int tt = 0; tt < NT; ++tt) {
-        const float* b0 = tt ? bt : bn;
-        const float* const base[4] = {b0, b0, b0, b0};
         float ws[4][4];
-        wpass<S, 4>(base, hh, hp, L0, W, Wp, valid, ws);
+        wpass<S, 4>(tt ? rt : rn, hh, hp, L0, tofs, lofs, ws);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           c0a[tt][e] = ws[0][e];
@@ -175,10 +212,8 @@ __global__ __launch_bounds__(256) void k_stage_fast(
       float pre[NT][4][4];   // [tensor][P00, P11, P10, P01][element]
 #pragma unroll
       for (int tt = 0; tt < NT; ++tt) {
-        const float* b0 = tt ? bt : bn;
-        const float* const base[6] = {b0, b0, b0, b0, b0, b0};
         float ws[6][4];
-        wpass<S, 6>(base, hh, hp, L0, W, Wp, valid, ws);
+        wpass<S, 6>(tt ? rt : rn, hh, hp, L0, tofs, lofs, ws);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           pre[tt][0][e] = c0a[tt][e] + c0b[tt][e] + ws[0][e];
@@ -191,26 +226,34 @@ __global__ __launch_bounds__(256) void k_stage_fast(
           cm[tt][e] = ws[5][e];
         }
       }
-      const int ra[4] = {2 * H, 2 * H + 1, 2 * H + 1, 2 * H};
-      const int rb[4] = {2 * Hp, 2 * Hp + 1, 2 * Hp, 2 * Hp + 1};
+      // (sq, rq) at rows 2H, 2H+1 of image i and 2Hp, 2Hp+1 of image j,
+      // columns (2W, 2W+1) and (2Wp, 2Wp+1).
+      const float4 qa0 = lq1v[((2 * H) * S + 2 * W) >> 1];
+      const float4 qa1 = lq1v[((2 * H + 1) * S + 2 * W) >> 1];
+      const float4 qb0 = lq2v[((2 * Hp) * S + 2 * Wp) >> 1];
+      const float4 qb1 = lq2v[((2 * Hp + 1) * S + 2 * Wp) >> 1];
+      // plane -> (row of i, row of j): P00 (0,0) P11 (1,1) P10 (1,0) P01 (0,1)
+      const float4 qa_of[4] = {qa0, qa1, qa1, qa0};
+      const float4 qb_of[4] = {qb0, qb1, qb0, qb1};
       float acc_n = 0.0f, acc_t = 0.0f;
 #pragma unroll
       for (int pl = 0; pl < 4; ++pl) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const float2 qa = lq1[ra[pl] * S + 2 * W + (e >> 1)];
-          const float2 qb = lq2[rb[pl] * S + 2 * Wp + (e & 1)];
+          const float4 A = qa_of[pl], B = qb_of[pl];
+          const float sa = (e >> 1) ? A.z : A.x, ra = (e >> 1) ? A.w : A.y;
+          const float sb = (e & 1) ? B.z : B.x, rb = (e & 1) ? B.w : B.y;
           const float cv = fmaf(wv9, pre[0][pl][e], b_var);
-          float ds;
-          acc_n += relu_map_norm(cv, qa.x * qb.x, qa.y * qb.y, &ds);
-          if constexpr (NTK) acc_t += fmaf(wv9, pre[NT - 1][pl][e], cv) * ds;
+          const float tv = NTK ? fmaf(wv9, pre[NT - 1][pl][e], cv) : 0.0f;
+          relu_acc<NTK>(cv, sa * sb, ra * rb, tv, acc_n, acc_t);
         }
       }
       if (valid) {
         const long long o =
             p * PER_OUT + ((long long)(H * SO + Hp) * SO + W) * SO + Wp;
-        out_n[o] = acc_n * 0.0625f;
-        if constexpr (NTK) out_t[o] = acc_t * 0.0625f;
+        constexpr float kScale = kInv2Pi / 16.0f;
+        out_n[o] = acc_n * kScale;
+        if constexpr (NTK) out_t[o] = acc_t * kScale;
       }
     }
   } else {
@@ -219,54 +262,48 @@ __global__ __launch_bounds__(256) void k_stage_fast(
     const int he = d < 0 ? S - 1 + d : S - 1;
     float cA[NT][4], cB[NT][4];
     {
-      int hh[NT], hp[NT];
-      const float* base[NT];
+      const int hh[1] = {hs};
+      const int hp[1] = {hs - d};
 #pragma unroll
       for (int tt = 0; tt < NT; ++tt) {
-        hh[tt] = hs;
-        hp[tt] = hs - d;
-        base[tt] = tt ? bt : bn;
-      }
-      float ws[NT][4];
-      wpass<S, NT>(base, hh, hp, L0, W, Wp, valid, ws);
-#pragma unroll
-      for (int tt = 0; tt < NT; ++tt)
+        float ws[1][4];
+        wpass<S, 1>(tt ? rt : rn, hh, hp, L0, tofs, lofs, ws);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           cA[tt][e] = 0.0f;        // plane (hs-1, hs-1-d) is always outside
-          cB[tt][e] = ws[tt][e];
+          cB[tt][e] = ws[0][e];
         }
+      }
     }
     for (int h = hs; h <= he; ++h) {
       const int hq = h - d;
-      int hh[NT], hp[NT];
-      const float* base[NT];
-#pragma unroll
-      for (int tt = 0; tt < NT; ++tt) {
-        hh[tt] = h + 1;
-        hp[tt] = h + 1 - d;
-        base[tt] = tt ? bt : bn;
-      }
-      float ws[NT][4];
-      wpass<S, NT>(base, hh, hp, L0, W, Wp, valid, ws);
+      const int hh[1] = {h + 1};
+      const int hp[1] = {h + 1 - d};
       float pre[NT][4];
 #pragma unroll
-      for (int tt = 0; tt < NT; ++tt)
+      for (int tt = 0; tt < NT; ++tt) {
+        float ws[1][4];
+        wpass<S, 1>(tt ? rt : rn, hh, hp, L0, tofs, lofs, ws);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          pre[tt][e] = cA[tt][e] + cB[tt][e] + ws[tt][e];
+          pre[tt][e] = cA[tt][e] + cB[tt][e] + ws[0][e];
           cA[tt][e] = cB[tt][e];
-          cB[tt][e] = ws[tt][e];
+          cB[tt][e] = ws[0][e];
         }
+      }
+      const float4 A = lq1v[(h * S + 2 * W) >> 1];
+      const float4 B = lq2v[(hq * S + 2 * Wp) >> 1];
       float vn[4], vt[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const float2 qa = lq1[h * S + 2 * W + (e >> 1)];
-        const float2 qb = lq2[hq * S + 2 * Wp + (e & 1)];
+        const float sa = (e >> 1) ? A.z : A.x, ra = (e >> 1) ? A.w : A.y;
+        const float sb = (e & 1) ? B.z : B.x, rb = (e & 1) ? B.w : B.y;
         const float cv = fmaf(wv9, pre[0][e], b_var);
-        float ds;
-        vn[e] = relu_map_norm(cv, qa.x * qb.x, qa.y * qb.y, &ds);
-        vt[e] = NTK ? fmaf(wv9, pre[NT - 1][e], cv) * ds : 0.0f;
+        const float tv = NTK ? fmaf(wv9, pre[NT - 1][e], cv) : 0.0f;
+        float an = 0.0f, at = 0.0f;
+        relu_acc<NTK>(cv, sa * sb, ra * rb, tv, an, at);
+        vn[e] = an * kInv2Pi;
+        vt[e] = at * kInv2Pi;
       }
       if (valid) {
         const long long o = p * PER_OUT + ((long long)(h * S + hq) * S +
@@ -306,6 +343,28 @@ __device__ __forceinline__ void load_padded(const float* __restrict__ x,
 }
 
 template <bool NTK>
+__device__ __forceinline__ void first32_epilogue(
+    const floatx16& acc, const float (&sq1r)[16], const float (&rq1r)[16],
+    float2 qb, float scale, float b_var, float* __restrict__ on,
+    float* __restrict__ ot, int half, int col) {
+  constexpr int S = 32;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int m = 8 * (r >> 2) + 4 * half + (r & 3);
+    const float cv = fmaf(scale, acc[r], b_var);
+    const float rho = cv * (rq1r[r] * qb.y);
+    const float a = fminf(fabsf(rho), 1.0f);
+    const float u = 1.0f - a;
+    const float th = acos_pos(a, u);
+    const float pm = rho < 0.0f ? th : kPi - th;
+    const float sn = fsqrt(u * (2.0f - u));
+    // sq1r already carries the 1 / (2 pi) factor.
+    on[m * S + col] = fmaf(pm, rho, sn) * (sq1r[r] * qb.x);
+    if constexpr (NTK) ot[m * S + col] = (cv * kInv2Pi) * pm;
+  }
+}
+
+template <bool NTK>
 __global__ __launch_bounds__(256) void k_first32(
     const float* __restrict__ x1, const float* __restrict__ x2, int n1,
     int n2, int C, int nblk, float scale, float b_var,
@@ -313,10 +372,12 @@ __global__ __launch_bounds__(256) void k_first32(
     float* __restrict__ out_n, float* __restrict__ out_t) {
   constexpr int S = 32, SP = S + 2, PLN = SP * SP;
   constexpr long long PER = (long long)S * S * S * S;
+  // xp1 is dead once the A operand is in registers; its space then holds
+  // the (sq, rq) normalisers of image j.
   __shared__ float xp1[4 * PLN];
   __shared__ float xp2[4 * PLN];
-  __shared__ float2 lq2[S * S];
   __shared__ float2 lq1[S];
+  float2* lq2 = reinterpret_cast<float2*>(xp1);
   const bool self = n2 == 0;
   const long long T = self ? (long long)n1 : (long long)n1 * n2;
   const int L = xcd_remap(blockIdx.x, nblk);
@@ -329,10 +390,10 @@ __global__ __launch_bounds__(256) void k_first32(
   const float* qbp = self ? q1 : q2;
   load_padded<S>(x1, i, C, true, xp1);
   load_padded<S>(xb, j, C, true, xp2);
-  for (int k = threadIdx.x; k < S * S; k += 256)
-    lq2[k] = qnorm(qbp[j * (S * S) + k]);
-  if (threadIdx.x < S) lq1[threadIdx.x] = qnorm(q1[(i * S + h) * S +
-                                                   threadIdx.x]);
+  if (threadIdx.x < S) {
+    float2 v = qnorm(q1[(i * S + h) * S + threadIdx.x]);
+    lq1[threadIdx.x] = make_float2(v.x * kInv2Pi, v.y);
+  }
   __syncthreads();
 
   const int lane = threadIdx.x & 63;
@@ -342,11 +403,13 @@ __global__ __launch_bounds__(256) void k_first32(
   // K = 9 taps x 4 channels; step kk covers tap kk >> 1 and channels
   // 2 (kk & 1) + {0, 1} (lane half).
   float a[18];
-  const float* pa = xp1 + half * PLN + h * SP + col;
+  {
+    const float* pa = xp1 + half * PLN + h * SP + col;
 #pragma unroll
-  for (int kk = 0; kk < 18; ++kk) {
-    const int tap = kk >> 1;
-    a[kk] = pa[2 * (kk & 1) * PLN + (tap / 3) * SP + tap % 3];
+    for (int kk = 0; kk < 18; ++kk) {
+      const int tap = kk >> 1;
+      a[kk] = pa[2 * (kk & 1) * PLN + (tap / 3) * SP + tap % 3];
+    }
   }
   float sq1r[16], rq1r[16];
 #pragma unroll
@@ -355,30 +418,38 @@ __global__ __launch_bounds__(256) void k_first32(
     sq1r[r] = lq1[m].x;
     rq1r[r] = lq1[m].y;
   }
-  for (int hq = wave; hq < S; hq += 4) {
+  __syncthreads();                      // xp1 reads done: reuse as lq2
+  for (int k = threadIdx.x; k < S * S; k += 256)
+    lq2[k] = qnorm(qbp[j * (S * S) + k]);
+  __syncthreads();
+
+  const float* pb0 = xp2 + half * PLN + col;
+  auto gram = [&](int hq) {
     floatx16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-    const float* pb = xp2 + half * PLN + hq * SP + col;
+    const float* pb = pb0 + hq * SP;
 #pragma unroll
     for (int kk = 0; kk < 18; ++kk) {
       const int tap = kk >> 1;
       const float bv = pb[2 * (kk & 1) * PLN + (tap / 3) * SP + tap % 3];
       acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[kk], bv, acc, 0, 0, 0);
     }
+    return acc;
+  };
+  // Planes hq = wave, wave + 4, ...; the Gram of the next plane is issued
+  // before the ReLU epilogue of the current one so MFMA and VALU overlap.
+  floatx16 cur = gram(wave);
+  for (int hq = wave; hq < S; hq += 4) {
+    floatx16 nxt;
+    if (hq + 4 < S) nxt = gram(hq + 4);
     const float2 qb = lq2[hq * S + col];
     float* on = out_n + p * PER + (long long)(h * S + hq) * (S * S);
     float* ot = NTK ? out_t + p * PER + (long long)(h * S + hq) * (S * S)
                     : nullptr;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int m = 8 * (r >> 2) + 4 * half + (r & 3);
-      const float cv = fmaf(scale, acc[r], b_var);
-      float ds;
-      const float v = relu_map_norm(cv, sq1r[r] * qb.x, rq1r[r] * qb.y, &ds);
-      on[m * S + col] = v;
-      if constexpr (NTK) ot[m * S + col] = cv * ds;
-    }
+    first32_epilogue<NTK>(cur, sq1r, rq1r, qb, scale, b_var, on, ot, half,
+                          col);
+    cur = nxt;
   }
 }
 
