@@ -118,6 +118,44 @@ int nta_head(const float* nngp, const float* ntk, int n1, int n2, int H,
              double e, float* out_nngp, float* out_ntk, long long ld,
              void* stream);
 
+/* ------------------------------------------------------------------------
+ * Kernel-regression solve (predict.py:1235-1258, cho_factor / cho_solve of
+ * K + lambda I, mixed precision).  Row-major matrices, element (i, j) at
+ * [i * ld + j].
+ */
+
+/*
+ * In-place FP32 Cholesky of the lower triangle of A (n x n, n a multiple of
+ * 64; upper triangle zeroed), with the trailing updates on fp32 MFMA, and
+ * W = L^-1 (lower, n x n).  *info (device int, zero on entry) receives the
+ * 1-based index of the first non-positive pivot.
+ * Replaces: jsp.linalg.cho_factor (predict.py:1244).
+ */
+int nta_chol_f32(float* A, int n, long long lda, float* W, long long ldw,
+                 int* info, void* stream);
+
+/*
+ * Y = (A + lam I) X in FP64: A fp32 (upcast exactly), X, Y fp64 [n][nrhs],
+ * nrhs <= 16.  The FP64 residual of the refinement.
+ * Replaces: the regularised operand of cho_solve (predict.py:1224-1232,
+ * 1254).
+ */
+int nta_symv64(const float* A, int n, long long lda, double lam,
+               const double* X, int nrhs, double* Y, void* stream);
+
+/*
+ * Y = T X for a triangular fp32 T (upper = 0: lower, j <= i; upper = 1:
+ * upper, j >= i), fp32 arithmetic, fp64 X and Y [n][nrhs], nrhs <= 16.
+ * The preconditioner M^-1 = W^T W of the refinement.
+ * Replaces: the triangular solves of cho_solve (predict.py:1254).
+ */
+int nta_trmv(const float* W, int n, long long ldw, int upper,
+             const double* X, int nrhs, double* Y, void* stream);
+
+/* WT = W^T (n x n fp32). */
+int nta_transpose_f32(const float* W, int n, long long ldw, float* WT,
+                      long long ldt, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

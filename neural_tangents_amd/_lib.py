@@ -39,6 +39,10 @@ SIGNATURES = {
                   _P, _P, _P],
     'nta_head': [_P, _P, _I, _I, _I, _I, _I, _D, _D, _D, _D, _D, _P, _P, _LL,
                  _P],
+    'nta_chol_f32': [_P, _I, _LL, _P, _LL, _P, _P],
+    'nta_symv64': [_P, _I, _LL, _D, _P, _I, _P, _P],
+    'nta_trmv': [_P, _I, _LL, _I, _P, _I, _P, _P],
+    'nta_transpose_f32': [_P, _I, _LL, _P, _LL, _P],
 }
 
 

@@ -68,19 +68,15 @@ __device__ __forceinline__ void relu_acc(float cv, float sqp, float rqp,
   if constexpr (NTK) acc_t = fmaf(pm, t, acc_t);
 }
 
-// (w, w')-diagonal 3-tap sum of K planes of one tensor.  Plane k is
+// Issues the buffer loads of K planes of one tensor: plane k is
 // (hh[k], hp[k]); planes outside [0, S)^2 (and every plane of a tail pair,
-// whose resource has size 0) read as zero.  Each thread owns the 2x2 block
-// (2W.., 2Wp..); ws[k][e], e = 2c + d, is element (2W + c, 2Wp + d).  Two
-// barriers; every thread of the block calls it the same number of times.
+// outside the resource) read as zero.  Each thread loads its 2x2 (w, w')
+// block (2W.., 2Wp..): r0 = row 2W, r1 = row 2W + 1.
 template <int S, int K>
-__device__ __forceinline__ void wpass(__amdgpu_buffer_rsrc_t rs,
-                                      const int* hh, const int* hp,
-                                      float* __restrict__ lds, unsigned tofs,
-                                      int lofs, float (&ws)[K][4]) {
-  constexpr int LS = S + 2;
-  constexpr int PL = LS * LS;
-  float2 r0[K], r1[K];
+__device__ __forceinline__ void issue_planes(__amdgpu_buffer_rsrc_t rs,
+                                             const int* hh, const int* hp,
+                                             unsigned tofs, float2 (&r0)[K],
+                                             float2 (&r1)[K]) {
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const bool ok = (unsigned)hh[k] < (unsigned)S &&
@@ -90,6 +86,19 @@ __device__ __forceinline__ void wpass(__amdgpu_buffer_rsrc_t rs,
     r0[k] = buf_load2(rs, tofs, soff);
     r1[k] = buf_load2(rs, tofs + S * 4, soff);
   }
+}
+
+// (w, w')-diagonal 3-tap sum of K loaded planes through an LDS exchange:
+// ws[k][e], e = 2c + d, is element (2W + c, 2Wp + d) of
+// X[w-1, w'-1] + X[w, w'] + X[w+1, w'+1].  Two barriers; every thread of the
+// block calls it the same number of times.
+template <int S, int K>
+__device__ __forceinline__ void exchange(const float2 (&r0)[K],
+                                         const float2 (&r1)[K],
+                                         float* __restrict__ lds, int lofs,
+                                         float (&ws)[K][4]) {
+  constexpr int LS = S + 2;
+  constexpr int PL = LS * LS;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     float* d = lds + k * PL + lofs;
@@ -117,7 +126,17 @@ __device__ __forceinline__ void wpass(__amdgpu_buffer_rsrc_t rs,
   __syncthreads();
 }
 
-template <int S, int POOL, bool NTK>
+template <int S, int K>
+__device__ __forceinline__ void wpass(__amdgpu_buffer_rsrc_t rs,
+                                      const int* hh, const int* hp,
+                                      float* __restrict__ lds, unsigned tofs,
+                                      int lofs, float (&ws)[K][4]) {
+  float2 r0[K], r1[K];
+  issue_planes<S, K>(rs, hh, hp, tofs, r0, r1);
+  exchange<S, K>(r0, r1, lds, lofs, ws);
+}
+
+template <int S, int POOL, bool NTK, bool PF>
 __global__ __launch_bounds__(256) void k_stage_fast(
     const float* __restrict__ in_n, const float* __restrict__ in_t, int n1,
     int n2, int ndiag, int ngroups, int nblk, float wv9, float b_var,
@@ -203,27 +222,57 @@ __global__ __launch_bounds__(256) void k_stage_fast(
         }
       }
     }
+    // Planes of step H: d0 @ 2H+1, 2H+2; dp @ 2H+1, 2H+2; dm @ 2H, 2H+1.
+    auto step_planes = [&](int H, int (&hh)[6], int (&hp)[6]) {
+      hh[0] = 2 * H + 1; hp[0] = 2 * H + 1 - d0;
+      hh[1] = 2 * H + 2; hp[1] = 2 * H + 2 - d0;
+      hh[2] = 2 * H + 1; hp[2] = 2 * H + 1 - dp;
+      hh[3] = 2 * H + 2; hp[3] = 2 * H + 2 - dp;
+      hh[4] = 2 * H;     hp[4] = 2 * H - dm;
+      hh[5] = 2 * H + 1; hp[5] = 2 * H + 1 - dm;
+    };
+    float2 nx0[NT][6], nx1[NT][6];     // prefetched planes of the next step
+    if constexpr (PF) {
+      int hh[6], hp[6];
+      step_planes(Hs, hh, hp);
+#pragma unroll
+      for (int tt = 0; tt < NT; ++tt)
+        issue_planes<S, 6>(tt ? rt : rn, hh, hp, tofs, nx0[tt], nx1[tt]);
+    }
     for (int H = Hs; H <= He; ++H) {
       const int Hp = H - D;
-      const int hh[6] = {2 * H + 1, 2 * H + 2, 2 * H + 1,
-                         2 * H + 2, 2 * H,     2 * H + 1};
-      const int hp[6] = {2 * H + 1 - d0, 2 * H + 2 - d0, 2 * H + 1 - dp,
-                         2 * H + 2 - dp, 2 * H - dm,     2 * H + 1 - dm};
       float pre[NT][4][4];   // [tensor][P00, P11, P10, P01][element]
 #pragma unroll
       for (int tt = 0; tt < NT; ++tt) {
         float ws[6][4];
-        wpass<S, 6>(tt ? rt : rn, hh, hp, L0, tofs, lofs, ws);
+        if constexpr (PF) {
+          exchange<S, 6>(nx0[tt], nx1[tt], L0, lofs, ws);
+        } else {
+          int hh[6], hp[6];
+          step_planes(H, hh, hp);
+          wpass<S, 6>(tt ? rt : rn, hh, hp, L0, tofs, lofs, ws);
+        }
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          pre[tt][0][e] = c0a[tt][e] + c0b[tt][e] + ws[0][e];
-          pre[tt][1][e] = c0b[tt][e] + ws[0][e] + ws[1][e];
+          const float s0 = c0b[tt][e] + ws[0][e];
+          pre[tt][0][e] = c0a[tt][e] + s0;
+          pre[tt][1][e] = s0 + ws[1][e];
           pre[tt][2][e] = cp[tt][e] + ws[2][e] + ws[3][e];
           pre[tt][3][e] = cm[tt][e] + ws[4][e] + ws[5][e];
           c0a[tt][e] = ws[0][e];
           c0b[tt][e] = ws[1][e];
           cp[tt][e] = ws[3][e];
           cm[tt][e] = ws[5][e];
+        }
+      }
+      if constexpr (PF) {
+        if (H < He) {
+          int hh[6], hp[6];
+          step_planes(H + 1, hh, hp);
+#pragma unroll
+          for (int tt = 0; tt < NT; ++tt)
+            issue_planes<S, 6>(tt ? rt : rn, hh, hp, tofs, nx0[tt],
+                               nx1[tt]);
         }
       }
       // (sq, rq) at rows 2H, 2H+1 of image i and 2Hp, 2Hp+1 of image j,
@@ -529,7 +578,7 @@ __global__ __launch_bounds__(256) void k_first16(
 
 static inline int pad8(long long n) { return (int)((n + 7) / 8 * 8); }
 
-template <int S, int POOL>
+template <int S, int POOL, bool PF>
 static int launch_stage_s(const float* in_n, const float* in_t, int n1,
                           int n2, bool ntk, float wv9, float b_var,
                           const float* q1, const float* q2, float* out_n,
@@ -542,15 +591,21 @@ static int launch_stage_s(const float* in_n, const float* in_t, int n1,
     return set_error(NTA_EINVAL, "nta_stage: tile too large (%lld pairs)", T);
   const int nblk = pad8(ngroups * ndiag);
   if (ntk)
-    hipLaunchKernelGGL((k_stage_fast<S, POOL, true>), dim3(nblk), dim3(256),
+    hipLaunchKernelGGL((k_stage_fast<S, POOL, true, PF>), dim3(nblk), dim3(256),
                        0, st, in_n, in_t, n1, n2, ndiag, (int)ngroups, nblk,
                        wv9, b_var, q1, q2, out_n, out_t);
   else
-    hipLaunchKernelGGL((k_stage_fast<S, POOL, false>), dim3(nblk), dim3(256),
+    hipLaunchKernelGGL((k_stage_fast<S, POOL, false, PF>), dim3(nblk), dim3(256),
                        0, st, in_n, in_t, n1, n2, ndiag, (int)ngroups, nblk,
                        wv9, b_var, q1, q2, out_n, out_t);
   return check_launch("nta_stage");
 }
+
+// Software prefetch of the next step's planes in k_stage_fast.
+#ifndef NTA_STAGE_PREFETCH
+#define NTA_STAGE_PREFETCH 1
+#endif
+constexpr bool kPrefetch = NTA_STAGE_PREFETCH != 0;
 
 int launch_stage_fast(const float* in_nngp, const float* in_ntk, int n1,
                       int n2, int S, int pool, int flags, float w_var,
@@ -560,25 +615,25 @@ int launch_stage_fast(const float* in_nngp, const float* in_ntk, int n1,
   const float wv9 = w_var / 9.0f;
   if (pool == 2) {
     switch (S) {
-      case 32: return launch_stage_s<32, 2>(in_nngp, in_ntk, n1, n2, ntk, wv9,
+      case 32: return launch_stage_s<32, 2, kPrefetch>(in_nngp, in_ntk, n1, n2, ntk, wv9,
                                             b_var, q1, q2, out_nngp, out_ntk,
                                             stream);
-      case 16: return launch_stage_s<16, 2>(in_nngp, in_ntk, n1, n2, ntk, wv9,
+      case 16: return launch_stage_s<16, 2, kPrefetch>(in_nngp, in_ntk, n1, n2, ntk, wv9,
                                             b_var, q1, q2, out_nngp, out_ntk,
                                             stream);
-      case 8: return launch_stage_s<8, 2>(in_nngp, in_ntk, n1, n2, ntk, wv9,
+      case 8: return launch_stage_s<8, 2, kPrefetch>(in_nngp, in_ntk, n1, n2, ntk, wv9,
                                           b_var, q1, q2, out_nngp, out_ntk,
                                           stream);
     }
   } else if (pool == 1) {
     switch (S) {
-      case 32: return launch_stage_s<32, 1>(in_nngp, in_ntk, n1, n2, ntk, wv9,
+      case 32: return launch_stage_s<32, 1, kPrefetch>(in_nngp, in_ntk, n1, n2, ntk, wv9,
                                             b_var, q1, q2, out_nngp, out_ntk,
                                             stream);
-      case 16: return launch_stage_s<16, 1>(in_nngp, in_ntk, n1, n2, ntk, wv9,
+      case 16: return launch_stage_s<16, 1, kPrefetch>(in_nngp, in_ntk, n1, n2, ntk, wv9,
                                             b_var, q1, q2, out_nngp, out_ntk,
                                             stream);
-      case 8: return launch_stage_s<8, 1>(in_nngp, in_ntk, n1, n2, ntk, wv9,
+      case 8: return launch_stage_s<8, 1, kPrefetch>(in_nngp, in_ntk, n1, n2, ntk, wv9,
                                           b_var, q1, q2, out_nngp, out_ntk,
                                           stream);
     }
