@@ -118,6 +118,27 @@ int nta_head(const float* nngp, const float* ntk, int n1, int n2, int H,
              double e, float* out_nngp, float* out_ntk, long long ld,
              void* stream);
 
+/*
+ * Adjoint of one [Conv] -> ReLU -> [AvgPool p x p] stage, per pre-pool
+ * element of a tile (no counterpart in the reference, which differentiates
+ * its XLA graph with jax.grad; this is the reverse of stax/elementwise.py:
+ * 444-455 with the _sqrt/_arctan2 JVP rules of 1278-1310 and of the pool
+ * stax/linear.py:3551-3571).  aN, aT: stencil means of the stage's nngp and
+ * ntk inputs [T][S][S][S][S] (aT = NULL: first stage, ntk input 0); gN, gT:
+ * pooled cotangents [T][S/p]^4.  Writes the cotangents of aN (gaN, already
+ * times w_var), aT, of the normalisers per element (gqa, gqb; to be summed
+ * over the partner axes) and the per-element terms of d/dw_var (gwv) and
+ * d/db_var (gbv).  Pairs (a, b) with a + self_off == b (and every pair of a
+ * self tile, n2 == 0) are pairs of identical images: their spatial diagonal
+ * is treated as rho = 1 exactly.  Pass self_off = INT_MAX for none.
+ */
+int nta_relu_bwd(const float* aN, const float* aT, int n1, int n2, int S,
+                 int pool, int self_off, float w_var, float b_var,
+                 const float* q1,
+                 const float* q2, const float* gN, const float* gT,
+                 float* gaN, float* gaT, float* gqa, float* gqb, float* gwv,
+                 float* gbv, void* stream);
+
 /* ------------------------------------------------------------------------
  * Kernel-regression solve (predict.py:1235-1258, cho_factor / cho_solve of
  * K + lambda I, mixed precision).  Row-major matrices, element (i, j) at

@@ -12,10 +12,9 @@ __all__ = ['stax', 'Kernel', 'predict', 'batch']
 
 
 def __getattr__(name):
+  import importlib
   if name == 'predict':
-    from . import predict
-    return predict
+    return importlib.import_module(__name__ + '.predict')
   if name == 'batch':
-    from .batching import batch
-    return batch
+    return importlib.import_module(__name__ + '.batching').batch
   raise AttributeError(name)

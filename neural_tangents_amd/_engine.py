@@ -248,11 +248,13 @@ def _state_elems(h, w):
 
 
 def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
-               collect_q=None, head_out=None, ld=0):
+               collect_q=None, head_out=None, ld=0, keep=None):
   """Runs all stages for one tile.  Self tile when ``n2 == 0`` (then
   ``collect_q`` receives the normaliser of each ReLU stage, computed from the
   self state on the way).  Returns the final (nngp, ntk) state when the plan
-  has no head, else writes the head output into ``head_out``."""
+  has no head, else writes the head output into ``head_out``.  ``keep``
+  (a list) receives the output state of every stage (the checkpoints the
+  adjoint recomputes from)."""
   dev = xa.device
   T = n1 if n2 == 0 else n1 * n2
   _, H, W, C = xa.shape
@@ -282,6 +284,8 @@ def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
               w_var, b_var, _lib.ptr(q1), _lib.ptr(q2), _lib.ptr(out),
               _lib.ptr(out_t), stream)
       nngp, ntk = out, out_t
+      if keep is not None:
+        keep.append((nngp, ntk))
       continue
     conv = s.conv if s.conv is not None else (1, 1, False, 1.0, 0.0)
     fh, fw, valid, w_var, b_var = conv
@@ -308,7 +312,9 @@ def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
             pw, flags, w_var, b_var, _lib.ptr(q1), _lib.ptr(q2),
             _lib.ptr(out), _lib.ptr(out_t), stream)
     nngp, ntk = out, out_t
-  if plan.head is None:
+    if keep is not None:
+      keep.append((nngp, ntk))
+  if plan.head is None or head_out is None:
     return nngp, ntk
   a, b, c, d, e = plan.head_coef
   hflags = (_lib.FLATTEN if plan.head == 'flatten' else 0) | (

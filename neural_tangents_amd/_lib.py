@@ -43,6 +43,8 @@ SIGNATURES = {
     'nta_symv64': [_P, _I, _LL, _D, _P, _I, _P, _P],
     'nta_trmv': [_P, _I, _LL, _I, _P, _I, _P, _P],
     'nta_transpose_f32': [_P, _I, _LL, _P, _LL, _P],
+    'nta_relu_bwd': [_P, _P, _I, _I, _I, _I, _I, _F, _F, _P, _P, _P, _P, _P,
+                     _P, _P, _P, _P, _P, _P],
 }
 
 

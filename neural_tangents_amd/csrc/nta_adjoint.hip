@@ -1,0 +1,122 @@
+// Adjoint of one [Conv] -> ReLU -> [AvgPool] stage, per pre-pool element.
+//
+// Forward of the element (h, h', w, w') of pair p (stax/linear.py:1387-1398,
+// stax/elementwise.py:444-455, stax/linear.py:3551-3571):
+//   c  = w_var * aN + b_var              (aN = stencil mean of nngp input)
+//   t  = w_var * aT + c   (has_t)  or  t = c   (first stage: ntk input 0)
+//   s  = sqrt(max(qa qb - c^2, 0)), theta = atan2(s, c), ds = (pi-theta)/2pi
+//   J  = (s + (pi - theta) c) / 2pi,     TD = t ds
+//   nngp_out = pool(J), ntk_out = pool(TD)   (pool = mean over p x p x p x p)
+// Given the pooled cotangents gN, gT this writes, per element,
+//   gaN = w_var (gc + gt)   gaT = w_var gt        (cotangents of aN, aT)
+//   gqa, gqb                                     (cotangents of qa, qb)
+//   gwv = (gc + gt) aN + gt aT,  gbv = gc + gt     (of w_var, b_var)
+// with gt = gTD ds and gc = gJ ds + gTD t d(ds)/dc.  The derivative rules at
+// s = 0 follow the reference's custom JVPs (_sqrt_jvp and _arctan2_jvp,
+// stax/elementwise.py:1283-1310): the tangent of s is zero there, so the
+// terms in 1/s vanish.  Pairs of identical images (a self tile, n2 == 0, or
+// the pairs (a, b) of a cross tile with a + self_off == b) have rho = 1
+// exactly on the spatial diagonal (h = h', w = w'); there s = 0 is used
+// instead of its rounded value, which gives the exact total derivative of
+// K(x, x).  The stencil mean is self-adjoint, so the cotangents of the stage
+// inputs are the forward stencil applied to gaN, gaT.
+#include "nta_common.h"
+
+namespace nta {
+
+__global__ void k_relu_bwd(const float* __restrict__ aN,
+                           const float* __restrict__ aT, int n1, int n2,
+                           int S, int pool, int self_off, float w_var,
+                           float b_var, const float* __restrict__ q1,
+                           const float* __restrict__ q2,
+                           const float* __restrict__ gN,
+                           const float* __restrict__ gT,
+                           float* __restrict__ gaN, float* __restrict__ gaT,
+                           float* __restrict__ gqa, float* __restrict__ gqb,
+                           float* __restrict__ gwv, float* __restrict__ gbv) {
+  const bool self = n2 == 0;
+  const uint64_t T = self ? (uint64_t)n1 : (uint64_t)n1 * n2;
+  const uint64_t per = (uint64_t)S * S * S * S;
+  const int So = S / pool;
+  const uint64_t per_o = (uint64_t)So * So * So * So;
+  const float inv_pool = 1.0f / (float)(pool * pool * pool * pool);
+  const uint64_t total = T * per;
+  for (uint64_t idx = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+       idx < total; idx += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = idx % per;
+    const uint64_t p = idx / per;
+    const int w2 = (int)(r % S);
+    const int w1 = (int)((r / S) % S);
+    const int h2 = (int)((r / ((uint64_t)S * S)) % S);
+    const int h1 = (int)(r / ((uint64_t)S * S * S));
+    const uint64_t i = self ? p : p / n2;
+    const uint64_t j = self ? p : p % n2;
+    const float qa = q1[(i * S + h1) * S + w1];
+    const float qb = (self ? q1 : q2)[(j * S + h2) * S + w2];
+    const float an = aN[idx];
+    const float at = aT ? aT[idx] : 0.0f;
+    const float c = fmaf(w_var, an, b_var);
+    const float t = aT ? fmaf(w_var, at, c) : c;
+    const bool same = (self || (long long)i + self_off == (long long)j) &&
+                      h1 == h2 && w1 == w2;
+    const float d = prod_minus_sq(qa, qb, c);
+    const float s = same ? 0.0f : sqrtf(fmaxf(d, 0.0f));
+    const float theta = (s == 0.0f && c == 0.0f) ? 0.5f * kPi : atan2f(s, c);
+    const float ds = (kPi - theta) * kInv2Pi;
+    const uint64_t o = p * per_o +
+                       (((uint64_t)(h1 / pool) * So + h2 / pool) * So +
+                        w1 / pool) * So + w2 / pool;
+    const float gj = gN ? gN[o] * inv_pool : 0.0f;
+    const float gtd = gT ? gT[o] * inv_pool : 0.0f;
+    // J derivatives: dJ/dc = ds, dJ/dq = s / (4 pi q)
+    float dj_qa = qa > 0.0f ? s * (0.25f / kPi) / qa : 0.0f;
+    float dj_qb = qb > 0.0f ? s * (0.25f / kPi) / qb : 0.0f;
+    // ds derivatives (zero where s = 0): d(ds)/dc = 1/(2 pi s),
+    // d(ds)/dq = -c / (4 pi s q)
+    float dd_c = 0.0f, dd_qa = 0.0f, dd_qb = 0.0f;
+    if (s > 0.0f) {
+      dd_c = kInv2Pi / s;
+      dd_qa = qa > 0.0f ? -c * (0.25f / kPi) / (s * qa) : 0.0f;
+      dd_qb = qb > 0.0f ? -c * (0.25f / kPi) / (s * qb) : 0.0f;
+    }
+    const float gt = gtd * ds;
+    const float gc = gj * ds + gtd * t * dd_c;
+    const float gct = gc + gt;
+    gaN[idx] = w_var * gct;
+    if (gaT) gaT[idx] = aT ? w_var * gt : 0.0f;
+    gqa[idx] = gj * dj_qa + gtd * t * dd_qa;
+    gqb[idx] = gj * dj_qb + gtd * t * dd_qb;
+    gwv[idx] = gct * an + gt * at;
+    gbv[idx] = gct;
+  }
+}
+
+}  // namespace nta
+
+using namespace nta;
+
+extern "C" int nta_relu_bwd(const float* aN, const float* aT, int n1, int n2,
+                            int S, int pool, int self_off, float w_var,
+                            float b_var,
+                            const float* q1, const float* q2, const float* gN,
+                            const float* gT, float* gaN, float* gaT,
+                            float* gqa, float* gqb, float* gwv, float* gbv,
+                            void* stream) {
+  if (n1 < 0 || n2 < 0 || S <= 0 || (pool != 1 && pool != 2) ||
+      S % pool != 0)
+    return set_error(NTA_EINVAL, "nta_relu_bwd: bad n1=%d n2=%d S=%d "
+                     "pool=%d", n1, n2, S, pool);
+  if (n1 == 0) return NTA_OK;
+  if (!aN || !q1 || (n2 > 0 && !q2) || !gaN || !gqa || !gqb || !gwv ||
+      !gbv)
+    return set_error(NTA_EINVAL, "nta_relu_bwd: null pointer");
+  const uint64_t T = n2 == 0 ? (uint64_t)n1 : (uint64_t)n1 * n2;
+  const uint64_t total = T * S * S * S * S;
+  uint64_t g = (total + 255) / 256;
+  if (g > (1u << 20)) g = 1u << 20;
+  hipLaunchKernelGGL(k_relu_bwd, dim3((unsigned)g), dim3(256), 0,
+                     (hipStream_t)stream, aN, aT, n1, n2, S, pool, self_off,
+                     w_var, b_var, q1, q2, gN, gT, gaN, gaT, gqa, gqb, gwv,
+                     gbv);
+  return check_launch("nta_relu_bwd");
+}
