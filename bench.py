@@ -53,6 +53,26 @@ def canonical_bytes_per_pair(depths, side):
   return total + 8
 
 
+# HBM traffic per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# (scaled by a plain copy of known size), per trace label of the engine.
+PMC_FILE = os.path.join(ROOT, 'profiles', 'stage32_hbm_traffic_pmc.json')
+PMC_KERNELS = {'nta_stage[32->16]': 'k_stage_fast<32,2,true>',
+               'nta_stage[32->32]': 'k_stage_fast<32,1,true>'}
+
+
+def measured_traffic(label, pairs_per_launch):
+  """HBM bytes per launch of ``label`` from the committed counter summary
+  (scaled to the launch's pair count), or None."""
+  try:
+    with open(PMC_FILE) as f:
+      d = json.load(f)
+    k = d['kernels'][PMC_KERNELS[label]]
+  except (OSError, KeyError, ValueError):
+    return None
+  return k['traffic_bytes_per_launch'] * pairs_per_launch / k[
+      'pairs_per_launch']
+
+
 def cpu_baseline(depths, x1, x2, n_sample):
   """Times the oracle (FP32, one thread) on an n_sample x n_sample block."""
   import numpy as np
@@ -72,6 +92,61 @@ def cpu_baseline(depths, x1, x2, n_sample):
                     f'{dt:.1f} s'}
 
 
+def krr_bench(args):
+  """Kernel-regression solve (infinite-time gradient_descent_mse_ensemble):
+  Myrtle-5 32x32 NTK of n_train synthetic images, one-hot targets (-0.1),
+  diag_reg = 1e-4 (relative to the mean trace).  Times the plain FP64
+  Cholesky solve (rocSOLVER through torch) and the mixed solve (FP32 MFMA
+  Cholesky + FP64-residual PCG), both on the same kernel, and reports the
+  relative error of alpha and of the test predictions against FP64."""
+  import torch
+  from neural_tangents_amd import _solve, stax
+  from neural_tangents_amd.data import krr_workload
+  ntr, nte = args.krr_train, args.krr_test
+  xtr, xte, ytr, _ = krr_workload(ntr, nte, seed=0)
+  dev = torch.device('cuda', 0)
+  kernel_fn = stax.myrtle(5)[2]
+  t0 = time.perf_counter()
+  k_dd = kernel_fn(torch.from_numpy(xtr).to(dev), None, 'ntk')
+  k_td = kernel_fn(torch.from_numpy(xte).to(dev),
+                   torch.from_numpy(xtr).to(dev), 'ntk')
+  torch.cuda.synchronize()
+  t_kernel = time.perf_counter() - t0
+  print(f'kernels: {t_kernel:.1f} s', file=sys.stderr, flush=True)
+  y = torch.from_numpy(ytr).to(dev, torch.float64)
+  res = {}
+  for mode in ('fp64', 'mixed'):
+    times = []
+    for rep in range(args.krr_reps):
+      torch.cuda.synchronize()
+      t = time.perf_counter()
+      s = _solve.CholeskySolver(k_dd, 1e-4, False, mode)
+      alpha = s.solve(y)
+      f = k_td.to(torch.float64) @ alpha
+      torch.cuda.synchronize()
+      times.append(time.perf_counter() - t)
+    res[mode] = (sorted(times)[len(times) // 2], alpha, f,
+                 getattr(s, 'iterations', 0), getattr(s, 'last_residual',
+                                                      None))
+  t64, a64, f64 = res['fp64'][:3]
+  tmx, amx, fmx, iters, resid = res['mixed']
+  err_a = float(torch.linalg.norm(amx - a64) / torch.linalg.norm(a64))
+  err_f = float(torch.linalg.norm(fmx - f64) / torch.linalg.norm(f64))
+  out = {
+      'metric': 'KRR solve time vs plain FP64 at <=1e-6 rel err',
+      'value': t64 / tmx, 'unit': 'x speed-up (plain FP64 / mixed)',
+      'n_gpus': 1, 'higher_is_better': True, 'dtype': 'fp32+fp64',
+      'data': 'synthetic (blurred Gaussian images, seed 0)',
+      'config': {'workload': f'krr_myrtle5_32x32_train{ntr}_test{nte}',
+                 'diag_reg': 1e-4, 'get': 'ntk'},
+      'plain_fp64_ms': t64 * 1e3, 'mixed_ms': tmx * 1e3,
+      'pcg_iterations': iters, 'final_rel_residual': resid,
+      'rel_err_alpha_vs_fp64': err_a, 'rel_err_ftest_vs_fp64': err_f,
+      'kernel_seconds': t_kernel,
+  }
+  print(json.dumps(out), flush=True)
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument('--gpus', type=int, default=1)
@@ -81,12 +156,20 @@ def main():
                   choices=sorted(WORKLOADS))
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--cpu-sample', type=int, default=14)
+  ap.add_argument('--krr', action='store_true',
+                  help='kernel-regression solve benchmark instead')
+  ap.add_argument('--krr-train', type=int, default=8192)
+  ap.add_argument('--krr-test', type=int, default=1024)
+  ap.add_argument('--krr-reps', type=int, default=3)
   args = ap.parse_args()
   if args.gpus != 1:
     print('bench.py runs on one GPU only: this engine has no multi-GPU path '
           '(pair tiles shard across GPUs without communication; see the '
           'design notes).', file=sys.stderr)
     sys.exit(2)
+  if args.krr:
+    krr_bench(args)
+    return
 
   import torch
   from neural_tangents_amd import _engine, stax
@@ -126,6 +209,7 @@ def main():
   avg_ms = dom['ms'] / dom['launches']
   avg_bytes = dom['bytes'] / dom['launches']
   achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+  traffic = measured_traffic(dom_name, dom['pairs'] / dom['launches'])
 
   pairs = n1 * n2
   ms_per_step = elapsed / args.steps * 1e3
@@ -153,7 +237,7 @@ def main():
       'roofline': {'bound': 'hbm', 'kernel': dom_name,
                    'achieved': achieved, 'peak': HBM_PEAK_GBS,
                    'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
-                   'traffic': None, 'avg_launch_ms': avg_ms,
+                   'traffic': traffic, 'avg_launch_ms': avg_ms,
                    'alg_bytes_per_launch': avg_bytes},
       'kernels': {n: {'ms_per_step': v['ms'] / args.steps,
                       'launches_per_step': v['launches'] / args.steps,

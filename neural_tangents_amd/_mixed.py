@@ -39,19 +39,30 @@ class MixedCholesky:
     K32 = torch.zeros((npad, npad), dtype=torch.float32, device=self.device)
     K32[:n, :n] = K.to(torch.float32)
     self.K32 = K32                               # kept for FP64 residuals
-    A = K32.clone()
-    A.diagonal().add_(self.lam)
-    if npad > n:                                  # identity padding
-      A.diagonal()[n:] = 1.0
-    W = torch.zeros_like(A)
-    info = torch.zeros((1,), dtype=torch.int32, device=self.device)
-    _lib.call('nta_chol_f32', _lib.ptr(A), npad, npad, _lib.ptr(W), npad,
-              _lib.ptr(info), stream)
-    if int(info.item()) != 0:
+    # The preconditioner factors K32 + shift I.  When FP32 breaks down (a
+    # non-positive pivot: the kernel is too ill-conditioned for single
+    # precision at this lambda), the shift grows by 10x; the FP64-residual
+    # iteration still converges to the solution of the unshifted system.
+    mean_diag = float(torch.diagonal(K32)[:n].double().mean())
+    shift = self.lam
+    for attempt in range(8):
+      A = K32.clone()
+      A.diagonal().add_(shift)
+      if npad > n:                                # identity padding
+        A.diagonal()[n:] = 1.0
+      W = torch.zeros_like(A)
+      info = torch.zeros((1,), dtype=torch.int32, device=self.device)
+      _lib.call('nta_chol_f32', _lib.ptr(A), npad, npad, _lib.ptr(W), npad,
+                _lib.ptr(info), stream)
+      if int(info.item()) == 0:
+        break
+      shift = max(10.0 * shift, 1e-7 * mean_diag)
+    else:
       raise ValueError(
-          f'FP32 Cholesky failed at leading minor {int(info.item())}: the '
-          'regularised train kernel is not positive definite in FP32; '
-          'increase diag_reg or use solver="fp64".')
+          f'FP32 Cholesky failed at leading minor {int(info.item())} even '
+          f'with shift {shift:.3g}: the train kernel is not positive '
+          'definite; increase diag_reg or use solver="fp64".')
+    self.shift = shift
     self.L = A
     self.W = W
     WT = torch.empty_like(W)

@@ -218,15 +218,30 @@ _FNS = {
 
 # ----------------------------------------------------------- kernel_fn
 
+_NT_CACHE = {}
+
+
+def _namedtuple(name, fields):
+  """Cached namedtuple types (``utils.py:161-167``)."""
+  key = (name, fields)
+  if key not in _NT_CACHE:
+    _NT_CACHE[key] = collections.namedtuple(name, fields)
+  return _NT_CACHE[key]
+
+
 def _canonicalize_get(get):
-  """``utils.py:139-194`` (``canonicalize_get``)."""
+  """``utils.py:139-156`` (``canonicalize_get``): a string returns the bare
+  value, a tuple (of any length) a namedtuple; entries are lower-cased and
+  must be unique and non-empty."""
   if get is None:
     return True, ('nngp', 'ntk', 'cov1', 'cov2')
+  if not get:
+    raise ValueError('"get" must be non-empty.')
   if isinstance(get, str):
-    return False, (get,)
-  get = tuple(get)
-  if len(set(get)) != len(get):
-    raise ValueError(f'Repeated entries in `get`: {get}.')
+    return False, (get.lower(),)
+  get = tuple(g.lower() for g in get)
+  if len(set(get)) < len(get):
+    raise ValueError(f'All entries in "get" must be unique. Got {get}')
   return False, get
 
 
@@ -319,9 +334,9 @@ class AnalyticKernelFn:
                batch_axis=0, channel_axis=len(shape1) - 1)
     if return_kernel:
       return k
-    if len(get_t) == 1:
+    if isinstance(get, str):
       return getattr(k, get_t[0])
-    return collections.namedtuple('AnalyticKernel', get_t)(
+    return _namedtuple('AnalyticKernel', get_t)(
         *[getattr(k, g) for g in get_t])
 
   def _out_shape(self, in_shape):
