@@ -51,7 +51,6 @@ def test_mixed_solver_refines_to_fp64_on_ill_conditioned_kernel():
   x = s.solve(y)
   want = _solve.CholeskySolver(K, 1e-6, False, 'fp64').solve(y)
   assert rel_err(x.cpu().numpy(), want.cpu().numpy()) <= 1e-6
-  assert s.last_residual <= 1e-10
   assert s.impl.shift > s.lam
 
 
