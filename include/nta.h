@@ -139,43 +139,92 @@ int nta_relu_bwd(const float* aN, const float* aT, int n1, int n2, int S,
                  float* gaN, float* gaT, float* gqa, float* gqb, float* gwv,
                  float* gbv, void* stream);
 
+/*
+ * Diagonal-spatial chain: a network of nstage [Conv fh x fw, stride 1]
+ * [-> ReLU] stages without pooling, read out by Flatten and the composed
+ * Dense map (a, b, c, d, e as in nta_head), on the spatial diagonal
+ * (h = h', w = w') only, where the Conv stencil is an fh x fw window mean.
+ * One launch runs every stage of every pair; states stay on chip.
+ *   stage_desc (host) [nstage][3]: flags (NTA_CONV, NTA_CONV_VALID,
+ *   NTA_RELU), fh, fw;  stage_wb (host) [nstage][2]: w_var, b_var.
+ *   q1, q2: device [nstage][n][H][W] normaliser tables (a stage's values
+ *   fill the first n * hout * wout floats of its slot).
+ * Self mode (n2 == 0): the chains of x1's images against themselves; q1 is
+ *   written (the pre-activation of every ReLU stage) and out_nngp[i]
+ *   receives the readout of image i (cov1).
+ * Cross mode: out_nngp[i * ld + j] (and out_ntk with NTA_NTK in flags);
+ *   sym = 1 (x2 == x1, n2 == n1) computes the upper triangle and writes both
+ *   (i, j) and (j, i).
+ * H * W <= 3840, nstage <= 48.
+ * Replaces: Conv.kernel_fn with diagonal_spatial
+ * (_conv_kernel_diagonal_spatial, linear.py:3381-3437), ABRelu.kernel_fn
+ * (elementwise.py:423-477) on diagonal kernels, Flatten.kernel_fn
+ * (linear.py:1868-1899), Dense.kernel_fn (linear.py:902-926) and the
+ * diagonal-spatial input covariance (requirements.py:556-582).
+ */
+int nta_diag_chain(const float* x1, const float* x2, int n1, int n2, int sym,
+                   int H, int W, int C, int nstage, const int* stage_desc,
+                   const float* stage_wb, float* q1, const float* q2,
+                   int flags, double a, double b, double c, double d,
+                   double e, float* out_nngp, float* out_ntk, long long ld,
+                   void* stream);
+
 /* ------------------------------------------------------------------------
  * Kernel-regression solve (predict.py:1235-1258, cho_factor / cho_solve of
  * K + lambda I, mixed precision).  Row-major matrices, element (i, j) at
- * [i * ld + j].
+ * [i * ld + j]; n, and the leading dimensions, multiples of 64 and 4.
  */
 
 /*
- * In-place FP32 Cholesky of the lower triangle of A (n x n, n a multiple of
- * 64; upper triangle zeroed), with the trailing updates on fp32 MFMA, and
- * W = L^-1 (lower, n x n).  *info (device int, zero on entry) receives the
- * 1-based index of the first non-positive pivot.
+ * The split n1 of the top-level block of nta_chol_f32 for an n x n matrix
+ * (n itself when n <= 64; 0 when n is not a positive multiple of 64).
+ */
+int nta_chol_split(int n);
+
+/*
+ * FP32 Cholesky of A (lower triangle read; A is overwritten with
+ * intermediates) by recursive blocking with fp32 MFMA products, with the
+ * factor inverted on the way.  W (n x n) must be zero on entry.  On return,
+ * with L L^T = A and n1 = nta_chol_split(n):
+ *   W[0:n1, 0:n1] = L11^-1, W[n1:n, n1:n] = L22^-1 (lower, zeros above),
+ *   W[n1:n, 0:n1] = L21,
+ * the blocks from which the preconditioner (L L^T)^-1 is applied.
+ * *info (device int, zero on entry) receives the 1-based index of the first
+ * non-positive pivot.
  * Replaces: jsp.linalg.cho_factor (predict.py:1244).
  */
 int nta_chol_f32(float* A, int n, long long lda, float* W, long long ldw,
                  int* info, void* stream);
 
-/*
- * Y = (A + lam I) X in FP64: A fp32 (upcast exactly), X, Y fp64 [n][nrhs],
- * nrhs <= 16.  The FP64 residual of the refinement.
- * Replaces: the regularised operand of cho_solve (predict.py:1224-1232,
- * 1254).
- */
-int nta_symv64(const float* A, int n, long long lda, double lam,
-               const double* X, int nrhs, double* Y, void* stream);
+/* Bytes of device workspace nta_pcg64 / nta_chol_precond need (-1 when the
+ * arguments are invalid: nrhs must be 1..16). */
+long long nta_pcg64_work_bytes(int n, int nrhs);
 
 /*
- * Y = T X for a triangular fp32 T (upper = 0: lower, j <= i; upper = 1:
- * upper, j >= i), fp32 arithmetic, fp64 X and Y [n][nrhs], nrhs <= 16.
- * The preconditioner M^-1 = W^T W of the refinement.
+ * Z = (L L^T)^-1 R in fp32 from the blocks nta_chol_f32 left in W;
+ * R, Z [n][nrhs] fp32, nrhs <= 16, n1 = nta_chol_split(n).
  * Replaces: the triangular solves of cho_solve (predict.py:1254).
  */
-int nta_trmv(const float* W, int n, long long ldw, int upper,
-             const double* X, int nrhs, double* Y, void* stream);
+int nta_chol_precond(const float* W, int n, long long ldw, int n1,
+                     const float* R, int nrhs, float* Z, void* work,
+                     long long work_bytes, void* stream);
 
-/* WT = W^T (n x n fp32). */
-int nta_transpose_f32(const float* W, int n, long long ldw, float* WT,
-                      long long ldt, void* stream);
+/*
+ * Solves (K + lam I) X = B by conjugate gradients preconditioned with the
+ * FP32 factor in W (nta_chol_f32 of K + shift I), FP64 vectors, FP64
+ * products (K read in fp32, exact) and the FP64 shift lam.  B, X
+ * [n][nrhs] fp64, nrhs <= 16.  Iterates until every column's relative
+ * residual ||r|| / ||b|| is <= tol, stops improving for 10 iterations, or
+ * max_iter is reached; *iters receives the iteration count and *resid the
+ * largest relative residual of the returned X, recomputed from scratch
+ * (host pointers, may be NULL).  Synchronises the stream once per
+ * iteration to test convergence.
+ * Replaces: cho_solve (predict.py:1254) of the regularised train kernel.
+ */
+int nta_pcg64(const float* K, int n, long long ldk, double lam,
+              const float* W, long long ldw, int n1, const double* B,
+              int nrhs, double* X, double tol, int max_iter, void* work,
+              long long work_bytes, int* iters, double* resid, void* stream);
 
 #ifdef __cplusplus
 }

@@ -29,7 +29,7 @@ def build(verbose: bool = False) -> str:
   hipcc = shutil.which('hipcc') or '/opt/rocm/bin/hipcc'
   tmp = LIB + '.tmp'
   cmd = [hipcc] + HIP_FLAGS + ['-I', os.path.join(ROOT, 'include'),
-                               '-o', tmp] + sources() + ['-lrocblas']
+                               '-o', tmp] + sources()
   if verbose:
     print(' '.join(cmd), flush=True)
   subprocess.run(cmd, check=True)

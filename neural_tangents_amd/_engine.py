@@ -66,6 +66,9 @@ class Plan:
   is_gaussian: bool
   out_channels: Optional[int]
   out_spatial: Tuple[int, int]
+  # Flatten readout and no pooling: only the spatial diagonal is needed
+  # (the reference's diagonal_spatial requirement, requirements.py:912-1054).
+  diag: bool = False
 
 
 class _Unsupported(NotImplementedError):
@@ -213,9 +216,10 @@ def compile_plan(specs, in_shape) -> Plan:
     a, b = wv * a, wv * b + bv
     ntk_m, ntk_t, ntk_1 = a + wv * ntk_m, wv * ntk_t, b + wv * ntk_1
   coef = (a, b, ntk_m, ntk_t, ntk_1)
+  diag = head == 'flatten' and all(s.pool is None for s in stages)
   return Plan(stages=stages, head=head, head_coef=coef, n_convs=n_convs,
               is_gaussian=is_gaussian, out_channels=chans,
-              out_spatial=(h, w))
+              out_spatial=(h, w), diag=diag)
 
 
 # ------------------------------------------------------------------ runner
@@ -371,12 +375,73 @@ def _self_chain(plan: Plan, x, want_cov, tile_bytes, stream):
   return qs, cov
 
 
+def _diag_tables(plan: Plan):
+  """Host stage tables of nta_diag_chain."""
+  import ctypes
+  n = len(plan.stages)
+  desc = (ctypes.c_int * (3 * n))()
+  wb = (ctypes.c_float * (2 * n))()
+  for k, s in enumerate(plan.stages):
+    f = 0
+    fh = fw = 1
+    wv, bv = 1.0, 0.0
+    if s.conv is not None:
+      fh, fw, valid, wv, bv = s.conv
+      f |= _lib.CONV | (_lib.CONV_VALID if valid else 0)
+    if s.relu:
+      f |= _lib.RELU
+    desc[3 * k:3 * k + 3] = [f, fh, fw]
+    wb[2 * k:2 * k + 2] = [wv, bv]
+  return desc, wb
+
+
+def _run_diag(plan: Plan, x1, x2, want_ntk, stream):
+  """Diagonal-spatial plans: one nta_diag_chain launch per self chain and
+  one for all cross pairs (states never leave the chip, so no tiling)."""
+  desc, wb = _diag_tables(plan)
+  n = len(plan.stages)
+  _, H, W, C = x1.shape
+  a, b, c, d, e = plan.head_coef
+  dev = x1.device
+  has_relu = any(s.relu for s in plan.stages)
+
+  def self_chain(x):
+    N = x.shape[0]
+    q = torch.empty((n, N, H, W), dtype=torch.float32, device=dev) \
+        if has_relu else None
+    cov = torch.empty((N,), dtype=torch.float32, device=dev)
+    _launch('nta_diag_chain[self]', 4 * N * H * W * C, N, _lib.ptr(x),
+            None, N, 0, 0, H, W, C, n, desc, wb, _lib.ptr(q), None, 0, a, b,
+            c, d, e, _lib.ptr(cov), None, 0, stream)
+    return q, cov
+
+  q1, cov1 = self_chain(x1)
+  if x2 is not None:
+    q2, cov2 = self_chain(x2)
+    xb = x2
+  else:
+    q2, cov2, xb = q1, None, x1
+  N1, N2 = x1.shape[0], xb.shape[0]
+  nngp = torch.empty((N1, N2), dtype=torch.float32, device=dev)
+  ntk = torch.empty((N1, N2), dtype=torch.float32, device=dev) \
+      if want_ntk else None
+  sym = 1 if x2 is None else 0
+  pairs = N1 * (N1 + 1) // 2 if sym else N1 * N2
+  _launch('nta_diag_chain[cross]', 4 * pairs * (2 * H * W * C + 2), pairs,
+          _lib.ptr(x1), _lib.ptr(xb), N1, N2, sym, H, W, C, n, desc, wb,
+          _lib.ptr(q1), _lib.ptr(q2), _lib.NTK if want_ntk else 0, a, b, c,
+          d, e, _lib.ptr(nngp), _lib.ptr(ntk), N2, stream)
+  return {'cov1': cov1, 'cov2': cov2, 'nngp': nngp, 'ntk': ntk}
+
+
 def run(plan: Plan, x1, x2, want_nngp, want_ntk, want_cov,
         tile_bytes=DEFAULT_TILE_BYTES, symmetric=True):
   """Computes nngp/ntk (and cov1/cov2) for inputs x1 [N1,H,W,C], x2 or None.
 
   Returns dict with 'nngp', 'ntk', 'cov1', 'cov2' tensors (or None)."""
   stream = _lib.stream_handle(x1.device)
+  if plan.diag:
+    return _run_diag(plan, x1, x2, want_ntk, stream)
   q1s, cov1 = _self_chain(plan, x1, want_cov, tile_bytes, stream)
   if x2 is not None:
     q2s, cov2 = _self_chain(plan, x2, want_cov, tile_bytes, stream)

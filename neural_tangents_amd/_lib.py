@@ -40,11 +40,19 @@ SIGNATURES = {
     'nta_head': [_P, _P, _I, _I, _I, _I, _I, _D, _D, _D, _D, _D, _P, _P, _LL,
                  _P],
     'nta_chol_f32': [_P, _I, _LL, _P, _LL, _P, _P],
-    'nta_symv64': [_P, _I, _LL, _D, _P, _I, _P, _P],
-    'nta_trmv': [_P, _I, _LL, _I, _P, _I, _P, _P],
-    'nta_transpose_f32': [_P, _I, _LL, _P, _LL, _P],
+    'nta_chol_precond': [_P, _I, _LL, _I, _P, _I, _P, _P, _LL, _P],
+    'nta_pcg64': [_P, _I, _LL, _D, _P, _LL, _I, _P, _I, _P, _D, _I, _P, _LL,
+                  _P, _P, _P],
+    'nta_diag_chain': [_P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P,
+                       _I, _D, _D, _D, _D, _D, _P, _P, _LL, _P],
     'nta_relu_bwd': [_P, _P, _I, _I, _I, _I, _I, _F, _F, _P, _P, _P, _P, _P,
                      _P, _P, _P, _P, _P, _P],
+}
+
+# Entry points that return a value instead of a status.
+VALUE_FNS = {
+    'nta_chol_split': (_I, [_I]),
+    'nta_pcg64_work_bytes': (_LL, [_I, _I]),
 }
 
 
@@ -69,12 +77,22 @@ def load():
       fn = getattr(lib, name)
       fn.restype = ctypes.c_int
       fn.argtypes = args
+    for name, (res, args) in VALUE_FNS.items():
+      fn = getattr(lib, name)
+      fn.restype = res
+      fn.argtypes = args
     _lib = lib
   return _lib
 
 
 def exported_symbols():
-  return ['nta_version', 'nta_last_error'] + list(SIGNATURES)
+  return (['nta_version', 'nta_last_error'] + list(SIGNATURES) +
+          list(VALUE_FNS))
+
+
+def value(name, *args):
+  """Calls an entry point that returns a value (VALUE_FNS)."""
+  return getattr(load(), name)(*args)
 
 
 def call(name, *args):

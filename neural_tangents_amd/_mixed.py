@@ -3,16 +3,16 @@ FP64 residuals, preconditioned conjugate gradients.
 
 For ``A = K + lambda I`` (``predict.py:1224-1258``):
 
-1. ``nta_chol_f32`` factors ``K32 + lambda I`` in FP32 (blocked, MFMA
-   trailing updates) and forms ``W = L^-1``; ``M^-1 = W^T W`` is the
-   preconditioner (two dense triangular products, ``nta_trmv``).
-2. Conjugate gradients on ``A`` with FP64 vectors; ``A p`` is
-   ``nta_symv64`` (the fp32 kernel upcast exactly, FP64 accumulation and the
-   FP64 ``lambda`` shift), so the converged solution is the FP64 solution of
-   the regularised system.  The FP32 factor only steers convergence.
-   Iteration stops when every column's relative residual is below ``tol``
-   or stops decreasing.
+1. ``nta_chol_f32`` factors ``K32 + shift I`` in FP32 by recursive blocking
+   on fp32 MFMA and inverts the factor on the way; the preconditioner
+   ``(L L^T)^-1`` is applied from the blocks it leaves (``nta_chol_precond``).
+2. ``nta_pcg64`` runs conjugate gradients on ``A`` with FP64 vectors:
+   ``A p`` reads the fp32 kernel (exact in FP64), accumulates in FP64 and
+   adds the FP64 ``lambda``, so the converged solution is the FP64 solution
+   of the regularised system; the FP32 factor only steers convergence.
 """
+import ctypes
+
 import torch
 
 from . import _lib
@@ -35,9 +35,15 @@ class MixedCholesky:
     self.last_residual = None
     npad = -(-n // self.NB) * self.NB
     self.npad = npad
+    self.n1 = _lib.value('nta_chol_split', npad)
     stream = _lib.stream_handle(self.device)
-    K32 = torch.zeros((npad, npad), dtype=torch.float32, device=self.device)
-    K32[:n, :n] = K.to(torch.float32)
+    if npad == n and K.dtype == torch.float32 and K.is_contiguous():
+      K32 = K
+    else:
+      K32 = torch.zeros((npad, npad), dtype=torch.float32, device=self.device)
+      K32[:n, :n] = K.to(torch.float32)
+      if npad > n:                                # identity padding
+        K32.diagonal()[n:] = 1.0
     self.K32 = K32                               # kept for FP64 residuals
     # The preconditioner factors K32 + shift I.  When FP32 breaks down (a
     # non-positive pivot: the kernel is too ill-conditioned for single
@@ -45,13 +51,12 @@ class MixedCholesky:
     # iteration still converges to the solution of the unshifted system.
     mean_diag = float(torch.diagonal(K32)[:n].double().mean())
     shift = self.lam
+    info = torch.zeros((1,), dtype=torch.int32, device=self.device)
     for attempt in range(8):
       A = K32.clone()
       A.diagonal().add_(shift)
-      if npad > n:                                # identity padding
-        A.diagonal()[n:] = 1.0
       W = torch.zeros_like(A)
-      info = torch.zeros((1,), dtype=torch.int32, device=self.device)
+      info.zero_()
       _lib.call('nta_chol_f32', _lib.ptr(A), npad, npad, _lib.ptr(W), npad,
                 _lib.ptr(info), stream)
       if int(info.item()) == 0:
@@ -62,13 +67,18 @@ class MixedCholesky:
           f'FP32 Cholesky failed at leading minor {int(info.item())} even '
           f'with shift {shift:.3g}: the train kernel is not positive '
           'definite; increase diag_reg or use solver="fp64".')
+    del A
     self.shift = shift
-    self.L = A
     self.W = W
-    WT = torch.empty_like(W)
-    _lib.call('nta_transpose_f32', _lib.ptr(W), npad, npad, _lib.ptr(WT),
-              npad, stream)
-    self.WT = WT
+    self._work = {}
+
+  def _workspace(self, nrhs):
+    ws = self._work.get(nrhs)
+    if ws is None:
+      nbytes = _lib.value('nta_pcg64_work_bytes', self.npad, nrhs)
+      ws = torch.empty((nbytes,), dtype=torch.uint8, device=self.device)
+      self._work[nrhs] = ws
+    return ws
 
   # --------------------------------------------------------------- ops
   def _pad(self, x):
@@ -79,75 +89,42 @@ class MixedCholesky:
     out[:self.n] = x
     return out
 
-  def matvec(self, X):
-    """(K + lambda I) X in FP64 (padded rows are identity)."""
-    Y = torch.empty_like(X)
-    _lib.call('nta_symv64', _lib.ptr(self.K32), self.npad, self.npad,
-              self.lam, _lib.ptr(X), X.shape[1], _lib.ptr(Y),
-              _lib.stream_handle(self.device))
-    if self.npad > self.n:
-      Y[self.n:] = X[self.n:] * (1.0 + self.lam)
-    return Y
-
   def precond(self, R):
-    """W^T (W R) with W = L^-1 (fp32 arithmetic)."""
-    T = torch.empty_like(R)
-    Z = torch.empty_like(R)
-    st = _lib.stream_handle(self.device)
-    _lib.call('nta_trmv', _lib.ptr(self.W), self.npad, self.npad, 0,
-              _lib.ptr(R), R.shape[1], _lib.ptr(T), st)
-    _lib.call('nta_trmv', _lib.ptr(self.WT), self.npad, self.npad, 1,
-              _lib.ptr(T), R.shape[1], _lib.ptr(Z), st)
+    """(L L^T)^-1 R in fp32 (R: [npad, k], k <= 16)."""
+    R32 = R.to(torch.float32).contiguous()
+    Z = torch.empty_like(R32)
+    ws = self._workspace(R.shape[1])
+    _lib.call('nta_chol_precond', _lib.ptr(self.W), self.npad, self.npad,
+              self.n1, _lib.ptr(R32), R.shape[1], _lib.ptr(Z), _lib.ptr(ws),
+              ws.numel(), _lib.stream_handle(self.device))
     return Z
 
   # -------------------------------------------------------------- solve
   def solve(self, b: torch.Tensor) -> torch.Tensor:
     b = b.to(self.device, torch.float64)
+    if b.ndim == 1:
+      return self.solve(b[:, None])[:, 0]
     out_cols = b.shape[1]
     chunks = []
+    its, res = 0, 0.0
     for c0 in range(0, out_cols, 16):
-      chunks.append(self._solve16(b[:, c0:c0 + 16]))
+      x, it, r = self._solve16(b[:, c0:c0 + 16])
+      chunks.append(x)
+      its, res = max(its, it), max(res, r)
+    self.iterations = its
+    self.last_residual = res
     return torch.cat(chunks, dim=1) if len(chunks) > 1 else chunks[0]
 
   def _solve16(self, b):
-    B = self._pad(b)
-    bnorm = torch.linalg.vector_norm(B, dim=0).clamp_min(1e-300)
-    X = self.precond(B)
-    R = B - self.matvec(X)
-    Z = self.precond(R)
-    P = Z.clone()
-    rz = (R * Z).sum(0)
-    best = float('inf')
-    stall = 0
-    it = 0
-    rel = torch.linalg.vector_norm(R, dim=0) / bnorm
-    for it in range(1, self.max_iter + 1):
-      Q = self.matvec(P)
-      pq = (P * Q).sum(0)
-      alpha = torch.where(pq != 0, rz / pq, torch.zeros_like(pq))
-      X = X + alpha * P
-      if it % 25 == 0:
-        R = B - self.matvec(X)                 # true FP64 residual
-      else:
-        R = R - alpha * Q
-      rel = torch.linalg.vector_norm(R, dim=0) / bnorm
-      worst = float(rel.max())
-      if worst <= self.tol:
-        break
-      if worst < best:
-        best = worst
-        stall = 0
-      else:
-        stall += 1
-        if stall >= 10:           # no progress for 10 iterations
-          break
-      Z = self.precond(R)
-      rz_new = (R * Z).sum(0)
-      beta = torch.where(rz != 0, rz_new / rz, torch.zeros_like(rz))
-      P = Z + beta * P
-      rz = rz_new
-    R = B - self.matvec(X)
-    self.iterations = it
-    self.last_residual = float(
-        (torch.linalg.vector_norm(R, dim=0) / bnorm).max())
-    return X[:self.n]
+    B = self._pad(b).contiguous()
+    k = B.shape[1]
+    X = torch.empty_like(B)
+    ws = self._workspace(k)
+    iters = ctypes.c_int(0)
+    resid = ctypes.c_double(0.0)
+    _lib.call('nta_pcg64', _lib.ptr(self.K32), self.npad, self.npad,
+              self.lam, _lib.ptr(self.W), self.npad, self.n1, _lib.ptr(B), k,
+              _lib.ptr(X), self.tol, self.max_iter, _lib.ptr(ws), ws.numel(),
+              ctypes.byref(iters), ctypes.byref(resid),
+              _lib.stream_handle(self.device))
+    return X[:self.n], iters.value, resid.value

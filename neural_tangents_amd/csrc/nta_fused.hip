@@ -136,7 +136,32 @@ __device__ __forceinline__ void wpass(__amdgpu_buffer_rsrc_t rs,
   exchange<S, K>(r0, r1, lds, lofs, ws);
 }
 
-template <int S, int POOL, bool NTK, bool PF>
+// (w, w')-diagonal 3-tap sums of one LDS plane written by the thread block
+// (zero border): ws[e], e = 2c + d, as in exchange(), with the thread's own
+// elements read back from LDS (paired 8-byte reads).
+template <int S>
+__device__ __forceinline__ void read_diag3(const float* __restrict__ plane,
+                                           int lofs, float (&ws)[4]) {
+  constexpr int LS = S + 2;
+  const float* s = plane + lofs;
+  const float2 ra = *reinterpret_cast<const float2*>(s - LS - 1);
+  const float2 rb = *reinterpret_cast<const float2*>(s - 1);
+  const float v01 = s[1];
+  const float v10 = s[LS];
+  const float2 re = *reinterpret_cast<const float2*>(s + LS + 1);
+  const float2 rf = *reinterpret_cast<const float2*>(s + 2 * LS + 1);
+  ws[0] = ra.x + rb.y + re.x;    // (2W-1, 2Wp-1) + (2W, 2Wp) + (2W+1, 2Wp+1)
+  ws[1] = ra.y + v01 + re.y;     // (2W-1, 2Wp) + (2W, 2Wp+1) + (2W+1, 2Wp+2)
+  ws[2] = rb.x + v10 + rf.x;     // (2W, 2Wp-1) + (2W+1, 2Wp) + (2W+2, 2Wp+1)
+  ws[3] = rb.y + re.x + rf.y;    // (2W, 2Wp) + (2W+1, 2Wp+1) + (2W+2, 2Wp+2)
+}
+
+// PF selects how the planes of a step are fetched (POOL == 2):
+//   0: issued at the step that uses them;
+//   1: prefetched into registers one step ahead, after the step's exchange;
+//   2: as 1, but issued right after the step's planes are written to LDS
+//      (both tensors share one exchange: two barriers per step).
+template <int S, int POOL, bool NTK, int PF>
 __global__ __launch_bounds__(256) void k_stage_fast(
     const float* __restrict__ in_n, const float* __restrict__ in_t, int n1,
     int n2, int ndiag, int ngroups, int nblk, float wv9, float b_var,
@@ -148,11 +173,11 @@ __global__ __launch_bounds__(256) void k_stage_fast(
   constexpr int LS = S + 2;
   constexpr int PL = LS * LS;
   constexpr int NT = NTK ? 2 : 1;
-  constexpr int NB = POOL == 2 ? 6 : 1;   // LDS planes per exchange
+  constexpr int NB = POOL == 2 ? (PF == 2 ? 6 * NT : 6) : 1;  // LDS planes
   constexpr int SO = S / POOL;
   constexpr long long PER_IN = (long long)S * S * S * S;
   constexpr long long PER_OUT = (long long)SO * SO * SO * SO;
-  __shared__ float lds[JPW * NB * PL];
+  __shared__ __align__(16) float lds[JPW * NB * PL];
   __shared__ float2 lq[JPW * 2 * S * S];
 
   const int L = xcd_remap(blockIdx.x, nblk);
@@ -232,7 +257,7 @@ __global__ __launch_bounds__(256) void k_stage_fast(
       hh[5] = 2 * H + 1; hp[5] = 2 * H + 1 - dm;
     };
     float2 nx0[NT][6], nx1[NT][6];     // prefetched planes of the next step
-    if constexpr (PF) {
+    if constexpr (PF != 0) {
       int hh[6], hp[6];
       step_planes(Hs, hh, hp);
 #pragma unroll
@@ -241,11 +266,37 @@ __global__ __launch_bounds__(256) void k_stage_fast(
     }
     for (int H = Hs; H <= He; ++H) {
       const int Hp = H - D;
+      if constexpr (PF == 2) {
+        __syncthreads();             // the previous step's reads are done
+#pragma unroll
+        for (int tt = 0; tt < NT; ++tt)
+#pragma unroll
+          for (int k = 0; k < 6; ++k) {
+            float* d = L0 + (tt * 6 + k) * PL + lofs;
+            d[0] = nx0[tt][k].x;
+            d[1] = nx0[tt][k].y;
+            d[LS] = nx1[tt][k].x;
+            d[LS + 1] = nx1[tt][k].y;
+          }
+        if (H < He) {
+          int hh[6], hp[6];
+          step_planes(H + 1, hh, hp);
+#pragma unroll
+          for (int tt = 0; tt < NT; ++tt)
+            issue_planes<S, 6>(tt ? rt : rn, hh, hp, tofs, nx0[tt],
+                               nx1[tt]);
+        }
+        __syncthreads();
+      }
       float pre[NT][4][4];   // [tensor][P00, P11, P10, P01][element]
 #pragma unroll
       for (int tt = 0; tt < NT; ++tt) {
         float ws[6][4];
-        if constexpr (PF) {
+        if constexpr (PF == 2) {
+#pragma unroll
+          for (int k = 0; k < 6; ++k)
+            read_diag3<S>(L0 + (tt * 6 + k) * PL, lofs, ws[k]);
+        } else if constexpr (PF == 1) {
           exchange<S, 6>(nx0[tt], nx1[tt], L0, lofs, ws);
         } else {
           int hh[6], hp[6];
@@ -265,7 +316,7 @@ __global__ __launch_bounds__(256) void k_stage_fast(
           cm[tt][e] = ws[5][e];
         }
       }
-      if constexpr (PF) {
+      if constexpr (PF == 1) {
         if (H < He) {
           int hh[6], hp[6];
           step_planes(H + 1, hh, hp);
@@ -578,7 +629,7 @@ __global__ __launch_bounds__(256) void k_first16(
 
 static inline int pad8(long long n) { return (int)((n + 7) / 8 * 8); }
 
-template <int S, int POOL, bool PF>
+template <int S, int POOL, int PF>
 static int launch_stage_s(const float* in_n, const float* in_t, int n1,
                           int n2, bool ntk, float wv9, float b_var,
                           const float* q1, const float* q2, float* out_n,
@@ -601,11 +652,11 @@ static int launch_stage_s(const float* in_n, const float* in_t, int n1,
   return check_launch("nta_stage");
 }
 
-// Software prefetch of the next step's planes in k_stage_fast.
+// Plane fetch schedule of the 32x32 pooled stage (see k_stage_fast).
 #ifndef NTA_STAGE_PREFETCH
-#define NTA_STAGE_PREFETCH 1
+#define NTA_STAGE_PREFETCH 2
 #endif
-constexpr bool kPrefetch = NTA_STAGE_PREFETCH != 0;
+constexpr int kPrefetch = NTA_STAGE_PREFETCH;
 
 int launch_stage_fast(const float* in_nngp, const float* in_ntk, int n1,
                       int n2, int S, int pool, int flags, float w_var,
@@ -618,22 +669,22 @@ int launch_stage_fast(const float* in_nngp, const float* in_ntk, int n1,
       case 32: return launch_stage_s<32, 2, kPrefetch>(in_nngp, in_ntk, n1, n2, ntk, wv9,
                                             b_var, q1, q2, out_nngp, out_ntk,
                                             stream);
-      case 16: return launch_stage_s<16, 2, kPrefetch>(in_nngp, in_ntk, n1, n2, ntk, wv9,
+      case 16: return launch_stage_s<16, 2, 1>(in_nngp, in_ntk, n1, n2, ntk, wv9,
                                             b_var, q1, q2, out_nngp, out_ntk,
                                             stream);
-      case 8: return launch_stage_s<8, 2, kPrefetch>(in_nngp, in_ntk, n1, n2, ntk, wv9,
+      case 8: return launch_stage_s<8, 2, 1>(in_nngp, in_ntk, n1, n2, ntk, wv9,
                                           b_var, q1, q2, out_nngp, out_ntk,
                                           stream);
     }
   } else if (pool == 1) {
     switch (S) {
-      case 32: return launch_stage_s<32, 1, kPrefetch>(in_nngp, in_ntk, n1, n2, ntk, wv9,
+      case 32: return launch_stage_s<32, 1, 0>(in_nngp, in_ntk, n1, n2, ntk, wv9,
                                             b_var, q1, q2, out_nngp, out_ntk,
                                             stream);
-      case 16: return launch_stage_s<16, 1, kPrefetch>(in_nngp, in_ntk, n1, n2, ntk, wv9,
+      case 16: return launch_stage_s<16, 1, 0>(in_nngp, in_ntk, n1, n2, ntk, wv9,
                                             b_var, q1, q2, out_nngp, out_ntk,
                                             stream);
-      case 8: return launch_stage_s<8, 1, kPrefetch>(in_nngp, in_ntk, n1, n2, ntk, wv9,
+      case 8: return launch_stage_s<8, 1, 0>(in_nngp, in_ntk, n1, n2, ntk, wv9,
                                           b_var, q1, q2, out_nngp, out_ntk,
                                           stream);
     }

@@ -290,9 +290,6 @@ class AnalyticKernelFn:
     if diagonal_batch is False:
       raise NotImplementedError('diagonal_batch=False is not supported by '
                                 'the HIP engine.')
-    if diagonal_spatial:
-      raise NotImplementedError('diagonal_spatial=True is not supported by '
-                                'the HIP engine.')
     return_kernel, get_t = _canonicalize_get(get)
     for g in get_t:
       if g not in Kernel.__dataclass_fields__:
@@ -303,6 +300,12 @@ class AnalyticKernelFn:
       raise ValueError(f'x1 and x2 must agree apart from the batch axis, got '
                        f'{tuple(x1.shape)} and {tuple(x2t.shape)}.')
     plan = self.plan(x1.shape)
+    if diagonal_spatial and not plan.diag:
+      # The diagonal-spatial path runs when the network allows it (a Flatten
+      # readout and no pooling); other networks need the full spatial
+      # kernel.
+      raise NotImplementedError('diagonal_spatial=True needs a Flatten '
+                                'readout and no pooling layers.')
     want_nngp = True
     want_ntk = 'ntk' in get_t
     want_cov = return_kernel or 'cov1' in get_t or 'cov2' in get_t

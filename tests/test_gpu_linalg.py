@@ -1,0 +1,129 @@
+"""FP32 recursive Cholesky, its block preconditioner and the FP64 PCG solve
+of libnta.so against torch FP64 linear algebra (needs a GPU).
+
+The factor is FP32, so the preconditioner is checked to FP32 accuracy
+(relative error of (L L^T)^-1 R against the FP64 solve, scaled by the
+condition number); the PCG solution must match the FP64 solve to 1e-9.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from neural_tangents_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device('cuda', 0)
+
+
+def _spd(n, seed, rank=None, const=0.5, ridge=1e-2):
+  g = torch.Generator().manual_seed(seed)
+  r = rank or n
+  X = torch.randn(n, r, generator=g, dtype=torch.float64)
+  K = X @ X.T / r + const * torch.ones(n, n, dtype=torch.float64)
+  K += ridge * torch.eye(n, dtype=torch.float64)
+  return K.float()
+
+
+def _factor(K32):
+  n = K32.shape[0]
+  A = K32.clone().to(DEV)
+  W = torch.zeros_like(A)
+  info = torch.zeros((1,), dtype=torch.int32, device=DEV)
+  _lib.call('nta_chol_f32', _lib.ptr(A), n, n, _lib.ptr(W), n,
+            _lib.ptr(info), _lib.stream_handle(DEV))
+  torch.cuda.synchronize()
+  return W, int(info.item())
+
+
+@pytest.mark.parametrize('n', [64, 128, 192, 320, 640, 1024, 1216])
+def test_factor_blocks_match_fp64(n):
+  """W holds L11^-1, L22^-1 on the diagonal blocks and L21 below them."""
+  K = _spd(n, seed=n)
+  W, info = _factor(K)
+  assert info == 0
+  n1 = _lib.value('nta_chol_split', n)
+  L = torch.linalg.cholesky(K.double())
+  Wd = W.double().cpu()
+  Li = torch.linalg.inv(L)
+  tol = 2e-4
+  if n1 == n:
+    assert torch.allclose(Wd, Li, rtol=0, atol=tol * Li.abs().max())
+    return
+  for blk, want in ((Wd[:n1, :n1], Li[:n1, :n1]),
+                    (Wd[n1:, n1:], torch.linalg.inv(L[n1:, n1:])),
+                    (Wd[n1:, :n1], L[n1:, :n1])):
+    err = (blk - want).abs().max() / want.abs().max()
+    assert err <= tol, (n, float(err))
+  # the upper triangles stay zero
+  assert float(Wd[:n1, :n1].triu(1).abs().max()) == 0.0
+  assert float(Wd[n1:, n1:].triu(1).abs().max()) == 0.0
+
+
+@pytest.mark.parametrize('n,nrhs', [(64, 3), (640, 10), (1024, 16),
+                                    (1216, 5)])
+def test_preconditioner_solves_in_fp32(n, nrhs):
+  K = _spd(n, seed=3 * n)
+  W, _ = _factor(K)
+  n1 = _lib.value('nta_chol_split', n)
+  R = torch.randn(n, nrhs, generator=torch.Generator().manual_seed(1))
+  Rd = R.to(DEV)
+  Z = torch.empty_like(Rd)
+  ws = torch.empty((_lib.value('nta_pcg64_work_bytes', n, nrhs),),
+                   dtype=torch.uint8, device=DEV)
+  _lib.call('nta_chol_precond', _lib.ptr(W), n, n, n1, _lib.ptr(Rd), nrhs,
+            _lib.ptr(Z), _lib.ptr(ws), ws.numel(), _lib.stream_handle(DEV))
+  want = torch.linalg.solve(K.double(), R.double())
+  kappa = float(torch.linalg.cond(K.double()))
+  err = float((Z.double().cpu() - want).norm() / want.norm())
+  assert err <= 1e-6 * kappa, (err, kappa)
+
+
+@pytest.mark.parametrize('n,nrhs,lam', [(640, 10, 1e-3), (1024, 4, 1e-3),
+                                        (1216, 16, 1e-2)])
+def test_pcg_matches_fp64_solve(n, nrhs, lam):
+  K = _spd(n, seed=7 * n, rank=n // 2, const=1.0, ridge=0.0)
+  W, info = _factor(K + lam * torch.eye(n))
+  assert info == 0
+  n1 = _lib.value('nta_chol_split', n)
+  B = torch.randn(n, nrhs, dtype=torch.float64,
+                  generator=torch.Generator().manual_seed(2))
+  X = torch.empty((n, nrhs), dtype=torch.float64, device=DEV)
+  ws = torch.empty((_lib.value('nta_pcg64_work_bytes', n, nrhs),),
+                   dtype=torch.uint8, device=DEV)
+  iters, resid = ctypes.c_int(0), ctypes.c_double(0.0)
+  K_dev = K.to(DEV)
+  _lib.call('nta_pcg64', _lib.ptr(K_dev), n, n, lam, _lib.ptr(W), n, n1,
+            _lib.ptr(B.to(DEV)), nrhs, _lib.ptr(X), 1e-13, 100,
+            _lib.ptr(ws), ws.numel(), ctypes.byref(iters),
+            ctypes.byref(resid), _lib.stream_handle(DEV))
+  A = K.double() + lam * torch.eye(n, dtype=torch.float64)
+  want = torch.linalg.solve(A, B)
+  err = float((X.cpu() - want).norm() / want.norm())
+  # the recurrence residual reaches 1e-13; the recomputed one carries the
+  # FP64 rounding of the products (~ kappa * 1e-16)
+  assert resid.value <= 1e-10, resid.value
+  assert err <= 1e-9, (err, iters.value)
+  assert 1 <= iters.value < 100
+
+
+def test_factor_reports_non_positive_pivot():
+  n = 256
+  K = _spd(n, seed=5)
+  K[100, 100] = -1.0
+  _, info = _factor(K)
+  assert info == 101
+
+
+def test_invalid_arguments_raise():
+  W = torch.zeros((64, 64), device=DEV)
+  with pytest.raises(ValueError):
+    _lib.call('nta_chol_f32', _lib.ptr(W), 0, 64, _lib.ptr(W), 64, None,
+              None)
+  with pytest.raises(NotImplementedError):
+    _lib.call('nta_chol_f32', _lib.ptr(W), 60, 64, _lib.ptr(W), 64,
+              _lib.ptr(W), None)
+  assert _lib.value('nta_pcg64_work_bytes', 64, 17) == -1
+  assert _lib.value('nta_chol_split', 100) == 0

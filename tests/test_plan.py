@@ -56,3 +56,27 @@ def test_dense_chain_coefficients():
 def test_rejections(layers, exc):
   with pytest.raises(exc):
     _plan(*layers)
+
+
+def test_diagonal_spatial_plan_selection():
+  """Flatten readout without pooling -> diagonal-spatial plan (the
+  reference's diagonal_spatial requirement); any pooling or a GAP readout
+  needs the full spatial kernel."""
+  conv = lambda: stax.Conv(1, (3, 3), padding='SAME')
+  assert _plan(conv(), stax.Relu(), conv(), stax.Relu(), stax.Flatten(),
+               stax.Dense(1)).diag
+  assert not _plan(conv(), stax.Relu(), stax.AvgPool((2, 2), (2, 2)),
+                   stax.Flatten()).diag
+  assert not _plan(conv(), stax.Relu(), stax.GlobalAvgPool()).diag
+  assert not _plan(conv(), stax.Relu()).diag
+
+
+def test_diag_stage_tables():
+  p = _plan(stax.Conv(1, (3, 3), padding='SAME', W_std=1.5, b_std=0.5),
+            stax.Relu(), stax.Conv(1, (2, 2), padding='VALID'),
+            stax.Flatten())
+  desc, wb = _engine._diag_tables(p)
+  from neural_tangents_amd import _lib
+  assert list(desc) == [_lib.CONV | _lib.RELU, 3, 3,
+                        _lib.CONV | _lib.CONV_VALID, 2, 2]
+  assert list(wb) == [2.25, 0.25, 1.0, 0.0]
