@@ -71,11 +71,15 @@ int nta_first_diag(const float* x, int n, int H, int W, int C, int fh, int fw,
  * x1).  q1: [n1][Ho][Wo], q2: [n2][Ho][Wo] (layer-1 normalisers; unused
  * without NTA_RELU).  Outputs nngp, ntk: [T][Ho][Ho][Wo][Wo] with
  * T = n1*n2 (or n1 for a self tile); ntk is written only with NTA_NTK.
+ * act_k1, act_k0: the activation is ABRelu(a, b) with act_k1 = (a - b)^2,
+ * act_k0 = a b (Relu: 1, 0; LeakyRelu(alpha): (1 - alpha)^2, alpha; Abs:
+ * 4, -1), i.e. nngp' = act_k1 J + act_k0 c and dot_sigma' = act_k1
+ * dot_sigma + act_k0 with the ReLU map's J and dot_sigma.
  */
 int nta_first_stage(const float* x1, const float* x2, int n1, int n2, int H,
                     int W, int C, int fh, int fw, int flags, float w_var,
-                    float b_var, const float* q1, const float* q2,
-                    float* nngp, float* ntk, void* stream);
+                    float b_var, float act_k1, float act_k0, const float* q1,
+                    const float* q2, float* nngp, float* ntk, void* stream);
 
 /*
  * ReLU normaliser of a later stage from the self state of each image:
@@ -97,12 +101,14 @@ int nta_stage_diag(const float* R, int n, int H, int W, int fh, int fw,
  * _pool_kernel 3499-3572).
  * in_nngp, in_ntk: [T][H][H][W][W]; out: [T][Ho'][Ho'][Wo'][Wo'].
  * q1: [n1][Hc][Wc], q2: [n2][Hc][Wc] at the conv-output resolution
- * (q2 ignored for a self tile, n2 == 0).
+ * (q2 ignored for a self tile, n2 == 0).  act_k1, act_k0: the ABRelu
+ * constants, as in nta_first_stage.
  */
 int nta_stage(const float* in_nngp, const float* in_ntk, int n1, int n2,
               int H, int W, int fh, int fw, int ph, int pw, int flags,
-              float w_var, float b_var, const float* q1, const float* q2,
-              float* out_nngp, float* out_ntk, void* stream);
+              float w_var, float b_var, float act_k1, float act_k0,
+              const float* q1, const float* q2, float* out_nngp,
+              float* out_ntk, void* stream);
 
 /*
  * Readout head: GlobalAvgPool (or Flatten with NTA_FLATTEN) followed by the
@@ -131,10 +137,11 @@ int nta_head(const float* nngp, const float* ntk, int n1, int n2, int H,
  * d/db_var (gbv).  Pairs (a, b) with a + self_off == b (and every pair of a
  * self tile, n2 == 0) are pairs of identical images: their spatial diagonal
  * is treated as rho = 1 exactly.  Pass self_off = INT_MAX for none.
+ * act_k1, act_k0: the ABRelu constants, as in nta_first_stage.
  */
 int nta_relu_bwd(const float* aN, const float* aT, int n1, int n2, int S,
                  int pool, int self_off, float w_var, float b_var,
-                 const float* q1,
+                 float act_k1, float act_k0, const float* q1,
                  const float* q2, const float* gN, const float* gT,
                  float* gaN, float* gaT, float* gqa, float* gqb, float* gwv,
                  float* gbv, void* stream);
@@ -146,7 +153,8 @@ int nta_relu_bwd(const float* aN, const float* aT, int n1, int n2, int S,
  * (h = h', w = w') only, where the Conv stencil is an fh x fw window mean.
  * One launch runs every stage of every pair; states stay on chip.
  *   stage_desc (host) [nstage][3]: flags (NTA_CONV, NTA_CONV_VALID,
- *   NTA_RELU), fh, fw;  stage_wb (host) [nstage][2]: w_var, b_var.
+ *   NTA_RELU), fh, fw;  stage_wb (host) [nstage][4]: w_var, b_var and
+ *   the activation constants act_k1, act_k0 (see nta_stage).
  *   q1, q2: device [nstage][n][H][W] normaliser tables (a stage's values
  *   fill the first n * hout * wout floats of its slot).
  * Self mode (n2 == 0): the chains of x1's images against themselves; q1 is

@@ -34,6 +34,7 @@ from . import _lib
 class Stage:
   conv: Optional[tuple] = None     # (fh, fw, valid, w_var, b_var)
   relu: bool = False
+  act: Tuple[float, float] = (1.0, 0.0)   # ABRelu (k1, k0) = ((a-b)^2, ab)
   pool: Optional[tuple] = None     # (ph, pw)
   first: bool = False
   # sizes, filled by the planner
@@ -147,6 +148,7 @@ def compile_plan(specs, in_shape) -> Plan:
         close()
         cur = Stage()
       cur.relu = True
+      cur.act = ((spec.a - spec.b) ** 2, spec.a * spec.b)
       is_gaussian = False
       continue
     if isinstance(spec, S._AvgPoolSpec):
@@ -285,8 +287,8 @@ def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
       nt = 2 if fl_ntk else 1
       _launch('nta_first_stage', T * nt * 4 * out[0].numel(), T,
               _lib.ptr(xa), _lib.ptr(xb), n1, n2, H, W, C, fh, fw, flags,
-              w_var, b_var, _lib.ptr(q1), _lib.ptr(q2), _lib.ptr(out),
-              _lib.ptr(out_t), stream)
+              w_var, b_var, s.act[0], s.act[1], _lib.ptr(q1), _lib.ptr(q2),
+              _lib.ptr(out), _lib.ptr(out_t), stream)
       nngp, ntk = out, out_t
       if keep is not None:
         keep.append((nngp, ntk))
@@ -313,8 +315,8 @@ def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
     _launch(f'nta_stage[{s.hin}->{s.hout}]',
             T * nt * 4 * (nngp[0].numel() + out[0].numel()), T,
             _lib.ptr(nngp), _lib.ptr(ntk), n1, n2, s.hin, s.win, fh, fw, ph,
-            pw, flags, w_var, b_var, _lib.ptr(q1), _lib.ptr(q2),
-            _lib.ptr(out), _lib.ptr(out_t), stream)
+            pw, flags, w_var, b_var, s.act[0], s.act[1], _lib.ptr(q1),
+            _lib.ptr(q2), _lib.ptr(out), _lib.ptr(out_t), stream)
     nngp, ntk = out, out_t
     if keep is not None:
       keep.append((nngp, ntk))
@@ -380,7 +382,7 @@ def _diag_tables(plan: Plan):
   import ctypes
   n = len(plan.stages)
   desc = (ctypes.c_int * (3 * n))()
-  wb = (ctypes.c_float * (2 * n))()
+  wb = (ctypes.c_float * (4 * n))()
   for k, s in enumerate(plan.stages):
     f = 0
     fh = fw = 1
@@ -391,7 +393,7 @@ def _diag_tables(plan: Plan):
     if s.relu:
       f |= _lib.RELU
     desc[3 * k:3 * k + 3] = [f, fh, fw]
-    wb[2 * k:2 * k + 2] = [wv, bv]
+    wb[4 * k:4 * k + 4] = [wv, bv, s.act[0], s.act[1]]
   return desc, wb
 
 

@@ -32,11 +32,11 @@ _LL = ctypes.c_longlong
 # name -> argtypes; every entry point returns int status.
 SIGNATURES = {
     'nta_first_diag': [_P, _I, _I, _I, _I, _I, _I, _I, _F, _F, _P, _P],
-    'nta_first_stage': [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _F, _F, _P,
-                        _P, _P, _P, _P],
+    'nta_first_stage': [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _F, _F, _F,
+                        _F, _P, _P, _P, _P, _P],
     'nta_stage_diag': [_P, _I, _I, _I, _I, _I, _I, _F, _F, _P, _P],
-    'nta_stage': [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _F, _F, _P, _P,
-                  _P, _P, _P],
+    'nta_stage': [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _F, _F, _F, _F,
+                  _P, _P, _P, _P, _P],
     'nta_head': [_P, _P, _I, _I, _I, _I, _I, _D, _D, _D, _D, _D, _P, _P, _LL,
                  _P],
     'nta_chol_f32': [_P, _I, _LL, _P, _LL, _P, _P],
@@ -45,8 +45,8 @@ SIGNATURES = {
                   _P, _P, _P],
     'nta_diag_chain': [_P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P,
                        _I, _D, _D, _D, _D, _D, _P, _P, _LL, _P],
-    'nta_relu_bwd': [_P, _P, _I, _I, _I, _I, _I, _F, _F, _P, _P, _P, _P, _P,
-                     _P, _P, _P, _P, _P, _P],
+    'nta_relu_bwd': [_P, _P, _I, _I, _I, _I, _I, _F, _F, _F, _F, _P, _P, _P,
+                     _P, _P, _P, _P, _P, _P, _P, _P],
 }
 
 # Entry points that return a value instead of a status.

@@ -7,6 +7,9 @@
 //   s  = sqrt(max(qa qb - c^2, 0)), theta = atan2(s, c), ds = (pi-theta)/2pi
 //   J  = (s + (pi - theta) c) / 2pi,     TD = t ds
 //   nngp_out = pool(J), ntk_out = pool(TD)   (pool = mean over p x p x p x p)
+// For ABRelu(a, b) the map is J' = k1 J + k0 c, ds' = k1 ds + k0
+// (k1 = (a - b)^2, k0 = a b; elementwise.py:444-455), and J, ds below stand
+// for J', ds'.
 // Given the pooled cotangents gN, gT this writes, per element,
 //   gaN = w_var (gc + gt)   gaT = w_var gt        (cotangents of aN, aT)
 //   gqa, gqb                                     (cotangents of qa, qb)
@@ -27,7 +30,8 @@ namespace nta {
 __global__ void k_relu_bwd(const float* __restrict__ aN,
                            const float* __restrict__ aT, int n1, int n2,
                            int S, int pool, int self_off, float w_var,
-                           float b_var, const float* __restrict__ q1,
+                           float b_var, float k1, float k0,
+                           const float* __restrict__ q1,
                            const float* __restrict__ q2,
                            const float* __restrict__ gN,
                            const float* __restrict__ gT,
@@ -62,22 +66,22 @@ __global__ void k_relu_bwd(const float* __restrict__ aN,
     const float d = prod_minus_sq(qa, qb, c);
     const float s = same ? 0.0f : sqrtf(fmaxf(d, 0.0f));
     const float theta = (s == 0.0f && c == 0.0f) ? 0.5f * kPi : atan2f(s, c);
-    const float ds = (kPi - theta) * kInv2Pi;
+    const float ds = fmaf(k1, (kPi - theta) * kInv2Pi, k0);   // dot_sigma'
     const uint64_t o = p * per_o +
                        (((uint64_t)(h1 / pool) * So + h2 / pool) * So +
                         w1 / pool) * So + w2 / pool;
     const float gj = gN ? gN[o] * inv_pool : 0.0f;
     const float gtd = gT ? gT[o] * inv_pool : 0.0f;
-    // J derivatives: dJ/dc = ds, dJ/dq = s / (4 pi q)
-    float dj_qa = qa > 0.0f ? s * (0.25f / kPi) / qa : 0.0f;
-    float dj_qb = qb > 0.0f ? s * (0.25f / kPi) / qb : 0.0f;
-    // ds derivatives (zero where s = 0): d(ds)/dc = 1/(2 pi s),
-    // d(ds)/dq = -c / (4 pi s q)
+    // J' = k1 J + k0 c derivatives: dJ'/dc = ds', dJ'/dq = k1 s / (4 pi q)
+    float dj_qa = qa > 0.0f ? k1 * s * (0.25f / kPi) / qa : 0.0f;
+    float dj_qb = qb > 0.0f ? k1 * s * (0.25f / kPi) / qb : 0.0f;
+    // ds' derivatives (zero where s = 0): d(ds')/dc = k1 / (2 pi s),
+    // d(ds')/dq = -k1 c / (4 pi s q)
     float dd_c = 0.0f, dd_qa = 0.0f, dd_qb = 0.0f;
     if (s > 0.0f) {
-      dd_c = kInv2Pi / s;
-      dd_qa = qa > 0.0f ? -c * (0.25f / kPi) / (s * qa) : 0.0f;
-      dd_qb = qb > 0.0f ? -c * (0.25f / kPi) / (s * qb) : 0.0f;
+      dd_c = k1 * kInv2Pi / s;
+      dd_qa = qa > 0.0f ? -k1 * c * (0.25f / kPi) / (s * qa) : 0.0f;
+      dd_qb = qb > 0.0f ? -k1 * c * (0.25f / kPi) / (s * qb) : 0.0f;
     }
     const float gt = gtd * ds;
     const float gc = gj * ds + gtd * t * dd_c;
@@ -97,7 +101,7 @@ using namespace nta;
 
 extern "C" int nta_relu_bwd(const float* aN, const float* aT, int n1, int n2,
                             int S, int pool, int self_off, float w_var,
-                            float b_var,
+                            float b_var, float act_k1, float act_k0,
                             const float* q1, const float* q2, const float* gN,
                             const float* gT, float* gaN, float* gaT,
                             float* gqa, float* gqb, float* gwv, float* gbv,
@@ -116,7 +120,7 @@ extern "C" int nta_relu_bwd(const float* aN, const float* aT, int n1, int n2,
   if (g > (1u << 20)) g = 1u << 20;
   hipLaunchKernelGGL(k_relu_bwd, dim3((unsigned)g), dim3(256), 0,
                      (hipStream_t)stream, aN, aT, n1, n2, S, pool, self_off,
-                     w_var, b_var, q1, q2, gN, gT, gaN, gaT, gqa, gqb, gwv,
-                     gbv);
+                     w_var, b_var, act_k1, act_k0, q1, q2, gN, gT, gaN, gaT,
+                     gqa, gqb, gwv, gbv);
   return check_launch("nta_relu_bwd");
 }

@@ -21,12 +21,13 @@ int check_launch(const char* what);
 constexpr int kNotHandled = 1 << 30;
 int launch_stage_fast(const float* in_nngp, const float* in_ntk, int n1,
                       int n2, int S, int pool, int flags, float w_var,
-                      float b_var, const float* q1, const float* q2,
-                      float* out_nngp, float* out_ntk, hipStream_t stream);
+                      float b_var, float k1, float k0, const float* q1,
+                      const float* q2, float* out_nngp, float* out_ntk,
+                      hipStream_t stream);
 int launch_first_fast(const float* x1, const float* x2, int n1, int n2, int S,
-                      int C, int flags, float w_var, float b_var,
-                      const float* q1, const float* q2, float* nngp,
-                      float* ntk, hipStream_t stream);
+                      int C, int flags, float w_var, float b_var, float k1,
+                      float k0, const float* q1, const float* q2,
+                      float* nngp, float* ntk, hipStream_t stream);
 
 // One-instruction square root on the device (v_sqrt_f32, ~1 ulp); the host
 // pass uses the C library so the same code can be unit-tested on the CPU.
@@ -58,6 +59,20 @@ __device__ __forceinline__ float relu_map(float c, float q1, float q2,
   float pm = kPi - theta;
   *dot_sigma = pm * kInv2Pi;
   return fmaf(pm, c, s) * kInv2Pi;
+}
+
+// ABRelu(a, b) = a min(x, 0) + b max(x, 0) on kernels (elementwise.py:
+// 423-477) is (a - b)^2 Relu + a b Id: nngp' = k1 J + k0 c and
+// dot_sigma' = k1 dot_sigma + k0 with k1 = (a - b)^2, k0 = a b, where J and
+// dot_sigma are the ReLU map's.  Relu is k1 = 1, k0 = 0 (bitwise the ReLU
+// map).  LeakyRelu(alpha) = ABRelu(alpha, 1), Abs = ABRelu(-1, 1).
+__device__ __forceinline__ float abrelu_map(float c, float q1, float q2,
+                                            float k1, float k0,
+                                            float* dot_sigma) {
+  float d;
+  const float j = relu_map(c, q1, q2, &d);
+  *dot_sigma = fmaf(k1, d, k0);
+  return fmaf(k1, j, k0 * c);
 }
 
 // Normalised form of the same map, used by the fused kernels.  With

@@ -20,7 +20,7 @@ constexpr int kDiagMaxHW = 3840;       // 4 planes + reduction <= 64 KiB LDS
 
 struct DiagStage {
   int conv, fh, fw, lo_h, lo_w, hin, win, hout, wout, relu;
-  float w_var, b_var;
+  float w_var, b_var, k1, k0;
 };
 struct DiagPlan {
   int nstage, H, W, C, ntk;
@@ -121,7 +121,8 @@ __global__ __launch_bounds__(256) void k_diag_chain(
             qb = q2[si * P.qslot2 + (long long)j * nout + e];
           }
           float ds;
-          const float nv = relu_map(cv, qa, qb, &ds);   // elementwise.py:444-455
+          // elementwise.py:444-455
+          const float nv = abrelu_map(cv, qa, qb, S.k1, S.k0, &ds);
           tv *= ds;
           cv = nv;
         }
@@ -219,8 +220,10 @@ extern "C" int nta_diag_chain(const float* x1, const float* x2, int n1,
     S.relu = (f & NTA_RELU) ? 1 : 0;
     S.fh = S.conv ? stage_desc[3 * s + 1] : 1;
     S.fw = S.conv ? stage_desc[3 * s + 2] : 1;
-    S.w_var = stage_wb[2 * s];
-    S.b_var = stage_wb[2 * s + 1];
+    S.w_var = stage_wb[4 * s];
+    S.b_var = stage_wb[4 * s + 1];
+    S.k1 = stage_wb[4 * s + 2];
+    S.k0 = stage_wb[4 * s + 3];
     S.hin = h;
     S.win = w;
     if (S.fh <= 0 || S.fw <= 0)

@@ -658,10 +658,14 @@ static int launch_stage_s(const float* in_n, const float* in_t, int n1,
 #endif
 constexpr int kPrefetch = NTA_STAGE_PREFETCH;
 
+// The fused kernels implement the ReLU map only (k1 = 1, k0 = 0); other
+// ABRelu slopes run the generic kernels.
 int launch_stage_fast(const float* in_nngp, const float* in_ntk, int n1,
                       int n2, int S, int pool, int flags, float w_var,
-                      float b_var, const float* q1, const float* q2,
-                      float* out_nngp, float* out_ntk, hipStream_t stream) {
+                      float b_var, float k1, float k0, const float* q1,
+                      const float* q2, float* out_nngp, float* out_ntk,
+                      hipStream_t stream) {
+  if (k1 != 1.0f || k0 != 0.0f) return kNotHandled;
   const bool ntk = flags & NTA_NTK;
   const float wv9 = w_var / 9.0f;
   if (pool == 2) {
@@ -693,9 +697,10 @@ int launch_stage_fast(const float* in_nngp, const float* in_ntk, int n1,
 }
 
 int launch_first_fast(const float* x1, const float* x2, int n1, int n2, int S,
-                      int C, int flags, float w_var, float b_var,
-                      const float* q1, const float* q2, float* nngp,
-                      float* ntk, hipStream_t stream) {
+                      int C, int flags, float w_var, float b_var, float k1,
+                      float k0, const float* q1, const float* q2,
+                      float* nngp, float* ntk, hipStream_t stream) {
+  if (k1 != 1.0f || k0 != 0.0f) return kNotHandled;
   if (C > 4 || (S != 32 && S != 16)) return kNotHandled;
   const long long T = n2 == 0 ? (long long)n1 : (long long)n1 * n2;
   if (T * S > 0x7ffffff0LL)

@@ -92,7 +92,7 @@ __global__ void k_first_diag(const float* __restrict__ x, int n, int H, int W,
 __global__ void k_first_stage(const float* __restrict__ x1,
                               const float* __restrict__ x2, int n1, int n2,
                               int H, int W, int C, StencilGeom g, int flags,
-                              float scale, float b_var,
+                              float scale, float b_var, float k1, float k0,
                               const float* __restrict__ q1,
                               const float* __restrict__ q2,
                               float* __restrict__ nngp,
@@ -132,7 +132,7 @@ __global__ void k_first_stage(const float* __restrict__ x1,
       float qa = q1[((uint64_t)i * Hc + h1) * Wc + w1];
       float qb = (self ? q1 : q2)[((uint64_t)j * Hc + h2) * Wc + w2];
       float ds;
-      v = relu_map(v, qa, qb, &ds);
+      v = abrelu_map(v, qa, qb, k1, k0, &ds);
       t *= ds;
     }
     nngp[idx] = v;
@@ -175,8 +175,8 @@ __global__ void k_stage_diag(const float* __restrict__ R, int n, int H, int W,
 __global__ void k_stage(const float* __restrict__ in_nngp,
                         const float* __restrict__ in_ntk, int n1, int n2,
                         int H, int W, StencilGeom g, int ph, int pw, int flags,
-                        float scale, float w_var, float b_var,
-                        const float* __restrict__ q1,
+                        float scale, float w_var, float b_var, float k1,
+                        float k0, const float* __restrict__ q1,
                         const float* __restrict__ q2,
                         float* __restrict__ out_nngp,
                         float* __restrict__ out_ntk) {
@@ -235,7 +235,7 @@ __global__ void k_stage(const float* __restrict__ in_nngp,
               float qa = q1[((uint64_t)i * Hc + h1) * Wc + w1];
               float qb = (self ? q1 : q2)[((uint64_t)j * Hc + h2) * Wc + w2];
               float ds;
-              v = relu_map(v, qa, qb, &ds);
+              v = abrelu_map(v, qa, qb, k1, k0, &ds);
               t *= ds;
             }
             acc_n += v;
@@ -342,8 +342,9 @@ extern "C" int nta_first_diag(const float* x, int n, int H, int W, int C,
 extern "C" int nta_first_stage(const float* x1, const float* x2, int n1,
                                int n2, int H, int W, int C, int fh, int fw,
                                int flags, float w_var, float b_var,
-                               const float* q1, const float* q2, float* nngp,
-                               float* ntk, void* stream) {
+                               float act_k1, float act_k0, const float* q1,
+                               const float* q2, float* nngp, float* ntk,
+                               void* stream) {
   int rc = check_filter(H, W, fh, fw, flags, "nta_first_stage");
   if (rc) return rc;
   if (n1 < 0 || n2 < 0 || C <= 0)
@@ -356,8 +357,9 @@ extern "C" int nta_first_stage(const float* x1, const float* x2, int n1,
                      "requested operand");
   if ((flags & NTA_RELU) && !(flags & NTA_CONV_VALID) && fh == 3 &&
       fw == 3 && H == W) {
-    int rc2 = launch_first_fast(x1, x2, n1, n2, H, C, flags, w_var, b_var, q1,
-                                q2, nngp, ntk, (hipStream_t)stream);
+    int rc2 = launch_first_fast(x1, x2, n1, n2, H, C, flags, w_var, b_var,
+                                act_k1, act_k0, q1, q2, nngp, ntk,
+                                (hipStream_t)stream);
     if (rc2 != kNotHandled) return rc2;
   }
   StencilGeom g = make_geom(H, W, fh, fw, flags);
@@ -366,7 +368,7 @@ extern "C" int nta_first_stage(const float* x1, const float* x2, int n1,
   uint64_t total = T * g.Hc * g.Hc * g.Wc * g.Wc;
   hipLaunchKernelGGL(k_first_stage, dim3(grid_for(total, 256)), dim3(256), 0,
                      (hipStream_t)stream, x1, x2, n1, n2, H, W, C, g, flags,
-                     scale, b_var, q1, q2, nngp, ntk);
+                     scale, b_var, act_k1, act_k0, q1, q2, nngp, ntk);
   return check_launch("nta_first_stage");
 }
 
@@ -395,9 +397,9 @@ extern "C" int nta_stage_diag(const float* R, int n, int H, int W, int fh,
 
 extern "C" int nta_stage(const float* in_nngp, const float* in_ntk, int n1,
                          int n2, int H, int W, int fh, int fw, int ph, int pw,
-                         int flags, float w_var, float b_var, const float* q1,
-                         const float* q2, float* out_nngp, float* out_ntk,
-                         void* stream) {
+                         int flags, float w_var, float b_var, float act_k1,
+                         float act_k0, const float* q1, const float* q2,
+                         float* out_nngp, float* out_ntk, void* stream) {
   int rc = check_filter(H, W, fh, fw, flags, "nta_stage");
   if (rc) return rc;
   if (n1 < 0 || n2 < 0 || ph <= 0 || pw <= 0)
@@ -417,8 +419,8 @@ extern "C" int nta_stage(const float* in_nngp, const float* in_ntk, int n1,
   if ((flags & NTA_CONV) && (flags & NTA_RELU) && !(flags & NTA_CONV_VALID) &&
       fh == 3 && fw == 3 && H == W && ph == pw) {
     int rc2 = launch_stage_fast(in_nngp, in_ntk, n1, n2, H, ph, flags, w_var,
-                                b_var, q1, q2, out_nngp, out_ntk,
-                                (hipStream_t)stream);
+                                b_var, act_k1, act_k0, q1, q2, out_nngp,
+                                out_ntk, (hipStream_t)stream);
     if (rc2 != kNotHandled) return rc2;
   }
   float scale = 1.0f / (float)(fh * fw);
@@ -426,8 +428,8 @@ extern "C" int nta_stage(const float* in_nngp, const float* in_ntk, int n1,
   uint64_t total = T * Ho * Ho * Wo * Wo;
   hipLaunchKernelGGL(k_stage, dim3(grid_for(total, 256)), dim3(256), 0,
                      (hipStream_t)stream, in_nngp, in_ntk, n1, n2, H, W, g,
-                     ph, pw, flags, scale, w_var, b_var, q1, q2, out_nngp,
-                     out_ntk);
+                     ph, pw, flags, scale, w_var, b_var, act_k1, act_k0, q1,
+                     q2, out_nngp, out_ntk);
   return check_launch("nta_stage");
 }
 

@@ -35,7 +35,7 @@ def _stencil(x, n, S, fh, fw, valid, stream):
   out = torch.empty_like(x)
   flags = _lib.CONV | (_lib.CONV_VALID if valid else 0)
   _lib.call('nta_stage', _lib.ptr(x), None, n, 0, S, S, fh, fw, 1, 1, flags,
-            1.0, 0.0, None, None, _lib.ptr(out), None, stream)
+            1.0, 0.0, 1.0, 0.0, None, None, _lib.ptr(out), None, stream)
   return out
 
 
@@ -68,7 +68,7 @@ NO_SELF = 2 ** 31 - 1
 
 
 def _relu_bwd(aN, aT, n1, n2, S, pool, w_var, b_var, q1, q2, gN, gT,
-              stream, self_off=NO_SELF):
+              stream, self_off=NO_SELF, act=(1.0, 0.0)):
   T = aN.shape[0]
   gaN = torch.empty_like(aN)
   gaT = torch.empty_like(aN) if aT is not None else None
@@ -77,7 +77,8 @@ def _relu_bwd(aN, aT, n1, n2, S, pool, w_var, b_var, q1, q2, gN, gT,
   gwv = torch.empty_like(aN)
   gbv = torch.empty_like(aN)
   _lib.call('nta_relu_bwd', _lib.ptr(aN), _lib.ptr(aT), n1, n2, S, pool,
-            self_off, w_var, b_var, _lib.ptr(q1), _lib.ptr(q2), _lib.ptr(gN),
+            self_off, w_var, b_var, act[0], act[1], _lib.ptr(q1),
+            _lib.ptr(q2), _lib.ptr(gN),
             _lib.ptr(gT), _lib.ptr(gaN), _lib.ptr(gaT), _lib.ptr(gqa),
             _lib.ptr(gqb), _lib.ptr(gwv), _lib.ptr(gbv), stream)
   v = lambda t: t.reshape(T, S, S, S, S)   # (h, h', w, w')
@@ -173,7 +174,7 @@ def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=64):
           aT = None
         gaN, gaT, gqa, gqb, dw, db = _relu_bwd(
             aN, aT, n1, n2, S, pool, w_var, b_var, qa[si], qb[si], gN, gT,
-            stream, self_off=(i0 - j0) if sym else NO_SELF)
+            stream, self_off=(i0 - j0) if sym else NO_SELF, act=s.act)
         gwv[si] += dw
         gbv[si] += db
         gq1[si][i0:i0 + n1] += gqa.reshape(n1, n2, S, S).sum(1)
@@ -204,7 +205,8 @@ def _first_gram(xa, xb, n1, n2, S, C, fh, fw, stream):
   out = torch.empty((T, S, S, S, S), dtype=torch.float32, device=xa.device)
   H, W = xa.shape[1], xa.shape[2]
   _lib.call('nta_first_stage', _lib.ptr(xa), _lib.ptr(xb), n1, n2, H, W, C,
-            fh, fw, 0, 1.0, 0.0, None, None, _lib.ptr(out), None, stream)
+            fh, fw, 0, 1.0, 0.0, 1.0, 0.0, None, None, _lib.ptr(out), None,
+            stream)
   return out
 
 
@@ -232,7 +234,8 @@ def _self_chain_vjp(plan, x, gq, gx, gwv, gbv, stream, C, fh, fw,
       else:
         aN = _first_gram(xa, None, n, 0, S, C, fh, fw, stream)
       gaN, _, gqa, gqb, dw, db = _relu_bwd(aN, None, n, 0, S, pool, w_var,
-                                           b_var, q, None, gR, None, stream)
+                                           b_var, q, None, gR, None, stream,
+                                           act=s.act)
       gq_tot = gq[si][n0:n0 + n] + gqa + gqb                 # [n, S, S]
       # q = diag(sigma), sigma = w_var aN + b_var
       v = gaN.reshape(n, S, S, S, S)

@@ -47,6 +47,9 @@ class _ConvSpec:
 
 @dataclasses.dataclass(frozen=True)
 class _ReluSpec:
+  """ABRelu(a, b) = a min(x, 0) + b max(x, 0); Relu is (0, 1)."""
+  a: float = 0.0
+  b: float = 1.0
   name: str = 'Relu'
 
 
@@ -155,7 +158,11 @@ def _dense_layer_fns(spec: _DenseSpec):
 
 
 def _relu_layer_fns(spec):
-  return (lambda rng, s: (tuple(s), ())), (lambda p, x, **kw: torch.relu(x))
+  a, b = spec.a, spec.b
+
+  def apply_fn(p, x, **kw):
+    return a * torch.clamp(x, max=0) + b * torch.clamp(x, min=0)
+  return (lambda rng, s: (tuple(s), ())), apply_fn
 
 
 def _avg_pool_layer_fns(spec: _AvgPoolSpec):
@@ -378,10 +385,26 @@ def Conv(out_chan: int, filter_shape: Sequence[int],
 
 
 def Relu(do_stabilize: bool = False):
-  """``Relu`` (``elementwise.py:482-491``)."""
-  if do_stabilize:
-    raise NotImplementedError('do_stabilize=True is not supported.')
+  """``Relu`` (``elementwise.py:482-491``).  ``do_stabilize`` rescales the
+  reference's intermediate kernels only; the map is positively homogeneous,
+  so the result is the same and the flag is accepted and ignored."""
   return _layer(_ReluSpec())
+
+
+def ABRelu(a: float, b: float, do_stabilize: bool = False):
+  """``ABRelu`` (``elementwise.py:405-479``): ``a min(x, 0) + b max(x, 0)``;
+  on kernels ``(a - b)^2 Relu + a b Id``."""
+  return _layer(_ReluSpec(float(a), float(b), name=f'ABReLU({a}, {b})'))
+
+
+def LeakyRelu(alpha: float, do_stabilize: bool = False):
+  """``LeakyRelu`` (``elementwise.py:494-506``) = ``ABRelu(alpha, 1)``."""
+  return _layer(_ReluSpec(float(alpha), 1.0, name=f'LeakyRelu({alpha})'))
+
+
+def Abs(do_stabilize: bool = False):
+  """``Abs`` (``elementwise.py:509-520``) = ``ABRelu(-1, 1)``."""
+  return _layer(_ReluSpec(-1.0, 1.0, name='Abs'))
 
 
 def AvgPool(window_shape: Sequence[int],

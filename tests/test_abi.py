@@ -45,6 +45,6 @@ def test_invalid_arguments_return_einval_without_gpu_work():
   rc = lib.nta_first_diag(None, -1, 8, 8, 3, 3, 3, 0, 1.0, 0.0, None, None)
   assert rc == _lib.NTA_EINVAL
   assert b'nta_first_diag' in lib.nta_last_error()
-  rc = lib.nta_stage(None, None, 1, 1, 8, 8, 3, 3, 0, 1, 0, 1.0, 0.0, None,
-                     None, None, None, None)
+  rc = lib.nta_stage(None, None, 1, 1, 8, 8, 3, 3, 0, 1, 0, 1.0, 0.0, 1.0,
+                     0.0, None, None, None, None, None)
   assert rc == _lib.NTA_EINVAL

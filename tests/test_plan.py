@@ -73,10 +73,20 @@ def test_diagonal_spatial_plan_selection():
 
 def test_diag_stage_tables():
   p = _plan(stax.Conv(1, (3, 3), padding='SAME', W_std=1.5, b_std=0.5),
-            stax.Relu(), stax.Conv(1, (2, 2), padding='VALID'),
+            stax.LeakyRelu(0.25), stax.Conv(1, (2, 2), padding='VALID'),
             stax.Flatten())
   desc, wb = _engine._diag_tables(p)
   from neural_tangents_amd import _lib
   assert list(desc) == [_lib.CONV | _lib.RELU, 3, 3,
                         _lib.CONV | _lib.CONV_VALID, 2, 2]
-  assert list(wb) == [2.25, 0.25, 1.0, 0.0]
+  # LeakyRelu(alpha) = ABRelu(alpha, 1): k1 = (1 - alpha)^2, k0 = alpha
+  assert list(wb) == [2.25, 0.25, 0.5625, 0.25, 1.0, 0.0, 1.0, 0.0]
+
+
+@pytest.mark.parametrize('layer,act', [(stax.Relu(), (1.0, 0.0)),
+                                       (stax.Abs(), (4.0, -1.0)),
+                                       (stax.ABRelu(-0.5, 2.0), (6.25, -1.0)),
+                                       (stax.LeakyRelu(0.1), (0.81, 0.1))])
+def test_activation_constants(layer, act):
+  p = _plan(stax.Conv(1, (3, 3), padding='SAME'), layer, stax.GlobalAvgPool())
+  assert p.stages[0].act == pytest.approx(act)
