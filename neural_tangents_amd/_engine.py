@@ -35,6 +35,7 @@ class Stage:
   conv: Optional[tuple] = None     # (fh, fw, valid, w_var, b_var)
   relu: bool = False
   act: Tuple[float, float] = (1.0, 0.0)   # ABRelu (k1, k0) = ((a-b)^2, ab)
+  wscale: float = 1.0              # SumPool factor folded into conv W_std^2
   pool: Optional[tuple] = None     # (ph, pw)
   first: bool = False
   # sizes, filled by the planner
@@ -70,6 +71,9 @@ class Plan:
   # Flatten readout and no pooling: only the spatial diagonal is needed
   # (the reference's diagonal_spatial requirement, requirements.py:912-1054).
   diag: bool = False
+  # Factor applied to head-less outputs (SumPool scales left after the last
+  # layer that could absorb them).
+  out_scale: float = 1.0
 
 
 class _Unsupported(NotImplementedError):
@@ -91,6 +95,13 @@ def compile_plan(specs, in_shape) -> Plan:
   is_gaussian = False
   n_convs = 0
   chans = C
+  # SumPool = AvgPool x window^2 on kernels.  Every layer here is linear or
+  # positively homogeneous of degree 1 in (nngp, ntk, cov) jointly (the
+  # ABRelu map is), so the factor is carried forward and folded into the
+  # next Conv's W_std^2 (which scales before b_std^2 is added), into the
+  # readout coefficients, or into the outputs.
+  pend = 1.0
+  gsum = False
 
   def close():
     nonlocal cur
@@ -126,10 +137,13 @@ def compile_plan(specs, in_shape) -> Plan:
                                   'supported by the HIP engine.')
       close()
       fh, fw = spec.filter_shape
-      w_var = float(spec.W_std) ** 2
+      w_var = float(spec.W_std) ** 2 * pend
+      pend = 1.0
       b_var = 0.0 if spec.b_std is None else float(spec.b_std) ** 2
       cur = Stage(conv=(fh, fw, spec.padding == 'VALID', w_var, b_var),
-                  first=not stages and n_convs == 0)
+                  first=not stages and n_convs == 0,
+                  wscale=w_var / float(spec.W_std) ** 2
+                  if spec.W_std != 0 else 1.0)
       if not cur.first and not stages:
         raise NotImplementedError('The network must start with a Conv.')
       n_convs += 1
@@ -163,10 +177,13 @@ def compile_plan(specs, in_shape) -> Plan:
         close()
         cur = Stage()
       cur.pool = (spec.padding,) + win
+      if spec.sum:
+        pend *= float(win[0] * win[1]) ** 2
       continue
     if isinstance(spec, (S._GlobalAvgPoolSpec, S._FlattenSpec)):
       close()
       head = 'gap' if isinstance(spec, S._GlobalAvgPoolSpec) else 'flatten'
+      gsum = head == 'gap' and spec.sum
       if isinstance(spec, S._FlattenSpec):
         is_gaussian = False
       continue
@@ -212,6 +229,11 @@ def compile_plan(specs, in_shape) -> Plan:
   # ntk = c m + d mt + e (linear.py:902-926 applied in sequence).
   a, b, c, d, e = 1.0, 0.0, 1.0, 0.0, 0.0   # nngp = m, ntk = mt
   ntk_m, ntk_t, ntk_1 = 0.0, 1.0, 0.0
+  if head is not None:
+    # m is the mean over the readout's entries; GlobalSumPool sums them.
+    a = pend * (float(h * h * w * w) if gsum else 1.0)
+    ntk_t = a
+    pend = 1.0
   for spec in dense:
     wv = float(spec.W_std) ** 2
     bv = 0.0 if spec.b_std is None else float(spec.b_std) ** 2
@@ -221,7 +243,7 @@ def compile_plan(specs, in_shape) -> Plan:
   diag = head == 'flatten' and all(s.pool is None for s in stages)
   return Plan(stages=stages, head=head, head_coef=coef, n_convs=n_convs,
               is_gaussian=is_gaussian, out_channels=chans,
-              out_spatial=(h, w), diag=diag)
+              out_spatial=(h, w), diag=diag, out_scale=pend)
 
 
 # ------------------------------------------------------------------ runner
@@ -500,6 +522,10 @@ def run(plan: Plan, x1, x2, want_nngp, want_ntk, want_cov,
       _symmetrize_from_upper(ntk)
   out['nngp'] = nngp
   out['ntk'] = ntk
+  if plan.out_scale != 1.0:
+    for k in ('nngp', 'ntk', 'cov1', 'cov2'):
+      if out[k] is not None:
+        out[k].mul_(plan.out_scale)
   return out
 
 

@@ -92,6 +92,8 @@ def _check_plan(plan):
   if plan.head is None:
     raise NotImplementedError('kernel gradients need a readout head '
                               '(GlobalAvgPool or Flatten).')
+  if plan.out_scale != 1.0 or any(s.wscale != 1.0 for s in plan.stages):
+    raise NotImplementedError('kernel gradients do not support SumPool.')
   for s in plan.stages:
     if s.conv is None or not s.relu:
       raise NotImplementedError('kernel gradients support stages of Conv + '

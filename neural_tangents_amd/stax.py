@@ -12,7 +12,8 @@ kernel_fn)`` (``utils/typing.py:225-229``):
   (``kernel_fn_any`` 955-1051): ``get=None`` returns a ``Kernel``, a string
   returns that tensor, a tuple returns an ``AnalyticKernel`` namedtuple.
 
-Layers: ``Conv``, ``Relu``, ``AvgPool``, ``GlobalAvgPool``, ``Dense``,
+Layers: ``Conv``, ``Relu``, ``LeakyRelu``, ``Abs``, ``ABRelu``,
+``AvgPool``, ``SumPool``, ``GlobalAvgPool``, ``GlobalSumPool``, ``Dense``,
 ``Flatten``, ``Identity``, combined with ``serial``.  Networks outside what
 the engine fuses raise ``NotImplementedError``; there is no CPU path.
 """
@@ -60,11 +61,13 @@ class _AvgPoolSpec:
   padding: str
   normalize_edges: bool
   name: str = 'AvgPool'
+  sum: bool = False                # SumPool: no 1 / window normalisation
 
 
 @dataclasses.dataclass(frozen=True)
 class _GlobalAvgPoolSpec:
   name: str = 'GlobalAvgPool'
+  sum: bool = False                # GlobalSumPool
 
 
 @dataclasses.dataclass(frozen=True)
@@ -192,6 +195,8 @@ def _avg_pool_layer_fns(spec: _AvgPoolSpec):
         y = y / cnt
     else:
       y = F.avg_pool2d(x, (wh, ww), (sh, sw))
+    if spec.sum:
+      y = y * (wh * ww)
     return y.permute(0, 2, 3, 1)
   return init_fn, apply_fn
 
@@ -199,7 +204,11 @@ def _avg_pool_layer_fns(spec: _AvgPoolSpec):
 def _gap_layer_fns(spec):
   def init_fn(rng, input_shape):
     return (input_shape[0], input_shape[-1]), ()
-  return init_fn, (lambda p, x, **kw: x.mean(dim=tuple(range(1, x.ndim - 1))))
+
+  def apply_fn(p, x, **kw):
+    axes = tuple(range(1, x.ndim - 1))
+    return x.sum(dim=axes) if spec.sum else x.mean(dim=axes)
+  return init_fn, apply_fn
 
 
 def _flatten_layer_fns(spec):
@@ -421,11 +430,34 @@ def AvgPool(window_shape: Sequence[int],
                              bool(normalize_edges)))
 
 
+def SumPool(window_shape: Sequence[int],
+            strides: Optional[Sequence[int]] = None, padding: str = 'VALID',
+            batch_axis: int = 0, channel_axis: int = -1):
+  """``SumPool`` (``linear.py:1505-1539``): on kernels, AvgPool times the
+  square of the window size (``_pool_kernel`` without ``_normalize``,
+  3499-3572)."""
+  if batch_axis != 0 or channel_axis not in (-1, 3):
+    raise NotImplementedError('Only NHWC pooling is supported.')
+  window_shape = tuple(int(w) for w in window_shape)
+  strides = tuple(strides) if strides is not None else (1,) * len(
+      window_shape)
+  return _layer(_AvgPoolSpec(window_shape, strides, padding.upper(), False,
+                             name='SumPool', sum=True))
+
+
 def GlobalAvgPool(batch_axis: int = 0, channel_axis: int = -1):
   """``GlobalAvgPool`` (``linear.py:1701-1721``)."""
   if batch_axis != 0 or channel_axis not in (-1, 3):
     raise NotImplementedError('Only NHWC global pooling is supported.')
   return _layer(_GlobalAvgPoolSpec())
+
+
+def GlobalSumPool(batch_axis: int = 0, channel_axis: int = -1):
+  """``GlobalSumPool`` (``linear.py:1676-1696``): the spatial sum, i.e.
+  GlobalAvgPool times the number of (h, h', w, w') entries."""
+  if batch_axis != 0 or channel_axis not in (-1, 3):
+    raise NotImplementedError('Only NHWC global pooling is supported.')
+  return _layer(_GlobalAvgPoolSpec(name='GlobalSumPool', sum=True))
 
 
 def Dense(out_dim: int, W_std: float = 1., b_std: Optional[float] = None,

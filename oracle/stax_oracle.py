@@ -358,12 +358,15 @@ def _reduce_window_sum(x, window, strides, padding):
 
 
 def pool_kernel(lhs, window_shape, strides, padding, normalize_edges,
-                batch_ndim):
+                batch_ndim, pool_sum=False):
   """``_pool_kernel`` ``linear.py:3499-3559`` + ``_normalize`` 3562-3572
-  (average pooling)."""
+  (average pooling; ``pool_sum``: the SUM pooling type, no ``_normalize``,
+  3551-3556)."""
   window = (1,) * batch_ndim + tuple(window_shape)
   st = (1,) * batch_ndim + tuple(strides)
   out = _reduce_window_sum(lhs, window, st, padding)
+  if pool_sum:
+    return out
   if padding == 'SAME' and normalize_edges:
     ones = np.ones_like(lhs)
     out = out / _reduce_window_sum(ones, window, st, padding)
@@ -373,8 +376,10 @@ def pool_kernel(lhs, window_shape, strides, padding, normalize_edges,
 
 
 def avg_pool_layer(window_shape=(2, 2), strides=None, padding='VALID',
-                   normalize_edges=False):
-  """``AvgPool`` ``linear.py:1459-1500`` -> ``_Pool.kernel_fn`` 1634-1664."""
+                   normalize_edges=False, pool_sum=False):
+  """``AvgPool`` ``linear.py:1459-1500`` -> ``_Pool.kernel_fn`` 1634-1664
+  (``pool_sum``: ``SumPool`` 1505-1539, the same kernel_fn with the SUM
+  pooling type)."""
   window_shape = tuple(window_shape)
   strides = tuple(strides) if strides is not None else (1,) * len(
       window_shape)
@@ -390,8 +395,8 @@ def avg_pool_layer(window_shape=(2, 2), strides=None, padding='VALID',
     def pool(mat, bnd):
       if mat is None or np.ndim(mat) == 0:
         return mat
-      out = pool_kernel(mat, wk, sk, padding, normalize_edges, bnd)
-      if k.diagonal_spatial:                  # 1651-1655
+      out = pool_kernel(mat, wk, sk, padding, normalize_edges, bnd, pool_sum)
+      if k.diagonal_spatial and not pool_sum:  # 1651-1655
         out = pool_kernel_diag_normalize(mat, out, normalize_edges, padding,
                                          (1,) * bnd + tuple(strides),
                                          (1,) * bnd + tuple(window_shape))
@@ -411,16 +416,22 @@ def pool_kernel_diag_normalize(lhs, out, normalize_edges, padding, strides,
   return out / lhs.dtype.type(np.prod(window))
 
 
-def global_avg_pool_layer():
+def sum_pool_layer(window_shape=(2, 2), strides=None, padding='VALID'):
+  """``SumPool`` ``linear.py:1505-1539``."""
+  return avg_pool_layer(window_shape, strides, padding, False, pool_sum=True)
+
+
+def global_avg_pool_layer(pool_sum=False):
   """``GlobalAvgPool`` ``linear.py:1701-1721`` -> ``_GlobalPool.kernel_fn``
-  1774-1801 (``mean_and_var`` without masks)."""
+  1774-1801 (``mean_and_var`` without masks; ``pool_sum``:
+  ``GlobalSumPool`` 1676-1696, ``jnp.sum`` at 1751)."""
   def kernel_fn(k: OKernel) -> OKernel:
     def _pool(mat, bnd):
       if mat is None or np.ndim(mat) == 0:
         return mat
       axes = tuple(range(bnd, mat.ndim))
-      out = np.mean(mat, axis=axes)
-      if k.diagonal_spatial:
+      out = np.sum(mat, axis=axes) if pool_sum else np.mean(mat, axis=axes)
+      if k.diagonal_spatial and not pool_sum:
         out = out / int(np.prod([mat.shape[a] for a in axes]))
       return out
     return k.replace(cov1=_pool(k.cov1, 1), cov2=_pool(k.cov2, 1),

@@ -91,3 +91,33 @@ def test_leaky_relu_kernel_gradient_matches_finite_differences():
   h = 1e-3
   fd = (f(x2 + h * v) - f(x2 - h * v)) / (2 * h)
   assert abs(float(np.sum(g * v)) - fd) <= 1e-4 * np.linalg.norm(g)
+
+
+def test_sum_pools_match_oracle():
+  """SumPool / GlobalSumPool through the AvgPool kernels with the window
+  factors folded into the next Conv, the readout and the outputs."""
+  x1, x2 = blurred_images([3, 2], 16, seed=9)
+  conv = lambda: stax.Conv(1, (3, 3), padding='SAME', W_std=1.1, b_std=0.2)
+  oconv = lambda: so.conv_layer((3, 3), (1, 1), 'SAME', 1.1, 0.2)
+  ours = [conv(), stax.Relu(), conv(), stax.Relu(),
+          stax.SumPool((2, 2), (2, 2)), conv(), stax.Relu(),
+          stax.AvgPool((2, 2), (2, 2)), stax.GlobalSumPool(),
+          stax.Dense(1, 0.5, 0.1)]
+  ref = [oconv(), so.relu_layer(), oconv(), so.relu_layer(),
+         so.sum_pool_layer((2, 2), (2, 2)), oconv(), so.relu_layer(),
+         so.avg_pool_layer((2, 2), (2, 2)),
+         so.global_avg_pool_layer(pool_sum=True), so.dense_layer(0.5, 0.1)]
+  got = stax.serial(*ours)[2](x1, x2, ('nngp', 'ntk'))
+  want = so.kernel_fn(ref, x1.astype(np.float64), x2.astype(np.float64),
+                      ('nngp', 'ntk'))
+  for g in ('nngp', 'ntk'):
+    assert rel_err(getattr(got, g).double().cpu().numpy(), want[g]) <= 1e-5
+  # a head-less network ending in SumPool returns the scaled state
+  ours = [conv(), stax.Relu(), stax.SumPool((2, 2), (2, 2))]
+  ref = [oconv(), so.relu_layer(), so.sum_pool_layer((2, 2), (2, 2))]
+  k = stax.serial(*ours)[2](x1[:2], x2[:1], None)
+  want = so.kernel_fn(ref, x1[:2].astype(np.float64),
+                      x2[:1].astype(np.float64), None)
+  assert rel_err(k.nngp.double().cpu().numpy(), want.nngp) <= 1e-5
+  assert rel_err(k.ntk.double().cpu().numpy(), want.ntk) <= 1e-5
+  assert rel_err(k.cov1.double().cpu().numpy(), want.cov1) <= 1e-5

@@ -235,6 +235,37 @@ def test_valid_avg_pool_is_block_mean():
   np.testing.assert_allclose(got, want, rtol=1e-13)
 
 
+def test_sum_pools_are_window_sums():
+  """SumPool is _pool_kernel without _normalize (linear.py:3551-3556):
+  brute-force window sums of a random kernel, SAME padding included;
+  GlobalSumPool is the plain sum (linear.py:1751)."""
+  rng = np.random.default_rng(5)
+  x = rng.standard_normal((2, 3, 4, 4, 6, 6))
+  got = so.pool_kernel(x, (2, 2, 2, 2), (2, 2, 2, 2), 'VALID', False, 2,
+                       pool_sum=True)
+  want = np.zeros((2, 3, 2, 2, 3, 3))
+  for a in range(2):
+    for b in range(2):
+      for c in range(3):
+        for d in range(3):
+          want[:, :, a, b, c, d] = x[:, :, 2 * a:2 * a + 2, 2 * b:2 * b + 2,
+                                     2 * c:2 * c + 2, 2 * d:2 * d + 2].sum(
+                                         axis=(2, 3, 4, 5))
+  np.testing.assert_allclose(got, want, rtol=1e-13)
+  # SAME 3x3 / 1: zero padded sums
+  y = rng.standard_normal((1, 1, 3, 3))
+  got = so.pool_kernel(y, (3, 3), (1, 1), 'SAME', False, 2, pool_sum=True)
+  yp = np.pad(y, ((0, 0), (0, 0), (1, 1), (1, 1)))
+  want = np.array([[yp[0, 0, i:i + 3, j:j + 3].sum() for j in range(3)]
+                   for i in range(3)])
+  np.testing.assert_allclose(got[0, 0], want, rtol=1e-13)
+  k = so.OKernel(nngp=x, ntk=None, cov1=x[:, 0], cov2=None, x1_is_x2=False,
+                 is_gaussian=True, is_reversed=False, diagonal_batch=True,
+                 diagonal_spatial=False, shape1=None, shape2=None)
+  out = so.global_avg_pool_layer(pool_sum=True)(k)
+  np.testing.assert_allclose(out.nngp, x.sum(axis=(2, 3, 4, 5)), rtol=1e-13)
+
+
 def test_gap_flatten_readout_equivalence():
   """Notebook readout [AvgPool x3 -> 1x1, Flatten, Dense(sqrt 2)] equals
   [AvgPool -> 4x4, GlobalAvgPool, Dense(1)] times W^2 = 2 (both are the

@@ -90,3 +90,22 @@ def test_diag_stage_tables():
 def test_activation_constants(layer, act):
   p = _plan(stax.Conv(1, (3, 3), padding='SAME'), layer, stax.GlobalAvgPool())
   assert p.stages[0].act == pytest.approx(act)
+
+
+def test_sum_pool_scale_folding():
+  """SumPool = AvgPool x window^2 on kernels: folded into the next Conv's
+  W_std^2, the readout, or the outputs; GlobalSumPool multiplies the mean
+  by the number of (h, h', w, w') entries."""
+  conv = lambda: stax.Conv(1, (3, 3), padding='SAME', W_std=2.0)
+  p = _plan(conv(), stax.Relu(), stax.SumPool((2, 2), (2, 2)), conv(),
+            stax.Relu(), stax.GlobalAvgPool())
+  assert [s.conv[3] if s.conv else None for s in p.stages] == [
+      4.0, None, 64.0]
+  assert p.head_coef == (1.0, 0.0, 0.0, 1.0, 0.0)
+  p = _plan(conv(), stax.Relu(), stax.SumPool((2, 2), (2, 2)),
+            stax.GlobalSumPool(), stax.Dense(1, 2.0))
+  # m = mean over 16^4 entries; GlobalSumPool -> 16^4 m; SumPool -> x16
+  a = 16.0 * 16 ** 4
+  assert p.head_coef == (4 * a, 0.0, 4 * a, 4 * a, 0.0)
+  p = _plan(conv(), stax.Relu(), stax.SumPool((2, 2), (2, 2)))
+  assert p.out_scale == 16.0
