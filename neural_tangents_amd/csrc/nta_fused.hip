@@ -51,6 +51,20 @@ __device__ __forceinline__ float2 buf_load2(__amdgpu_buffer_rsrc_t r,
 
 constexpr unsigned kOutside = 0x40000000u;   // byte offset past any state
 
+// State written by a stage is read back only by the next launch, after
+// gigabytes of other traffic: streaming (non-temporal) stores keep it from
+// displacing the planes the current sweep re-reads from L2.
+#ifndef NTA_NT_STORE
+#define NTA_NT_STORE 0
+#endif
+__device__ __forceinline__ void st_state(float* p, float v) {
+#if NTA_NT_STORE
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
 // ReLU contribution of one pre-activation element, accumulated for the
 // pool: acc_n += sq1 sq2 (sqrt(1 - rho^2) + (pi - theta) rho) and
 // acc_t += (pi - theta) t; both sums are scaled by 1 / (2 pi) by the caller.
@@ -352,8 +366,8 @@ __global__ __launch_bounds__(256) void k_stage_fast(
         const long long o =
             p * PER_OUT + ((long long)(H * SO + Hp) * SO + W) * SO + Wp;
         constexpr float kScale = kInv2Pi / 16.0f;
-        out_n[o] = acc_n * kScale;
-        if constexpr (NTK) out_t[o] = acc_t * kScale;
+        st_state(out_n + o, acc_n * kScale);
+        if constexpr (NTK) st_state(out_t + o, acc_t * kScale);
       }
     }
   } else {
@@ -459,8 +473,8 @@ __device__ __forceinline__ void first32_epilogue(
     const float pm = rho < 0.0f ? th : kPi - th;
     const float sn = fsqrt(u * (2.0f - u));
     // sq1r already carries the 1 / (2 pi) factor.
-    on[m * S + col] = fmaf(pm, rho, sn) * (sq1r[r] * qb.x);
-    if constexpr (NTK) ot[m * S + col] = (cv * kInv2Pi) * pm;
+    st_state(on + m * S + col, fmaf(pm, rho, sn) * (sq1r[r] * qb.x));
+    if constexpr (NTK) st_state(ot + m * S + col, (cv * kInv2Pi) * pm);
   }
 }
 
@@ -619,8 +633,8 @@ __global__ __launch_bounds__(256) void k_first16(
       const float cv = fmaf(scale, acc[r], b_var);
       float ds;
       const float v = relu_map_norm(cv, sq1r[r] * qb.x, rq1r[r] * qb.y, &ds);
-      on[m * S + col] = v;
-      if constexpr (NTK) ot[m * S + col] = cv * ds;
+      st_state(on + m * S + col, v);
+      if constexpr (NTK) st_state(ot + m * S + col, cv * ds);
     }
   }
 }
