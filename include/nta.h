@@ -196,43 +196,47 @@ int nta_chol_split(int n);
  * with L L^T = A and n1 = nta_chol_split(n):
  *   W[0:n1, 0:n1] = L11^-1, W[n1:n, n1:n] = L22^-1 (lower, zeros above),
  *   W[n1:n, 0:n1] = L21,
- * the blocks from which the preconditioner (L L^T)^-1 is applied.
+ * the blocks from which the preconditioner (L L^T)^-1 is applied, and WT
+ * (n x n, leading dimension ldw) holds W^T on and above its diagonal.
  * *info (device int, zero on entry) receives the 1-based index of the first
  * non-positive pivot.
  * Replaces: jsp.linalg.cho_factor (predict.py:1244).
  */
 int nta_chol_f32(float* A, int n, long long lda, float* W, long long ldw,
-                 int* info, void* stream);
+                 float* WT, int* info, void* stream);
 
 /* Bytes of device workspace nta_pcg64 / nta_chol_precond need (-1 when the
  * arguments are invalid: nrhs must be 1..16). */
 long long nta_pcg64_work_bytes(int n, int nrhs);
 
 /*
- * Z = (L L^T)^-1 R in fp32 from the blocks nta_chol_f32 left in W;
- * R, Z [n][nrhs] fp32, nrhs <= 16, n1 = nta_chol_split(n).
+ * Z = (L L^T)^-1 R in fp32 from the blocks nta_chol_f32 left in W and WT
+ * (six row-major products on fp32 MFMA); R, Z [n][nrhs] fp32, nrhs <= 16,
+ * n1 = nta_chol_split(n).
  * Replaces: the triangular solves of cho_solve (predict.py:1254).
  */
-int nta_chol_precond(const float* W, int n, long long ldw, int n1,
-                     const float* R, int nrhs, float* Z, void* work,
+int nta_chol_precond(const float* W, const float* WT, int n, long long ldw,
+                     int n1, const float* R, int nrhs, float* Z, void* work,
                      long long work_bytes, void* stream);
 
 /*
  * Solves (K + lam I) X = B by conjugate gradients preconditioned with the
- * FP32 factor in W (nta_chol_f32 of K + shift I), FP64 vectors, FP64
- * products (K read in fp32, exact) and the FP64 shift lam.  B, X
+ * FP32 factor in W, WT (nta_chol_f32 of K + shift I), FP64 vectors, FP64
+ * products on fp64 MFMA (K read in fp32, exact) and the FP64 shift lam.  B, X
  * [n][nrhs] fp64, nrhs <= 16.  Iterates until every column's relative
- * residual ||r|| / ||b|| is <= tol, stops improving for 10 iterations, or
- * max_iter is reached; *iters receives the iteration count and *resid the
+ * residual ||r|| / ||b|| is <= tol, the residual recomputed every 10
+ * iterations stops improving for 50 iterations, or max_iter is reached;
+ * *iters receives the iteration count and *resid the
  * largest relative residual of the returned X, recomputed from scratch
  * (host pointers, may be NULL).  Synchronises the stream once per
  * iteration to test convergence.
  * Replaces: cho_solve (predict.py:1254) of the regularised train kernel.
  */
 int nta_pcg64(const float* K, int n, long long ldk, double lam,
-              const float* W, long long ldw, int n1, const double* B,
-              int nrhs, double* X, double tol, int max_iter, void* work,
-              long long work_bytes, int* iters, double* resid, void* stream);
+              const float* W, const float* WT, long long ldw, int n1,
+              const double* B, int nrhs, double* X, double tol, int max_iter,
+              void* work, long long work_bytes, int* iters, double* resid,
+              void* stream);
 
 #ifdef __cplusplus
 }

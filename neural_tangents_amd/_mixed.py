@@ -56,9 +56,10 @@ class MixedCholesky:
       A = K32.clone()
       A.diagonal().add_(shift)
       W = torch.zeros_like(A)
+      WT = torch.empty_like(A)
       info.zero_()
       _lib.call('nta_chol_f32', _lib.ptr(A), npad, npad, _lib.ptr(W), npad,
-                _lib.ptr(info), stream)
+                _lib.ptr(WT), _lib.ptr(info), stream)
       if int(info.item()) == 0:
         break
       shift = max(10.0 * shift, 1e-7 * mean_diag)
@@ -70,6 +71,7 @@ class MixedCholesky:
     del A
     self.shift = shift
     self.W = W
+    self.WT = WT
     self._work = {}
 
   def _workspace(self, nrhs):
@@ -94,9 +96,10 @@ class MixedCholesky:
     R32 = R.to(torch.float32).contiguous()
     Z = torch.empty_like(R32)
     ws = self._workspace(R.shape[1])
-    _lib.call('nta_chol_precond', _lib.ptr(self.W), self.npad, self.npad,
-              self.n1, _lib.ptr(R32), R.shape[1], _lib.ptr(Z), _lib.ptr(ws),
-              ws.numel(), _lib.stream_handle(self.device))
+    _lib.call('nta_chol_precond', _lib.ptr(self.W), _lib.ptr(self.WT),
+              self.npad, self.npad, self.n1, _lib.ptr(R32), R.shape[1],
+              _lib.ptr(Z), _lib.ptr(ws), ws.numel(),
+              _lib.stream_handle(self.device))
     return Z
 
   # -------------------------------------------------------------- solve
@@ -123,7 +126,8 @@ class MixedCholesky:
     iters = ctypes.c_int(0)
     resid = ctypes.c_double(0.0)
     _lib.call('nta_pcg64', _lib.ptr(self.K32), self.npad, self.npad,
-              self.lam, _lib.ptr(self.W), self.npad, self.n1, _lib.ptr(B), k,
+              self.lam, _lib.ptr(self.W), _lib.ptr(self.WT), self.npad,
+              self.n1, _lib.ptr(B), k,
               _lib.ptr(X), self.tol, self.max_iter, _lib.ptr(ws), ws.numel(),
               ctypes.byref(iters), ctypes.byref(resid),
               _lib.stream_handle(self.device))

@@ -61,38 +61,42 @@ struct GemmJobs {
   GemmJob j[2];
 };
 
-constexpr int GT = 128;          // tile edge
 constexpr int GK = 32;           // k per stage
 constexpr int GSR = GK + 4;      // row-layout stride (floats)
-constexpr int GSK = GT + 4;      // k-major-layout stride (floats)
-constexpr int GBUF = GT * GSR;   // floats per operand buffer (>= GK * GSK)
 
-template <bool BKM>
+// One TILE x TILE output tile (TILE = 128: 4 waves of 64x64, 2x2 MFMA
+// tiles each; TILE = 64: 4 waves of 32x32) with 32-deep k stages
+// double-buffered in LDS.
+template <int TILE, bool BKM>
 __device__ __forceinline__ void gemm_tile(const GemmJob& J, int ti, int tj,
                                           float* __restrict__ As,
                                           float* __restrict__ Bs) {
+  constexpr int MT = TILE / 64;                 // MFMA tiles per wave edge
+  constexpr int NLD = TILE / 32;                // float4 loads per operand
+  constexpr int GSK = TILE + 4;                 // k-major stride
+  constexpr int GBUF = TILE * GSR;              // floats per operand buffer
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int r = lane & 31, h = lane >> 5;
-  const int i0 = ti * GT, j0 = tj * GT;
+  const int i0 = ti * TILE, j0 = tj * TILE;
   int kb = 0, ke = J.K;
-  if (J.tri == 1) ke = min(J.K, j0 + GT);
+  if (J.tri == 1) ke = min(J.K, j0 + TILE);
   if (J.tri == 2) kb = j0;
-  if (J.tri == 3) ke = min(J.K, i0 + GT);
+  if (J.tri == 3) ke = min(J.K, i0 + TILE);
 
-  floatx16 acc[2][2];
+  floatx16 acc[MT][MT];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < MT; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < MT; ++b)
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.0f;
 
-  float4 ra[4], rb[4];
+  float4 ra[NLD], rb[NLD];
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < NLD; ++u) {
       const int f = t + 256 * u;
       const int row = f >> 3, c4 = (f & 7) * 4;
       ra[u] = (i0 + row < J.M)
@@ -100,7 +104,8 @@ __device__ __forceinline__ void gemm_tile(const GemmJob& J, int ti, int tj,
                                      k0 + c4)
                   : make_float4(0.f, 0.f, 0.f, 0.f);
       if (BKM) {
-        const int kr = f >> 5, n4 = (f & 31) * 4;
+        constexpr int NQ = TILE / 4;            // float4 per k-row
+        const int kr = f / NQ, n4 = (f % NQ) * 4;
         rb[u] = (j0 + n4 < J.N)
                     ? *(const float4*)(J.B + (long long)(k0 + kr) * J.ldb +
                                        j0 + n4)
@@ -117,12 +122,13 @@ __device__ __forceinline__ void gemm_tile(const GemmJob& J, int ti, int tj,
     float* as = As + buf * GBUF;
     float* bs = Bs + buf * GBUF;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < NLD; ++u) {
       const int f = t + 256 * u;
       const int row = f >> 3, c4 = (f & 7) * 4;
       *(float4*)(as + row * GSR + c4) = ra[u];
       if (BKM) {
-        const int kr = f >> 5, n4 = (f & 31) * 4;
+        constexpr int NQ = TILE / 4;
+        const int kr = f / NQ, n4 = (f % NQ) * 4;
         *(float4*)(bs + kr * GSK + n4) = rb[u];
       } else {
         *(float4*)(bs + row * GSR + c4) = rb[u];
@@ -141,10 +147,10 @@ __device__ __forceinline__ void gemm_tile(const GemmJob& J, int ti, int tj,
     if (more) gload(k0 + GK);
     const float* as = As + buf * GBUF;
     const float* bs = Bs + buf * GBUF;
-    float af[2][16], bf[2][16];
+    float af[MT][16], bf[MT][16];
 #pragma unroll
-    for (int mi = 0; mi < 2; ++mi) {
-      const float* p = as + (64 * wm + 32 * mi + r) * GSR + 16 * h;
+    for (int mi = 0; mi < MT; ++mi) {
+      const float* p = as + (32 * MT * wm + 32 * mi + r) * GSR + 16 * h;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const float4 v = *(const float4*)(p + 4 * q);
@@ -155,8 +161,8 @@ __device__ __forceinline__ void gemm_tile(const GemmJob& J, int ti, int tj,
       }
     }
 #pragma unroll
-    for (int ni = 0; ni < 2; ++ni) {
-      const int n = 64 * wn + 32 * ni + r;
+    for (int ni = 0; ni < MT; ++ni) {
+      const int n = 32 * MT * wn + 32 * ni + r;
       if (BKM) {
 #pragma unroll
         for (int kk = 0; kk < 16; ++kk) bf[ni][kk] = bs[(16 * h + kk) * GSK + n];
@@ -175,9 +181,9 @@ __device__ __forceinline__ void gemm_tile(const GemmJob& J, int ti, int tj,
 #pragma unroll
     for (int kk = 0; kk < 16; ++kk)
 #pragma unroll
-      for (int mi = 0; mi < 2; ++mi)
+      for (int mi = 0; mi < MT; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni)
+        for (int ni = 0; ni < MT; ++ni)
           acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x2f32(
               af[mi][kk], bf[ni][kk], acc[mi][ni], 0, 0, 0);
     if (more) lstore(buf ^ 1);
@@ -186,14 +192,14 @@ __device__ __forceinline__ void gemm_tile(const GemmJob& J, int ti, int tj,
   }
 
 #pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
+  for (int mi = 0; mi < MT; ++mi)
 #pragma unroll
-    for (int ni = 0; ni < 2; ++ni)
+    for (int ni = 0; ni < MT; ++ni)
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
-        const int row = i0 + 64 * wm + 32 * mi + 8 * (e >> 2) + 4 * h +
+        const int row = i0 + 32 * MT * wm + 32 * mi + 8 * (e >> 2) + 4 * h +
                         (e & 3);
-        const int col = j0 + 64 * wn + 32 * ni + r;
+        const int col = j0 + 32 * MT * wn + 32 * ni + r;
         if (row < J.M && col < J.N) {
           float* c = J.C + (long long)row * J.ldc + col;
           float v = J.alpha * acc[mi][ni][e];
@@ -203,9 +209,10 @@ __device__ __forceinline__ void gemm_tile(const GemmJob& J, int ti, int tj,
       }
 }
 
+template <int TILE>
 __global__ __launch_bounds__(256) void k_gemm(GemmJobs jobs) {
-  __shared__ __align__(16) float As[2 * GBUF];
-  __shared__ __align__(16) float Bs[2 * GBUF];
+  __shared__ __align__(16) float As[2 * TILE * GSR];
+  __shared__ __align__(16) float Bs[2 * TILE * GSR];
   int b = blockIdx.x;
   const bool second = b >= jobs.j[0].ntiles;
   const GemmJob J = second ? jobs.j[1] : jobs.j[0];
@@ -221,15 +228,19 @@ __global__ __launch_bounds__(256) void k_gemm(GemmJobs jobs) {
     tj = b % J.tiles_n;
   }
   if (J.bkm)
-    gemm_tile<true>(J, ti, tj, As, Bs);
+    gemm_tile<TILE, true>(J, ti, tj, As, Bs);
   else
-    gemm_tile<false>(J, ti, tj, As, Bs);
+    gemm_tile<TILE, false>(J, ti, tj, As, Bs);
 }
+
+// Jobs of up to 512 x 512 outputs use 64x64 tiles (more workgroups, no
+// MFMA work on empty quadrants); larger ones 128x128.
+static int tile_for(int M, int N) { return (M <= 512 && N <= 512) ? 64 : 128; }
 
 static GemmJob make_job(const float* A, long long lda, const float* B,
                         long long ldb, float* C, long long ldc, int M, int N,
                         int K, float alpha, float beta, int bkm, int tri,
-                        int lower) {
+                        int lower, int tile) {
   GemmJob j;
   j.A = A;
   j.B = B;
@@ -245,13 +256,13 @@ static GemmJob make_job(const float* A, long long lda, const float* B,
   j.bkm = bkm;
   j.tri = tri;
   j.lower = lower;
-  const int tm = (M + GT - 1) / GT;
-  j.tiles_n = (N + GT - 1) / GT;
+  const int tm = (M + tile - 1) / tile;
+  j.tiles_n = (N + tile - 1) / tile;
   j.ntiles = lower ? tm * (tm + 1) / 2 : tm * j.tiles_n;
   return j;
 }
 
-static void launch_gemm(const GemmJob& a, const GemmJob* b,
+static void launch_gemm(const GemmJob& a, const GemmJob* b, int tile,
                         hipStream_t st) {
   GemmJobs jobs;
   jobs.j[0] = a;
@@ -262,84 +273,206 @@ static void launch_gemm(const GemmJob& a, const GemmJob* b,
     jobs.j[1].ntiles = 0;
   }
   const int grid = jobs.j[0].ntiles + jobs.j[1].ntiles;
-  hipLaunchKernelGGL(k_gemm, dim3(grid), dim3(256), 0, st, jobs);
+  if (tile == 64)
+    hipLaunchKernelGGL(k_gemm<64>, dim3(grid), dim3(256), 0, st, jobs);
+  else
+    hipLaunchKernelGGL(k_gemm<128>, dim3(grid), dim3(256), 0, st, jobs);
 }
 
 // ---------------------------------------------------------------- leaf
 // Factors the 64x64 diagonal block of A at (k0, k0) (lower triangle read)
 // and writes its inverse factor W = L^-1 (64x64, zeros above the diagonal)
-// at (k0, k0) of W.  One wavefront: lane i holds row i of the block in
-// registers; each column step broadcasts the scaled column through LDS.
+// at (k0, k0) of W (whose upper-right 32x32 block is left untouched, zero).
+// One wavefront, in 2 x 2 blocks of 32:
+//   1. the 64 x 32 panel [A11; A21] is factored in registers (lane i holds
+//      row i; column j's scaled entries are broadcast with readlane), giving
+//      L11 (lanes 0-31) and L21 (lanes 32-63) together;
+//   2. A22 -= L21 L21^T on v_mfma_f32_32x32x2f32;
+//   3. A22 is factored the same way (lanes 0-31);
+//   4. L11^-1 (lanes 0-31) and L22^-1 (lanes 32-63) by forward
+//      substitution, one column per lane;
+//   5. W21 = -L22^-1 (L21 L11^-1) on MFMA.
 // *info receives the 1-based global index of the first non-positive pivot
 // (the factorisation continues with a unit pivot so the output stays
 // finite).
+constexpr int LP = 33;                      // LDS row stride of 32x32 tiles
+
+constexpr int LQ = 36;                      // 16-byte aligned LDS rows
+
+// Right-looking factorisation of the 32 columns held in a[] (lane i: row i
+// of the panel; lanes < 32 are the diagonal block).  The scaled column of
+// each step goes through lcol (LDS) and comes back as 4-wide broadcast
+// reads.  Pivots use the hardware reciprocal square root: the factor only
+// preconditions.  Returns the 1-based column of the first non-positive
+// pivot (0 if none).
+__device__ __forceinline__ int panel32(float (&a)[32], int i,
+                                       float* __restrict__ lcol) {
+  int bad = 0;
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    float piv = __int_as_float(
+        __builtin_amdgcn_readlane(__float_as_int(a[j]), j));
+    if (!(piv > 0.0f)) {
+      if (bad == 0) bad = j + 1;
+      piv = 1.0f;
+    }
+    const float rs = __builtin_amdgcn_rsqf(piv);
+    float l = (i == j) ? piv * rs : a[j] * rs;
+    if (i < j) l = 0.0f;
+    a[j] = l;
+    lcol[i] = l;
+    __builtin_amdgcn_s_waitcnt(0xc07f);      // lgkmcnt(0): write visible
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int k = (j + 1) & ~3; k < 32; k += 4) {
+      const float4 v = *(const float4*)(lcol + k);
+      const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (k + u > j) a[k + u] = fmaf(-l, vv[u], a[k + u]);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  return bad;
+}
+
+// D (32x32, MFMA layout) = X Y for 32x32 LDS tiles; ta / tb: read X / Y
+// transposed (element (r, c) of the operand at [c * LP + r]).
+__device__ __forceinline__ floatx16 mm32(const float* __restrict__ X,
+                                         bool ta, const float* __restrict__ Y,
+                                         bool tb, int lane) {
+  const int r = lane & 31, h = lane >> 5;
+  floatx16 acc;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+#pragma unroll
+  for (int kk = 0; kk < 16; ++kk) {
+    const int k = 2 * kk + h;
+    const float av = ta ? X[k * LP + r] : X[r * LP + k];
+    const float bv = tb ? Y[r * LP + k] : Y[k * LP + r];
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
 __global__ __launch_bounds__(64) void k_leaf64(const float* __restrict__ A,
                                                long long lda,
                                                float* __restrict__ W,
                                                long long ldw, int k0,
                                                int* __restrict__ info) {
+  __shared__ __align__(16) float L11[32 * LQ], L22[32 * LQ];
+  __shared__ float L21[32 * LP], A22[32 * LP];
+  __shared__ float W11[32 * LP], W22[32 * LP], T[32 * LP];
   __shared__ __align__(16) float lcol[64];
-  __shared__ __align__(16) float Ls[64 * 68];
-  __shared__ __align__(16) float dinv[64];
+  __shared__ float dg[64];
   const int i = threadIdx.x;
-  float a[64];
+  const int r = i & 31, h = i >> 5;
+  // A22 to LDS (the MFMA update below touches all of it).
+#pragma unroll
+  for (int e = i; e < 32 * 32; e += 64) {
+    const int rr = e >> 5, cc = e & 31;
+    A22[rr * LP + cc] =
+        A[(long long)(k0 + 32 + rr) * lda + k0 + 32 + cc];
+  }
+  // 1. panel [A11; A21]
+  float a[32];
   const float* row = A + (long long)(k0 + i) * lda + k0;
 #pragma unroll
-  for (int c = 0; c < 64; c += 4) {
+  for (int c = 0; c < 32; c += 4) {
     const float4 v = *(const float4*)(row + c);
-    a[c] = c <= i ? v.x : 0.0f;
-    a[c + 1] = c + 1 <= i ? v.y : 0.0f;
-    a[c + 2] = c + 2 <= i ? v.z : 0.0f;
-    a[c + 3] = c + 3 <= i ? v.w : 0.0f;
+    a[c] = (i >= 32 || c <= i) ? v.x : 0.0f;
+    a[c + 1] = (i >= 32 || c + 1 <= i) ? v.y : 0.0f;
+    a[c + 2] = (i >= 32 || c + 2 <= i) ? v.z : 0.0f;
+    a[c + 3] = (i >= 32 || c + 3 <= i) ? v.w : 0.0f;
   }
-  int bad = 0;
+  int bad = panel32(a, i, lcol);
+  if (bad) bad += k0;
+  if (h == 0) {
 #pragma unroll
-  for (int j = 0; j < 64; ++j) {
-    float piv = __int_as_float(
-        __builtin_amdgcn_readlane(__float_as_int(a[j]), j));
-    if (!(piv > 0.0f)) {
-      if (bad == 0) bad = k0 + j + 1;
-      piv = 1.0f;
+    for (int c = 0; c < 32; c += 4)
+      *(float4*)(L11 + r * LQ + c) = make_float4(a[c], a[c + 1], a[c + 2],
+                                                a[c + 3]);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 32; ++c) L21[r * LP + c] = a[c];
+  }
+  __syncthreads();
+  if (h == 0) dg[r] = __builtin_amdgcn_rcpf(L11[r * LQ + r]);
+  // 2. A22 -= L21 L21^T
+  {
+    const floatx16 d = mm32(L21, false, L21, true, i);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int rr = 8 * (e >> 2) + 4 * h + (e & 3);
+      A22[rr * LP + r] -= d[e];
     }
-    const float rs = 1.0f / sqrtf(piv);
-    float l = (i == j) ? sqrtf(piv) : a[j] * rs;
-    if (i < j) l = 0.0f;
-    a[j] = l;
-    lcol[i] = l;
-    __syncthreads();
+  }
+  __syncthreads();
+  // 3. A22 = L22 L22^T (lanes 0-31 hold its rows)
 #pragma unroll
-    for (int k = j + 1; k < 64; ++k) a[k] = fmaf(-l, lcol[k], a[k]);
-    __syncthreads();
+  for (int c = 0; c < 32; ++c)
+    a[c] = (h == 0 && c <= r) ? A22[r * LP + c] : 0.0f;
+  {
+    int bad2 = panel32(a, h ? 64 : r, lcol);
+    if (bad == 0 && bad2) bad = k0 + 32 + bad2;
+  }
+  if (h == 0) {
+#pragma unroll
+    for (int c = 0; c < 32; c += 4)
+      *(float4*)(L22 + r * LQ + c) = make_float4(a[c], a[c + 1], a[c + 2],
+                                                a[c + 3]);
   }
   if (bad != 0 && i == 0 && *info == 0) *info = bad;
-  // L rows to LDS (zeros above the diagonal), reciprocal diagonal.
-#pragma unroll
-  for (int c = 0; c < 64; c += 4) {
-    float4 v;
-    v.x = c <= i ? a[c] : 0.0f;
-    v.y = c + 1 <= i ? a[c + 1] : 0.0f;
-    v.z = c + 2 <= i ? a[c + 2] : 0.0f;
-    v.w = c + 3 <= i ? a[c + 3] : 0.0f;
-    *(float4*)(Ls + i * 68 + c) = v;
-  }
-  float di = 0.0f;
-#pragma unroll
-  for (int c = 0; c < 64; ++c)
-    if (c == i) di = a[c];
-  dinv[i] = 1.0f / di;
   __syncthreads();
-  // Lane i computes column i of W by forward substitution.
-  float w[64];
+  if (h == 0) dg[32 + r] = __builtin_amdgcn_rcpf(L22[r * LQ + r]);
+  __syncthreads();
+  // 4. column r of L11^-1 (h = 0) or of L22^-1 (h = 1); two interleaved
+  // partial sums per row halve the dependent-FMA chain.
+  {
+    const float* Ls = h ? L22 : L11;
+    const float* dv = dg + 32 * h;
+    float w[32];
 #pragma unroll
-  for (int k = 0; k < 64; ++k) {
-    float s = (k == i) ? 1.0f : 0.0f;
+    for (int k = 0; k < 32; ++k) {
+      float s0 = (k == r) ? 1.0f : 0.0f, s1 = 0.0f;
 #pragma unroll
-    for (int m = 0; m < k; ++m) s = fmaf(-Ls[k * 68 + m], w[m], s);
-    w[k] = s * dinv[k];
+      for (int m = 0; m < k; m += 4) {
+        const float4 lv = *(const float4*)(Ls + k * LQ + m);
+        s0 = fmaf(-lv.x, w[m], s0);
+        if (m + 1 < k) s1 = fmaf(-lv.y, w[m + 1], s1);
+        if (m + 2 < k) s0 = fmaf(-lv.z, w[m + 2], s0);
+        if (m + 3 < k) s1 = fmaf(-lv.w, w[m + 3], s1);
+      }
+      w[k] = (s0 + s1) * dv[k];
+    }
+    float* Wd = h ? W22 : W11;
+#pragma unroll
+    for (int k = 0; k < 32; ++k) Wd[k * LP + r] = w[k];
+  }
+  __syncthreads();
+  // 5. T = L21 W11, W21 = -W22 T
+  {
+    const floatx16 d = mm32(L21, false, W11, false, i);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int rr = 8 * (e >> 2) + 4 * h + (e & 3);
+      T[rr * LP + r] = d[e];
+    }
+  }
+  __syncthreads();
+  const floatx16 w21 = mm32(W22, false, T, false, i);
+  // Output: W11, W22 (lower, zeros above) and W21.
+#pragma unroll
+  for (int e = i; e < 32 * 32; e += 64) {
+    const int rr = e >> 5, cc = e & 31;
+    W[(long long)(k0 + rr) * ldw + k0 + cc] = W11[rr * LP + cc];
+    W[(long long)(k0 + 32 + rr) * ldw + k0 + 32 + cc] = W22[rr * LP + cc];
   }
 #pragma unroll
-  for (int k = 0; k < 64; ++k)
-    W[(long long)(k0 + k) * ldw + k0 + i] = w[k];
+  for (int e = 0; e < 16; ++e) {
+    const int rr = 8 * (e >> 2) + 4 * h + (e & 3);
+    W[(long long)(k0 + 32 + rr) * ldw + k0 + r] = -w21[e];
+  }
 }
 
 // Split of a block of n rows into n1 + n2 (n1 a multiple of 64; at the top
@@ -364,422 +497,374 @@ static void chol_node(float* A, long long lda, float* W, long long ldw, int o,
   float* W11 = W + (long long)o * ldw + o;
   float* W21 = W + (long long)r2 * ldw + o;
   float* W22 = W + (long long)r2 * ldw + r2;
+  // One tile size for the whole node (the SYRK and T products share a
+  // launch).
+  const int tile = tile_for(n2, n1 > n2 ? n1 : n2);
   // S = A21 W11^T  (W11 lower: B of the NT form is lower)
   launch_gemm(make_job(A21, lda, W11, ldw, W21, ldw, n2, n1, n1, 1.0f, 0.0f,
-                       0, 1, 0),
-              nullptr, st);
+                       0, 1, 0, tile),
+              nullptr, tile, st);
   // A22 -= S S^T (lower tiles), and T = S W11 into the A21 slot.
   const GemmJob syrk = make_job(W21, ldw, W21, ldw, A22, lda, n2, n2, n1,
-                                -1.0f, 1.0f, 0, 0, 1);
+                                -1.0f, 1.0f, 0, 0, 1, tile);
   if (top) {
-    launch_gemm(syrk, nullptr, st);
+    launch_gemm(syrk, nullptr, tile, st);
   } else {
     const GemmJob tt = make_job(W21, ldw, W11, ldw, A21, lda, n2, n1, n1,
-                                1.0f, 0.0f, 1, 2, 0);
-    launch_gemm(syrk, &tt, st);
+                                1.0f, 0.0f, 1, 2, 0, tile);
+    launch_gemm(syrk, &tt, tile, st);
   }
   chol_node(A, lda, W, ldw, r2, n2, false, info, st);
   if (!top) {
     // W21 = -W22 T
     launch_gemm(make_job(W22, ldw, A21, lda, W21, ldw, n2, n1, n2, -1.0f,
-                         0.0f, 1, 3, 0),
-                nullptr, st);
+                         0.0f, 1, 3, 0, tile),
+                nullptr, tile, st);
+  }
+}
+
+// Transpose of the lower blocks of W (tiles on or below the diagonal) into
+// WT (WT's tiles strictly below the diagonal are left untouched; the
+// preconditioner never reads them).
+__global__ __launch_bounds__(256) void k_transpose_lower(
+    const float* __restrict__ W, long long ldw, int n, float* __restrict__ WT,
+    long long ldt) {
+  __shared__ float tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;   // W tile (by, bx)
+  if (bx > by) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    const int i = by + r, j = bx + tx;
+    tile[r][tx] = (i < n && j < n) ? W[(long long)i * ldw + j] : 0.0f;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = bx + r, j = by + tx;
+    if (i < n && j < n) WT[(long long)i * ldt + j] = tile[tx][r];
   }
 }
 
 // ----------------------------------------------------------- PCG kernels
-// Vectors are [n][NR] (NR right-hand sides, padded).  Partial sums are
-// indexed [block][n][NR] and summed in a fixed order, so every result is
-// deterministic.
-constexpr int SV_R = 64;     // rows of a (K p) / N-product tile
-constexpr int SV_C = 512;    // columns of the same
-constexpr int GTR = 128;     // reduced rows of a T-product tile
-constexpr int GTC = 256;     // output columns of a T-product tile
-constexpr int VR = 64;       // rows per workgroup of the vector kernels
+// Vectors are [n][16] (up to 16 right-hand sides, zero padded): one MFMA
+// 16x16 tile spans the right-hand sides.  Every product is row-complete
+// (a workgroup owns whole output rows), so there are no partial-sum
+// buffers; per-workgroup dot-product partials are summed in a fixed order.
+constexpr int RHS = 16;
+constexpr int VROWS = 256;        // rows per workgroup of the vector kernels
 
-// Sum of nslot partials [slot][NR] of column c (fixed order).
-template <int NR>
-__device__ __forceinline__ double sum_slots(const double* __restrict__ s,
-                                            int nslot, int c) {
-  double v = 0.0;
-  for (int k = 0; k < nslot; ++k) v += s[k * NR + c];
-  return v;
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// Column range of the k-loop of row stripe [i0, i0 + 16) for a lower (tri =
+// 1: columns <= row) or upper (tri = 2: columns >= row) triangular operand.
+__device__ __forceinline__ void tri_range(int tri, int i0, int k, int& kb,
+                                          int& ke) {
+  kb = 0;
+  ke = k;
+  if (tri == 1) ke = min(k, i0 + 16);
+  if (tri == 2) kb = i0;
 }
 
-// Tile partial of (K + lambda I) X over 64 rows x 512 columns, FP64.
-// mode 0 (CG step): X = z + beta p_old with beta = rz_new / rz_old per
-//   column (0 on the first step); row-block-0 tiles store X into p_new;
-//   every tile writes its share of p^T (K + lambda I) p into pq_part.
-// mode 1: X = p_old (plain product, no stores besides the partials).
-template <int NR>
-__global__ __launch_bounds__(256) void k_symv_part(
-    const float* __restrict__ K, long long ldk, int n, double lam, int mode,
-    const float* __restrict__ z, const double* __restrict__ p_old,
-    double* __restrict__ p_new, const double* __restrict__ rz_new,
-    const double* __restrict__ rz_old, int nslot, int first,
-    double* __restrict__ P64, double* __restrict__ pq_part) {
-  __shared__ __align__(16) double xs[SV_C * NR];
-  __shared__ double beta_s[NR];
-  __shared__ double red[SV_R * NR];
-  const int t = threadIdx.x;
-  const int cb = blockIdx.x, rb = blockIdx.y;
-  const int i0 = rb * SV_R, j0 = cb * SV_C;
-  const int ncols = min(SV_C, n - j0);
-  if (t < NR) {
-    double b = 0.0;
-    if (mode == 0 && !first) {
-      const double num = sum_slots<NR>(rz_new, nslot, t);
-      const double den = sum_slots<NR>(rz_old, nslot, t);
-      b = den != 0.0 ? num / den : 0.0;
-    }
-    beta_s[t] = b;
-  }
-  __syncthreads();
-  for (int e = t; e < ncols * NR; e += 256) {
-    const int j = e / NR, c = e % NR;
-    const long long g = (long long)(j0 + j) * NR + c;
-    double x;
-    if (mode == 0) {
-      x = (double)z[g] + beta_s[c] * p_old[g];
-      if (rb == 0) p_new[g] = x;
-    } else {
-      x = p_old[g];
-    }
-    xs[j * NR + c] = x;
-  }
-  __syncthreads();
-  const int row = t >> 2, q = t & 3;
-  double acc[NR];
+// One 16-column chunk of a row-major fp32 operand and the matching 16 rows
+// of a [k][RHS] right-hand side: lane l holds A[row l%16][4 (l/16) + kk]
+// and X[4 (l/16) + kk][rhs l%16] for MFMA kk (the reduction index is
+// permuted identically in both operands).
+template <typename TX>
+struct Chunk {
+  float4 a;
+  TX x[4];
+};
+
+template <typename TX>
+__device__ __forceinline__ void load_chunk(Chunk<TX>& c, bool ok,
+                                           const float* __restrict__ arow,
+                                           const TX* __restrict__ xcol,
+                                           int j0) {
+  if (ok) {
+    c.a = *(const float4*)(arow + j0);
 #pragma unroll
-  for (int c = 0; c < NR; ++c) acc[c] = 0.0;
-  const float* krow = K + (long long)(i0 + row) * ldk + j0;
-  for (int j = 4 * q; j < ncols; j += 16) {
-    const float4 kv = *(const float4*)(krow + j);
-    const float kk[4] = {kv.x, kv.y, kv.z, kv.w};
+    for (int kk = 0; kk < 4; ++kk) c.x[kk] = xcol[(long long)(j0 + kk) * RHS];
+  } else {
+    c.a = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) c.x[kk] = (TX)0;
+  }
+}
+
+__device__ __forceinline__ void mfma_chunk(doublex4& acc,
+                                           const Chunk<double>& c) {
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)c.a.x, c.x[0], acc, 0,
+                                             0, 0);
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)c.a.y, c.x[1], acc, 0,
+                                             0, 0);
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)c.a.z, c.x[2], acc, 0,
+                                             0, 0);
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)c.a.w, c.x[3], acc, 0,
+                                             0, 0);
+}
+
+__device__ __forceinline__ void mfma_chunk(floatx4& acc,
+                                           const Chunk<float>& c) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(c.a.x, c.x[0], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(c.a.y, c.x[1], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(c.a.z, c.x[2], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(c.a.w, c.x[3], acc, 0, 0, 0);
+}
+
+// acc += A[16 rows, chunks c0, c0 + step, ...] X over the chunk indices
+// c0 + step * i < c1 (chunks of 16 columns), software-pipelined: the loads
+// of the next group of 4 chunks are in flight while the current group's
+// MFMAs run.
+template <typename TX, typename ACC>
+__device__ __forceinline__ void row_stripe_product(
+    ACC& acc, const float* __restrict__ arow, const TX* __restrict__ xcol,
+    int c0, int c1, int step) {
+  Chunk<TX> cur[4], nxt[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int c = c0 + step * u;
+    load_chunk(cur[u], c < c1, arow, xcol, 16 * c);
+  }
+  for (int cb = c0; cb < c1; cb += 4 * step) {
+    const int cn = cb + 4 * step;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const double kd = (double)kk[u];
-      const double* xr = xs + (j + u) * NR;
-#pragma unroll
-      for (int c = 0; c < NR; ++c) acc[c] = fma(kd, xr[c], acc[c]);
+      const int c = cn + step * u;
+      load_chunk(nxt[u], c < c1, arow, xcol, 16 * c);
     }
-  }
 #pragma unroll
-  for (int c = 0; c < NR; ++c) {
-    double v = acc[c];
-    v += __shfl_xor(v, 1);
-    v += __shfl_xor(v, 2);
-    acc[c] = v;
-  }
-  const int gi = i0 + row;
-  const bool diag = (gi >= j0 && gi < j0 + ncols);
-  if (q == 0) {
+    for (int u = 0; u < 4; ++u) mfma_chunk(acc, cur[u]);
 #pragma unroll
-    for (int c = 0; c < NR; ++c)
-      P64[((long long)cb * n + gi) * NR + c] = acc[c];
-  }
-  if (mode != 0) return;
-  // p_i (K_tile p)_i (+ lambda p_i^2 on the diagonal tile), per column.
-  if (q == 0) {
-#pragma unroll
-    for (int c = 0; c < NR; ++c) {
-      const long long g = (long long)gi * NR + c;
-      const double pi = (double)z[g] + beta_s[c] * p_old[g];
-      double v = pi * acc[c];
-      if (diag) v += lam * pi * pi;
-      red[row * NR + c] = v;
-    }
-  }
-  __syncthreads();
-  if (t < NR) {
-    double s = 0.0;
-    for (int r = 0; r < SV_R; ++r) s += red[r * NR + t];
-    pq_part[((long long)rb * gridDim.x + cb) * NR + t] = s;
+    for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
   }
 }
 
-// q = sum of the (K p) partials + lambda p; alpha = rz / pq;
-// x += alpha p; r -= alpha q; r32 = r; rr_part = per-block sum of r^2.
-template <int NR>
-__global__ __launch_bounds__(256) void k_pcg_q(
-    int n, double lam, int ncb, const double* __restrict__ P64,
-    const double* __restrict__ pq_part, int ntiles,
-    const double* __restrict__ rz, int nslot, const double* __restrict__ p,
-    double* __restrict__ x, double* __restrict__ r, float* __restrict__ r32,
-    double* __restrict__ rr_part) {
-  __shared__ double alpha_s[NR];
-  __shared__ double red[256 * NR > VR * NR ? 256 * NR : VR * NR];
-  const int t = threadIdx.x;
-  // pq: thread-strided partial sums, then a fixed-order tree.
-  {
-    double s[NR];
-#pragma unroll
-    for (int c = 0; c < NR; ++c) s[c] = 0.0;
-    for (int k = t; k < ntiles; k += 256)
-#pragma unroll
-      for (int c = 0; c < NR; ++c) s[c] += pq_part[(long long)k * NR + c];
-#pragma unroll
-    for (int c = 0; c < NR; ++c) red[t * NR + c] = s[c];
-  }
+// (K + lam I) P in FP64 for 32 rows per workgroup of 8 waves: wave w
+// computes rows [i0 + 16 (w & 1), +16) over the 16-column chunks
+// c = w >> 1 (mod 4) on v_mfma_f64_16x16x4f64 (K read in fp32, exact in
+// fp64); the four partial tiles of a row stripe are added through LDS in a
+// fixed order.
+// mode 0: Q = (K + lam I) P, part[wg][c] = sum_rows P Q.
+// mode 1: D = B - (K + lam I) P; R = D, R32 = D (when R is given),
+//         part[wg][c] = sum_rows D^2.
+__global__ __launch_bounds__(512) void k_symv_rows(
+    const float* __restrict__ K, long long ldk, int n, double lam, int mode,
+    const double* __restrict__ P, double* __restrict__ Q,
+    const double* __restrict__ B, double* __restrict__ R,
+    float* __restrict__ R32, double* __restrict__ part) {
+  __shared__ doublex4 red[4][2][64];
+  __shared__ double dots[2][RHS];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int s = w & 1, qd = w >> 1;
+  const int r = lane & 15, g = lane >> 4;
+  const int i0 = blockIdx.x * 32 + 16 * s;
+  doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+  row_stripe_product(acc, K + (long long)(i0 + r) * ldk + 4 * g,
+                     P + (long long)(4 * g) * RHS + r, qd, n / 16, 4);
+  red[qd][s][lane] = acc;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w)
+  if (qd == 0) {
+    const doublex4 a1 = red[1][s][lane], a2 = red[2][s][lane],
+                   a3 = red[3][s][lane];
+    double d = 0.0;
+    // v_mfma_f64_16x16x4f64 output: register v of lane l is row
+    // 4 v + l / 16, column l % 16 (the f32 16x16x4 layout is row
+    // 4 (l / 16) + v).
 #pragma unroll
-      for (int c = 0; c < NR; ++c) red[t * NR + c] += red[(t + w) * NR + c];
-    __syncthreads();
-  }
-  if (t < NR) {
-    const double pq = red[t];
-    const double rzc = sum_slots<NR>(rz, nslot, t);
-    alpha_s[t] = pq != 0.0 ? rzc / pq : 0.0;
-  }
-  __syncthreads();
-  const int i0 = blockIdx.x * VR;
-  for (int e = t; e < VR * NR; e += 256) {
-    const int i = i0 + e / NR, c = e % NR;
-    double v = 0.0;
-    double rr = 0.0;
-    if (i < n) {
-      const long long g = (long long)i * NR + c;
-      double qv = 0.0;
-      for (int k = 0; k < ncb; ++k) qv += P64[((long long)k * n + i) * NR + c];
-      const double pv = p[g];
-      qv = fma(lam, pv, qv);
-      x[g] = fma(alpha_s[c], pv, x[g]);
-      v = fma(-alpha_s[c], qv, r[g]);
-      r[g] = v;
-      r32[g] = (float)v;
-      rr = v * v;
-    }
-    red[e] = rr;
-  }
-  __syncthreads();
-  if (t < NR) {
-    double s = 0.0;
-    for (int k = 0; k < VR; ++k) s += red[k * NR + t];
-    rr_part[(long long)blockIdx.x * NR + t] = s;
-  }
-}
-
-// Y partials of A X (fp32) for a row-major block A (m x k), tiles of 64 rows
-// x 512 reduced columns: P[cb][i][c] = sum_{j in cb} A[i][j] X[j][c].
-// tri: A lower triangular (tiles above the diagonal are zero).
-template <int NR>
-__global__ __launch_bounds__(256) void k_gemv_n(
-    const float* __restrict__ A, long long lda, int m, int k, int tri,
-    const float* __restrict__ X, float* __restrict__ P) {
-  __shared__ __align__(16) float xs[SV_C * NR];
-  const int t = threadIdx.x;
-  const int cb = blockIdx.x, rb = blockIdx.y;
-  const int i0 = rb * SV_R, j0 = cb * SV_C;
-  const int ncols = min(SV_C, k - j0);
-  const int row = t >> 2, q = t & 3;
-  float acc[NR];
-#pragma unroll
-  for (int c = 0; c < NR; ++c) acc[c] = 0.0f;
-  if (!(tri && j0 > i0 + SV_R - 1)) {
-    for (int e = t; e < ncols * NR; e += 256)
-      xs[e] = X[(long long)j0 * NR + e];
-    __syncthreads();
-    const float* arow = A + (long long)(i0 + row) * lda + j0;
-    for (int j = 4 * q; j < ncols; j += 16) {
-      const float4 av = *(const float4*)(arow + j);
-      const float aa[4] = {av.x, av.y, av.z, av.w};
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const float* xr = xs + (j + u) * NR;
-#pragma unroll
-        for (int c = 0; c < NR; ++c) acc[c] = fmaf(aa[u], xr[c], acc[c]);
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < NR; ++c) {
-      float v = acc[c];
-      v += __shfl_xor(v, 1);
-      v += __shfl_xor(v, 2);
-      acc[c] = v;
-    }
-  }
-  if (q == 0 && i0 + row < m) {
-#pragma unroll
-    for (int c = 0; c < NR; ++c)
-      P[((long long)cb * m + i0 + row) * NR + c] = acc[c];
-  }
-}
-
-// Partials of A^T Y (fp32) for a row-major block A (k rows x m columns):
-// tiles of 128 reduced rows x 256 output columns,
-// P[rb][j][c] = sum_{i in rb} A[i][j] Y[i][c].  tri: A lower triangular.
-template <int NR>
-__global__ __launch_bounds__(256) void k_gemv_t(
-    const float* __restrict__ A, long long lda, int k, int m, int tri,
-    const float* __restrict__ Y, float* __restrict__ P) {
-  __shared__ __align__(16) float ys[GTR * NR];
-  __shared__ __align__(16) float red[4 * GTC * NR];
-  const int t = threadIdx.x;
-  const int lane = t & 63, wave = t >> 6;
-  const int cb = blockIdx.x, rb = blockIdx.y;
-  const int j0 = cb * GTC, i0 = rb * GTR;
-  const int nrows = min(GTR, k - i0);
-  const int jc = j0 + 4 * lane;
-  float acc[4][NR];
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int c = 0; c < NR; ++c) acc[u][c] = 0.0f;
-  const bool zero = tri && j0 > i0 + GTR - 1;
-  if (!zero) {
-    for (int e = t; e < nrows * NR; e += 256)
-      ys[e] = Y[(long long)i0 * NR + e];
-    __syncthreads();
-    if (jc < m) {
-      for (int i = wave; i < nrows; i += 4) {
-        const float4 av = *(const float4*)(A + (long long)(i0 + i) * lda + jc);
-        const float* yr = ys + i * NR;
-#pragma unroll
-        for (int c = 0; c < NR; ++c) {
-          const float y = yr[c];
-          acc[0][c] = fmaf(av.x, y, acc[0][c]);
-          acc[1][c] = fmaf(av.y, y, acc[1][c]);
-          acc[2][c] = fmaf(av.z, y, acc[2][c]);
-          acc[3][c] = fmaf(av.w, y, acc[3][c]);
+    for (int v = 0; v < 4; ++v) {
+      const long long e = (long long)(i0 + 4 * v + g) * RHS + r;
+      const double pv = P[e];
+      const double kp = ((acc[v] + a1[v]) + a2[v]) + a3[v];
+      const double qv = fma(lam, pv, kp);
+      if (mode == 0) {
+        Q[e] = qv;
+        d += pv * qv;
+      } else {
+        const double dv = B[e] - qv;
+        if (R) {
+          R[e] = dv;
+          R32[e] = (float)dv;
         }
+        d += dv * dv;
       }
     }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int c = 0; c < NR; ++c)
-        red[(wave * GTC + 4 * lane + u) * NR + c] = acc[u][c];
-    __syncthreads();
+    d += __shfl_xor(d, 16);
+    d += __shfl_xor(d, 32);
+    if (g == 0) dots[s][r] = d;
   }
-  for (int e = t; e < GTC * NR; e += 256) {
-    const int j = e / NR;
-    if (j0 + j >= m) continue;
-    float v = 0.0f;
-    if (!zero)
-      v = red[e] + red[GTC * NR + e] + red[2 * GTC * NR + e] +
-          red[3 * GTC * NR + e];
-    P[((long long)rb * m + j0) * NR + e] = v;
-  }
+  __syncthreads();
+  if (t < RHS) part[(long long)blockIdx.x * RHS + t] = dots[0][t] + dots[1][t];
 }
 
-// out[i][c] = s1 sum_k P[k][i][c] + s2 V[i][c] (V optional), i < len.
-// With r64 given, also rz partials: rzp[slot0 + block][c] =
-// sum_i r64[i][c] out[i][c] (r64 indexed from the same origin as out).
-template <int NR>
-__global__ __launch_bounds__(256) void k_reduce(
-    const float* __restrict__ P, int np, int len, float s1,
-    const float* __restrict__ V, float s2, float* __restrict__ out,
-    const double* __restrict__ r64, double* __restrict__ rzp, int slot0) {
-  __shared__ double red[VR * NR];
-  const int t = threadIdx.x;
-  const int i0 = blockIdx.x * VR;
-  for (int e = t; e < VR * NR; e += 256) {
-    const int i = i0 + e / NR, c = e % NR;
-    double rz = 0.0;
-    if (i < len) {
-      const long long g = (long long)i * NR + c;
-      float v = 0.0f;
-      for (int k = 0; k < np; ++k) v += P[(long long)k * len * NR + g];
-      v *= s1;
-      if (V) v = fmaf(s2, V[g], v);
-      out[g] = v;
-      if (r64) rz = r64[g] * (double)v;
-    }
-    red[e] = rz;
+// Out = sa (A X) + sv V (fp32) for a row-major block A (m x k) on
+// v_mfma_f32_16x16x4f32: a workgroup of 8 waves owns 16 output rows, the
+// waves take the 16-column chunks round-robin and are added through LDS in
+// a fixed order.  tri: A is lower (1) or upper (2) triangular.  With r64
+// given, rzp[wg][c] = sum_rows r64 * Out (fp64).
+__global__ __launch_bounds__(512) void k_gemv_rows(
+    const float* __restrict__ A, long long lda, int m, int k, int tri,
+    const float* __restrict__ X, float* __restrict__ Out, float sa,
+    const float* __restrict__ V, float sv, const double* __restrict__ r64,
+    double* __restrict__ rzp) {
+  __shared__ floatx4 red[8][64];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int r = lane & 15, g = lane >> 4;
+  const int i0 = blockIdx.x * 16;
+  int kb, ke;
+  tri_range(tri, i0, k, kb, ke);
+  floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  row_stripe_product(acc, A + (long long)(i0 + r) * lda + 4 * g,
+                     X + (long long)(4 * g) * RHS + r, kb / 16 + w, ke / 16,
+                     8);
+  red[w][lane] = acc;
+  __syncthreads();
+  if (w != 0) return;
+  double d = 0.0;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    float o = acc[v];
+#pragma unroll
+    for (int u = 1; u < 8; ++u) o += red[u][lane][v];
+    const long long e = (long long)(i0 + 4 * g + v) * RHS + r;
+    o *= sa;
+    if (V) o = fmaf(sv, V[e], o);
+    Out[e] = o;
+    if (r64) d += r64[e] * (double)o;
   }
   if (!r64) return;
+  d += __shfl_xor(d, 16);
+  d += __shfl_xor(d, 32);
+  if (g == 0) rzp[(long long)blockIdx.x * RHS + r] = d;
+}
+
+// Per-column sums of nslot partials [slot][RHS] by a 256-thread block:
+// thread t adds slots t/16, t/16 + 16, ... of column t%16, then a fixed
+// order over the 16 strands.  Result in out[RHS] (shared); ends with a
+// barrier.
+__device__ __forceinline__ void block_slot_sums(const double* __restrict__ s,
+                                                int nslot,
+                                                double* __restrict__ scratch,
+                                                double* __restrict__ out) {
+  const int t = threadIdx.x;
+  const int c = t % RHS, strand = t / RHS;
+  double v = 0.0;
+  for (int k = strand; k < nslot; k += 256 / RHS) v += s[k * RHS + c];
+  scratch[t] = v;
   __syncthreads();
-  if (t < NR) {
-    double s = 0.0;
-    for (int k = 0; k < VR; ++k) s += red[k * NR + t];
-    rzp[(long long)(slot0 + blockIdx.x) * NR + t] = s;
+  if (t < RHS) {
+    double a = 0.0;
+    for (int q = 0; q < 256 / RHS; ++q) a += scratch[q * RHS + t];
+    out[t] = a;
+  }
+  __syncthreads();
+}
+
+// P_new = Z + beta P_old, beta = rz_new / rz_old per column (0 on the first
+// step or when rz_old = 0).
+__global__ __launch_bounds__(256) void k_pcg_p(
+    int n, const float* __restrict__ Z, const double* __restrict__ P_old,
+    double* __restrict__ P_new, const double* __restrict__ rz_new,
+    const double* __restrict__ rz_old, int nslot, int first) {
+  __shared__ double beta[RHS], num[RHS], den[RHS], scratch[256];
+  const int t = threadIdx.x;
+  if (!first) {
+    block_slot_sums(rz_new, nslot, scratch, num);
+    block_slot_sums(rz_old, nslot, scratch, den);
+  }
+  if (t < RHS) beta[t] = (first || den[t] == 0.0) ? 0.0 : num[t] / den[t];
+  __syncthreads();
+  const long long e0 = (long long)blockIdx.x * VROWS * RHS;
+  for (int e = t; e < VROWS * RHS; e += 256) {
+    const long long g = e0 + e;
+    if (g >= (long long)n * RHS) break;
+    P_new[g] = first ? (double)Z[g] : fma(beta[e % RHS], P_old[g],
+                                          (double)Z[g]);
   }
 }
 
-// b, r <- B (nrhs columns, zero padded to NR); r32 = b; x = p0 = p1 = 0;
-// bb_part = per-block sum of b^2.
-template <int NR>
-__global__ __launch_bounds__(256) void k_pcg_init(
-    int n, const double* __restrict__ B, int nrhs, double* __restrict__ b,
-    double* __restrict__ r, float* __restrict__ r32, double* __restrict__ x,
-    double* __restrict__ p0, double* __restrict__ p1,
-    double* __restrict__ bb_part) {
-  __shared__ double red[VR * NR];
+// alpha = rz / pq per column; X += alpha P; R -= alpha Q; R32 = R;
+// rr[wg][c] = sum_rows R^2.
+__global__ __launch_bounds__(256) void k_pcg_xr(
+    int n, const double* __restrict__ pq, int npq,
+    const double* __restrict__ rz, int nrz, const double* __restrict__ P,
+    const double* __restrict__ Q, double* __restrict__ X,
+    double* __restrict__ R, float* __restrict__ R32,
+    double* __restrict__ rr) {
+  __shared__ double alpha[RHS], num[RHS], den[RHS];
+  __shared__ double red[256];
   const int t = threadIdx.x;
-  const int i0 = blockIdx.x * VR;
-  for (int e = t; e < VR * NR; e += 256) {
-    const int i = i0 + e / NR, c = e % NR;
+  block_slot_sums(pq, npq, red, den);
+  block_slot_sums(rz, nrz, red, num);
+  if (t < RHS) alpha[t] = den[t] != 0.0 ? num[t] / den[t] : 0.0;
+  __syncthreads();
+  const int c = t % RHS;
+  const long long e0 = (long long)blockIdx.x * VROWS * RHS;
+  double s = 0.0;
+  for (int e = t; e < VROWS * RHS; e += 256) {
+    const long long g = e0 + e;
+    if (g >= (long long)n * RHS) break;
+    X[g] = fma(alpha[c], P[g], X[g]);
+    const double rv = fma(-alpha[c], Q[g], R[g]);
+    R[g] = rv;
+    R32[g] = (float)rv;
+    s += rv * rv;
+  }
+  red[t] = s;
+  __syncthreads();
+  if (t < RHS) {
     double v = 0.0;
-    if (i < n) {
-      const long long g = (long long)i * NR + c;
-      v = c < nrhs ? B[(long long)i * nrhs + c] : 0.0;
-      b[g] = v;
-      r[g] = v;
-      r32[g] = (float)v;
-      x[g] = 0.0;
-      p0[g] = 0.0;
-      p1[g] = 0.0;
-    }
-    red[e] = v * v;
-  }
-  __syncthreads();
-  if (t < NR) {
-    double s = 0.0;
-    for (int k = 0; k < VR; ++k) s += red[k * NR + t];
-    bb_part[(long long)blockIdx.x * NR + t] = s;
+    for (int k = t; k < 256; k += RHS) v += red[k];
+    rr[(long long)blockIdx.x * RHS + t] = v;
   }
 }
 
-// True residual d = b - (K + lambda I) x from the (K x) partials, and its
-// per-block sum of squares in rr_part.  With r given, the CG residual is
-// replaced by d (r = d, r32 = d); with X given, x (nrhs columns) is copied
-// out.
-template <int NR>
-__global__ __launch_bounds__(256) void k_pcg_resid(
-    int n, double lam, int ncb, const double* __restrict__ P64,
-    const double* __restrict__ b, const double* __restrict__ x,
-    double* __restrict__ r, float* __restrict__ r32, int nrhs,
-    double* __restrict__ X, double* __restrict__ rr_part) {
-  __shared__ double red[VR * NR];
+// B, R <- rhs (nrhs columns, zero padded); R32 = B; X = P0 = 0;
+// bb[wg][c] = sum_rows B^2.  With X_out given instead, copies X (nrhs
+// columns) out.
+__global__ __launch_bounds__(256) void k_pcg_io(
+    int n, int nrhs, const double* __restrict__ rhs, double* __restrict__ B,
+    double* __restrict__ R, float* __restrict__ R32, double* __restrict__ X,
+    double* __restrict__ P0, double* __restrict__ bb,
+    double* __restrict__ X_out) {
+  __shared__ double red[256];
   const int t = threadIdx.x;
-  const int i0 = blockIdx.x * VR;
-  for (int e = t; e < VR * NR; e += 256) {
-    const int i = i0 + e / NR, c = e % NR;
-    double rr = 0.0;
-    if (i < n) {
-      const long long g = (long long)i * NR + c;
-      double qv = 0.0;
-      for (int k = 0; k < ncb; ++k) qv += P64[((long long)k * n + i) * NR + c];
-      qv = fma(lam, x[g], qv);
-      const double d = b[g] - qv;
-      rr = d * d;
-      if (r) {
-        r[g] = d;
-        r32[g] = (float)d;
-      }
-      if (X && c < nrhs) X[(long long)i * nrhs + c] = x[g];
+  const int c = t % RHS;
+  const long long e0 = (long long)blockIdx.x * VROWS * RHS;
+  double s = 0.0;
+  for (int e = t; e < VROWS * RHS; e += 256) {
+    const long long g = e0 + e;
+    if (g >= (long long)n * RHS) break;
+    const long long i = g / RHS;
+    if (X_out) {
+      if (c < nrhs) X_out[i * nrhs + c] = X[g];
+      continue;
     }
-    red[e] = rr;
+    const double v = c < nrhs ? rhs[i * nrhs + c] : 0.0;
+    B[g] = v;
+    R[g] = v;
+    R32[g] = (float)v;
+    X[g] = 0.0;
+    P0[g] = 0.0;
+    s += v * v;
   }
+  if (X_out) return;
+  red[t] = s;
   __syncthreads();
-  if (t < NR) {
-    double s = 0.0;
-    for (int k = 0; k < VR; ++k) s += red[k * NR + t];
-    rr_part[(long long)blockIdx.x * NR + t] = s;
+  if (t < RHS) {
+    double v = 0.0;
+    for (int k = t; k < 256; k += RHS) v += red[k];
+    bb[(long long)blockIdx.x * RHS + t] = v;
   }
 }
 
-// fp32 [n][nrhs] <-> padded [n][NR]
-template <int NR>
+// fp32 [n][nrhs] <-> padded [n][RHS]
 __global__ __launch_bounds__(256) void k_pad32(int n, const float* __restrict__ src,
                                                int nrhs, float* __restrict__ dst,
                                                int to_padded) {
   const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (long long)n * NR) return;
-  const long long i = e / NR;
-  const int c = (int)(e % NR);
+  if (e >= (long long)n * RHS) return;
+  const long long i = e / RHS;
+  const int c = (int)(e % RHS);
   if (to_padded)
     dst[e] = c < nrhs ? src[i * nrhs + c] : 0.0f;
   else if (c < nrhs)
@@ -788,19 +873,12 @@ __global__ __launch_bounds__(256) void k_pad32(int n, const float* __restrict__ 
 
 // ------------------------------------------------------------ host side
 struct Ws {
-  double *b, *x, *r, *p0, *p1, *P64, *pq, *rzp0, *rzp1, *rr, *bb;
-  float *r32, *z32, *y1, *u2, *y2, *v1, *P32;
+  double *b, *x, *r, *p0, *p1, *q, *pq, *rz0, *rz1, *rr, *bb;
+  float *r32, *z32, *y1, *u2, *y2, *v1;
   size_t bytes;
 };
 
-static int pick_nr(int nrhs) {
-  if (nrhs <= 4) return 4;
-  if (nrhs <= 8) return 8;
-  if (nrhs <= 12) return 12;
-  return 16;
-}
-
-static Ws carve(char* base, int n, int NR) {
+static Ws carve(char* base, int n) {
   Ws w;
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -808,203 +886,157 @@ static Ws carve(char* base, int n, int NR) {
     off += (bytes + 255) & ~(size_t)255;
     return p;
   };
-  const size_t v64 = (size_t)n * NR * sizeof(double);
-  const size_t v32 = (size_t)n * NR * sizeof(float);
-  const int ncb = (n + SV_C - 1) / SV_C;
-  const int nslot = (n + VR - 1) / VR;
-  const int ntiles = (n / SV_R) * ncb;
-  const int np_max = (n + GTR - 1) / GTR > ncb ? (n + GTR - 1) / GTR : ncb;
+  const size_t v64 = (size_t)n * RHS * sizeof(double);
+  const size_t v32 = (size_t)n * RHS * sizeof(float);
+  const size_t slots = (size_t)((n + 15) / 16) * RHS * sizeof(double);
   w.b = (double*)take(v64);
   w.x = (double*)take(v64);
   w.r = (double*)take(v64);
   w.p0 = (double*)take(v64);
   w.p1 = (double*)take(v64);
-  w.P64 = (double*)take(v64 * ncb);
-  w.pq = (double*)take((size_t)ntiles * NR * sizeof(double));
-  w.rzp0 = (double*)take((size_t)nslot * NR * sizeof(double));
-  w.rzp1 = (double*)take((size_t)nslot * NR * sizeof(double));
-  w.rr = (double*)take((size_t)nslot * NR * sizeof(double));
-  w.bb = (double*)take((size_t)nslot * NR * sizeof(double));
+  w.q = (double*)take(v64);
+  w.pq = (double*)take(slots);
+  w.rz0 = (double*)take(slots);
+  w.rz1 = (double*)take(slots);
+  w.rr = (double*)take(slots);
+  w.bb = (double*)take(slots);
   w.r32 = (float*)take(v32);
   w.z32 = (float*)take(v32);
   w.y1 = (float*)take(v32);
   w.u2 = (float*)take(v32);
   w.y2 = (float*)take(v32);
   w.v1 = (float*)take(v32);
-  w.P32 = (float*)take(v32 * np_max);
   w.bytes = off;
   return w;
 }
 
-template <int NR>
-static void gemv_n(const float* A, long long lda, int m, int k, int tri,
-                   const float* X, float* P, hipStream_t st) {
-  dim3 grid((k + SV_C - 1) / SV_C, (m + SV_R - 1) / SV_R);
-  hipLaunchKernelGGL(k_gemv_n<NR>, grid, dim3(256), 0, st, A, lda, m, k, tri,
-                     X, P);
-}
-template <int NR>
-static void gemv_t(const float* A, long long lda, int k, int m, int tri,
-                   const float* Y, float* P, hipStream_t st) {
-  dim3 grid((m + GTC - 1) / GTC, (k + GTR - 1) / GTR);
-  hipLaunchKernelGGL(k_gemv_t<NR>, grid, dim3(256), 0, st, A, lda, k, m, tri,
-                     Y, P);
-}
-template <int NR>
-static void reduce(const float* P, int np, int len, float s1, const float* V,
-                   float s2, float* out, const double* r64, double* rzp,
-                   int slot0, hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce<NR>, dim3((len + VR - 1) / VR), dim3(256), 0,
-                     st, P, np, len, s1, V, s2, out, r64, rzp, slot0);
+static void gemv(const float* A, long long lda, int m, int k, int tri,
+                 const float* X, float* out, float sa, const float* V,
+                 float sv, const double* r64, double* rzp, hipStream_t st) {
+  hipLaunchKernelGGL(k_gemv_rows, dim3(m / 16), dim3(512), 0, st, A, lda, m,
+                     k, tri, X, out, sa, V, sv, r64, rzp);
 }
 
-// z = (L L^T)^-1 r (fp32) from the blocks in W; rz partials into rzp when
-// r64 is given.
-template <int NR>
-static void precond(const float* W, long long ldw, int n, int n1,
-                    const float* r32, float* z32, const double* r64,
+// z = (L L^T)^-1 r (fp32) from W (W11, L21, W22) and WT = W^T; rz partials
+// (one slot per 16 rows, n / 16 slots) when r64 is given.
+static void precond(const float* W, const float* WT, long long ld, int n,
+                    int n1, const float* r32, float* z32, const double* r64,
                     double* rzp, const Ws& w, hipStream_t st) {
   const int n2 = n - n1;
-  const float* W11 = W;
-  const int npn1 = (n1 + SV_C - 1) / SV_C, npt1 = (n1 + GTR - 1) / GTR;
-  gemv_n<NR>(W11, ldw, n1, n1, 1, r32, w.P32, st);
-  reduce<NR>(w.P32, npn1, n1, 1.0f, nullptr, 0.0f, w.y1, nullptr, nullptr, 0,
-             st);
-  const float* v1 = w.y1;
-  if (n2 > 0) {
-    const float* S = W + (long long)n1 * ldw;
-    const float* W22 = S + n1;
-    const int npn2 = (n2 + SV_C - 1) / SV_C, npt2 = (n2 + GTR - 1) / GTR;
-    const long long off2 = (long long)n1 * NR;
-    gemv_n<NR>(S, ldw, n2, n1, 0, w.y1, w.P32, st);
-    reduce<NR>(w.P32, npn1, n2, -1.0f, r32 + off2, 1.0f, w.u2, nullptr,
-               nullptr, 0, st);
-    gemv_n<NR>(W22, ldw, n2, n2, 1, w.u2, w.P32, st);
-    reduce<NR>(w.P32, npn2, n2, 1.0f, nullptr, 0.0f, w.y2, nullptr, nullptr,
-               0, st);
-    gemv_t<NR>(W22, ldw, n2, n2, 1, w.y2, w.P32, st);
-    reduce<NR>(w.P32, npt2, n2, 1.0f, nullptr, 0.0f, z32 + off2,
-               r64 ? r64 + off2 : nullptr, rzp, n1 / VR, st);
-    gemv_t<NR>(S, ldw, n2, n1, 0, z32 + off2, w.P32, st);
-    reduce<NR>(w.P32, npt2, n1, -1.0f, w.y1, 1.0f, w.v1, nullptr, nullptr, 0,
-               st);
-    v1 = w.v1;
+  const long long o2 = (long long)n1 * RHS;
+  if (n2 == 0) {
+    gemv(W, ld, n, n, 1, r32, w.y1, 1.0f, nullptr, 0.0f, nullptr, nullptr,
+         st);
+    gemv(WT, ld, n, n, 2, w.y1, z32, 1.0f, nullptr, 0.0f, r64, rzp, st);
+    return;
   }
-  gemv_t<NR>(W11, ldw, n1, n1, 1, v1, w.P32, st);
-  reduce<NR>(w.P32, npt1, n1, 1.0f, nullptr, 0.0f, z32, r64, rzp, 0, st);
+  const float* S = W + (long long)n1 * ld;            // L21 (n2 x n1)
+  const float* W22 = S + n1;
+  const float* ST = WT + n1;                           // L21^T (n1 x n2)
+  const float* W22T = WT + (long long)n1 * ld + n1;
+  gemv(W, ld, n1, n1, 1, r32, w.y1, 1.0f, nullptr, 0.0f, nullptr, nullptr,
+       st);                                            // y1 = W11 r1
+  gemv(S, ld, n2, n1, 0, w.y1, w.u2, -1.0f, r32 + o2, 1.0f, nullptr,
+       nullptr, st);                                   // u2 = r2 - S y1
+  gemv(W22, ld, n2, n2, 1, w.u2, w.y2, 1.0f, nullptr, 0.0f, nullptr,
+       nullptr, st);                                   // y2 = W22 u2
+  gemv(W22T, ld, n2, n2, 2, w.y2, z32 + o2, 1.0f, nullptr, 0.0f,
+       r64 ? r64 + o2 : nullptr, rzp ? rzp + (long long)(n1 / 16) * RHS
+                                     : nullptr,
+       st);                                            // z2 = W22^T y2
+  gemv(ST, ld, n1, n2, 0, z32 + o2, w.v1, -1.0f, w.y1, 1.0f, nullptr,
+       nullptr, st);                                   // v1 = y1 - S^T z2
+  gemv(WT, ld, n1, n1, 2, w.v1, z32, 1.0f, nullptr, 0.0f, r64, rzp,
+       st);                                            // z1 = W11^T v1
 }
 
-template <int NR>
-static void symv(const float* K, long long ldk, int n, double lam, int mode,
-                 const float* z, const double* p_old, double* p_new,
-                 const double* rz_new, const double* rz_old, int first,
-                 const Ws& w, hipStream_t st) {
-  const int ncb = (n + SV_C - 1) / SV_C;
-  dim3 grid(ncb, n / SV_R);
-  hipLaunchKernelGGL(k_symv_part<NR>, grid, dim3(256), 0, st, K, ldk, n, lam,
-                     mode, z, p_old, p_new, rz_new, rz_old, (n + VR - 1) / VR,
-                     first, w.P64, w.pq);
-}
-
-template <int NR>
 static int pcg_run(const float* K, int n, long long ldk, double lam,
-                   const float* W, long long ldw, int n1, const double* B,
-                   int nrhs, double* X, double tol, int max_iter, char* work,
-                   int* iters, double* resid, hipStream_t st) {
-  const Ws w = carve(work, n, NR);
-  const int ncb = (n + SV_C - 1) / SV_C;
-  const int nslot = (n + VR - 1) / VR;
-  const int ntiles = (n / SV_R) * ncb;
-  std::vector<double> host((size_t)nslot * NR);
-  auto col_sums = [&](const double* dev, double* out) -> int {
+                   const float* W, const float* WT, long long ldw, int n1,
+                   const double* Bin, int nrhs, double* Xout, double tol,
+                   int max_iter, char* work, int* iters, double* resid,
+                   hipStream_t st) {
+  const Ws w = carve(work, n);
+  const int nvec = (n + VROWS - 1) / VROWS;    // vector-kernel workgroups
+  const int nsym = n / 32;                     // k_symv_rows workgroups
+  const int nrz = n / 16;                      // rz slots
+  std::vector<double> host((size_t)(n / 16 + nvec) * RHS);
+  auto col_sums = [&](const double* dev, int nslot, double* out) -> int {
     hipError_t e = hipMemcpyAsync(host.data(), dev,
-                                  host.size() * sizeof(double),
+                                  (size_t)nslot * RHS * sizeof(double),
                                   hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess)
       return set_error((int)e, "nta_pcg64: %s", hipGetErrorString(e));
-    for (int c = 0; c < NR; ++c) {
+    for (int c = 0; c < RHS; ++c) {
       double s = 0.0;
-      for (int k = 0; k < nslot; ++k) s += host[(size_t)k * NR + c];
+      for (int k = 0; k < nslot; ++k) s += host[(size_t)k * RHS + c];
       out[c] = s;
     }
     return 0;
   };
-  hipLaunchKernelGGL(k_pcg_init<NR>, dim3(nslot), dim3(256), 0, st, n, B,
-                     nrhs, w.b, w.r, w.r32, w.x, w.p0, w.p1, w.bb);
-  double bn[NR], rn[NR];
-  int rc = col_sums(w.bb, bn);
+  hipLaunchKernelGGL(k_pcg_io, dim3(nvec), dim3(256), 0, st, n, nrhs, Bin,
+                     w.b, w.r, w.r32, w.x, w.p0, w.bb, nullptr);
+  double bn[RHS], rn[RHS];
+  int rc = col_sums(w.bb, nvec, bn);
   if (rc) return rc;
-  for (int c = 0; c < NR; ++c) bn[c] = sqrt(bn[c]);
-  precond<NR>(W, ldw, n, n1, w.r32, w.z32, w.r, w.rzp0, w, st);
-  double* pbuf[2] = {w.p0, w.p1};
-  double* rzp[2] = {w.rzp0, w.rzp1};
-  // Relative residual (max over the columns) from per-column sums.
+  for (int c = 0; c < RHS; ++c) bn[c] = sqrt(bn[c]);
   auto rel_of = [&](const double* ss) {
     double v = 0.0;
     for (int c = 0; c < nrhs; ++c)
       if (bn[c] > 0.0) v = fmax(v, sqrt(ss[c]) / bn[c]);
     return v;
   };
-  // r <- b - A x recomputed in FP64 (residual replacement).
-  auto true_residual = [&](double* r, float* r32, double* X) -> int {
-    symv<NR>(K, ldk, n, lam, 1, nullptr, w.x, nullptr, nullptr, nullptr, 0, w,
-             st);
-    hipLaunchKernelGGL(k_pcg_resid<NR>, dim3(nslot), dim3(256), 0, st, n,
-                       lam, ncb, w.P64, w.b, w.x, r, r32, nrhs, X, w.rr);
-    return col_sums(w.rr, rn);
+  // r <- b - A x (FP64), replacing the recurrence residual when R is given.
+  auto true_residual = [&](double* R, float* R32) -> int {
+    hipLaunchKernelGGL(k_symv_rows, dim3(nsym), dim3(512), 0, st, K, ldk, n,
+                       lam, 1, w.x, nullptr, w.b, R, R32, w.rr);
+    return col_sums(w.rr, nsym, rn);
   };
+  precond(W, WT, ldw, n, n1, w.r32, w.z32, w.r, w.rz0, w, st);
+  double* pbuf[2] = {w.p0, w.p1};
+  double* rzs[2] = {w.rz0, w.rz1};
   double best = INFINITY;
   int stall = 0, it = 0;
   for (it = 1; it <= max_iter; ++it) {
     const double* p_old = pbuf[(it + 1) & 1];
     double* p_new = pbuf[it & 1];
-    symv<NR>(K, ldk, n, lam, 0, w.z32, p_old, p_new, rzp[(it + 1) & 1],
-             rzp[it & 1], it == 1, w, st);
-    hipLaunchKernelGGL(k_pcg_q<NR>, dim3(nslot), dim3(256), 0, st, n, lam,
-                       ncb, w.P64, w.pq, ntiles, rzp[(it + 1) & 1], nslot,
-                       p_new, w.x, w.r, w.r32, w.rr);
-    rc = col_sums(w.rr, rn);
+    const double* rz_new = rzs[(it + 1) & 1];
+    hipLaunchKernelGGL(k_pcg_p, dim3(nvec), dim3(256), 0, st, n, w.z32,
+                       p_old, p_new, rz_new, rzs[it & 1], nrz, it == 1);
+    hipLaunchKernelGGL(k_symv_rows, dim3(nsym), dim3(512), 0, st, K, ldk, n,
+                       lam, 0, p_new, w.q, nullptr, nullptr, nullptr, w.pq);
+    hipLaunchKernelGGL(k_pcg_xr, dim3(nvec), dim3(256), 0, st, n, w.pq, nsym,
+                       rz_new, nrz, p_new, w.q, w.x, w.r, w.r32, w.rr);
+    rc = col_sums(w.rr, nvec, rn);
     if (rc) return rc;
     double rel = rel_of(rn);
     // The recurrence drifts from b - A x on ill-conditioned systems: every
     // 10 iterations, and before accepting convergence, r is recomputed.
+    // CG residual norms are not monotone, so stagnation is judged on these
+    // checkpoints only: 5 in a row without a new best stop the iteration.
     if (!(rel > tol) || it % 10 == 0) {
-      rc = true_residual(w.r, w.r32, nullptr);
+      rc = true_residual(w.r, w.r32);
       if (rc) return rc;
       rel = rel_of(rn);
       if (!(rel > tol)) break;
+      if (rel < best) {
+        best = rel;
+        stall = 0;
+      } else if (++stall >= 5) {
+        break;
+      }
     }
-    if (rel < best) {
-      best = rel;
-      stall = 0;
-    } else if (++stall >= 20) {
-      break;
-    }
-    precond<NR>(W, ldw, n, n1, w.r32, w.z32, w.r, rzp[it & 1], w, st);
+    precond(W, WT, ldw, n, n1, w.r32, w.z32, w.r, rzs[it & 1], w, st);
   }
   if (it > max_iter) it = max_iter;
-  // True residual of the returned x.
-  rc = true_residual(nullptr, nullptr, X);
+  rc = true_residual(nullptr, nullptr);
   if (rc) return rc;
+  hipLaunchKernelGGL(k_pcg_io, dim3(nvec), dim3(256), 0, st, n, nrhs,
+                     nullptr, nullptr, nullptr, nullptr, w.x, nullptr,
+                     nullptr, Xout);
   if (iters) *iters = it;
   if (resid) *resid = rel_of(rn);
   return check_launch("nta_pcg64");
-}
-
-template <int NR>
-static int precond_run(const float* W, int n, long long ldw, int n1,
-                       const float* R, int nrhs, float* Z, char* work,
-                       hipStream_t st) {
-  const Ws w = carve(work, n, NR);
-  const long long tot = (long long)n * NR;
-  const unsigned g = (unsigned)((tot + 255) / 256);
-  hipLaunchKernelGGL(k_pad32<NR>, dim3(g), dim3(256), 0, st, n, R, nrhs,
-                     w.r32, 1);
-  precond<NR>(W, ldw, n, n1, w.r32, w.z32, nullptr, nullptr, w, st);
-  hipLaunchKernelGGL(k_pad32<NR>, dim3(g), dim3(256), 0, st, n, w.z32, nrhs,
-                     Z, 0);
-  return check_launch("nta_chol_precond");
 }
 
 }  // namespace nta
@@ -1017,67 +1049,74 @@ extern "C" int nta_chol_split(int n) {
 }
 
 extern "C" int nta_chol_f32(float* A, int n, long long lda, float* W,
-                            long long ldw, int* info, void* stream) {
-  if (n <= 0 || lda < n || ldw < n || !A || !W || !info)
+                            long long ldw, float* WT, int* info,
+                            void* stream) {
+  if (n <= 0 || lda < n || ldw < n || !A || !W || !WT || !info)
     return set_error(NTA_EINVAL, "nta_chol_f32: bad n=%d lda=%lld ldw=%lld "
                      "or null pointer", n, lda, ldw);
   if (n % 64 != 0 || lda % 4 != 0 || ldw % 4 != 0)
     return set_error(NTA_EUNSUPPORTED, "nta_chol_f32: n=%d, lda and ldw must "
                      "be multiples of 64, 4 and 4 (pad the matrix)", n);
-  chol_node(A, lda, W, ldw, 0, n, true, info, (hipStream_t)stream);
+  hipStream_t st = (hipStream_t)stream;
+  chol_node(A, lda, W, ldw, 0, n, true, info, st);
+  dim3 grid((n + 31) / 32, (n + 31) / 32);
+  hipLaunchKernelGGL(k_transpose_lower, grid, dim3(256), 0, st, W, ldw, n,
+                     WT, ldw);
   return check_launch("nta_chol_f32");
 }
 
-#define NTA_NR_DISPATCH(NRV, FN, ...)  \
-  switch (NRV) {                         \
-    case 4: return FN<4>(__VA_ARGS__);   \
-    case 8: return FN<8>(__VA_ARGS__);   \
-    case 12: return FN<12>(__VA_ARGS__); \
-    default: return FN<16>(__VA_ARGS__); \
-  }
-
 extern "C" long long nta_pcg64_work_bytes(int n, int nrhs) {
-  if (n <= 0 || nrhs <= 0 || nrhs > 16) return -1;
-  return (long long)carve(nullptr, n, pick_nr(nrhs)).bytes;
+  if (n <= 0 || nrhs <= 0 || nrhs > RHS) return -1;
+  return (long long)carve(nullptr, n).bytes;
 }
 
-extern "C" int nta_chol_precond(const float* W, int n, long long ldw, int n1,
-                                const float* R, int nrhs, float* Z,
-                                void* work, long long work_bytes,
-                                void* stream) {
+static int check_solve_args(const char* who, int n, long long ldw, int n1,
+                            int nrhs, long long work_bytes) {
   if (n <= 0 || n % 64 != 0 || ldw < n || ldw % 4 != 0 || nrhs <= 0 ||
-      nrhs > 16 || !W || !R || !Z || !work)
-    return set_error(NTA_EINVAL, "nta_chol_precond: bad n=%d ldw=%lld "
-                     "nrhs=%d or null pointer", n, ldw, nrhs);
+      nrhs > RHS)
+    return set_error(NTA_EINVAL, "%s: bad n=%d ldw=%lld nrhs=%d", who, n, ldw,
+                     nrhs);
   if (n1 != nta_chol_split(n))
-    return set_error(NTA_EINVAL, "nta_chol_precond: n1=%d is not the split "
-                     "%d of nta_chol_f32", n1, nta_chol_split(n));
+    return set_error(NTA_EINVAL, "%s: n1=%d is not the split %d of "
+                     "nta_chol_f32", who, n1, nta_chol_split(n));
   if (work_bytes < nta_pcg64_work_bytes(n, nrhs))
-    return set_error(NTA_EINVAL, "nta_chol_precond: work_bytes=%lld < %lld",
+    return set_error(NTA_EINVAL, "%s: work_bytes=%lld < %lld", who,
                      work_bytes, nta_pcg64_work_bytes(n, nrhs));
+  return 0;
+}
+
+extern "C" int nta_chol_precond(const float* W, const float* WT, int n,
+                                long long ldw, int n1, const float* R,
+                                int nrhs, float* Z, void* work,
+                                long long work_bytes, void* stream) {
+  int rc = check_solve_args("nta_chol_precond", n, ldw, n1, nrhs,
+                            work_bytes);
+  if (rc) return rc;
+  if (!W || !WT || !R || !Z || !work)
+    return set_error(NTA_EINVAL, "nta_chol_precond: null pointer");
   hipStream_t st = (hipStream_t)stream;
-  NTA_NR_DISPATCH(pick_nr(nrhs), precond_run, W, n, ldw, n1, R, nrhs, Z,
-                  (char*)work, st);
+  const Ws w = carve((char*)work, n);
+  const unsigned g = (unsigned)(((long long)n * RHS + 255) / 256);
+  hipLaunchKernelGGL(k_pad32, dim3(g), dim3(256), 0, st, n, R, nrhs, w.r32,
+                     1);
+  precond(W, WT, ldw, n, n1, w.r32, w.z32, nullptr, nullptr, w, st);
+  hipLaunchKernelGGL(k_pad32, dim3(g), dim3(256), 0, st, n, w.z32, nrhs, Z,
+                     0);
+  return check_launch("nta_chol_precond");
 }
 
 extern "C" int nta_pcg64(const float* K, int n, long long ldk, double lam,
-                         const float* W, long long ldw, int n1,
-                         const double* B, int nrhs, double* X, double tol,
-                         int max_iter, void* work, long long work_bytes,
-                         int* iters, double* resid, void* stream) {
-  if (n <= 0 || n % 64 != 0 || ldk < n || ldk % 4 != 0 || ldw < n ||
-      ldw % 4 != 0 || nrhs <= 0 || nrhs > 16 || max_iter < 1 || !K || !W ||
-      !B || !X || !work)
-    return set_error(NTA_EINVAL, "nta_pcg64: bad n=%d ldk=%lld ldw=%lld "
-                     "nrhs=%d max_iter=%d or null pointer", n, ldk, ldw, nrhs,
-                     max_iter);
-  if (n1 != nta_chol_split(n))
-    return set_error(NTA_EINVAL, "nta_pcg64: n1=%d is not the split %d of "
-                     "nta_chol_f32", n1, nta_chol_split(n));
-  if (work_bytes < nta_pcg64_work_bytes(n, nrhs))
-    return set_error(NTA_EINVAL, "nta_pcg64: work_bytes=%lld < %lld",
-                     work_bytes, nta_pcg64_work_bytes(n, nrhs));
-  hipStream_t st = (hipStream_t)stream;
-  NTA_NR_DISPATCH(pick_nr(nrhs), pcg_run, K, n, ldk, lam, W, ldw, n1, B,
-                  nrhs, X, tol, max_iter, (char*)work, iters, resid, st);
+                         const float* W, const float* WT, long long ldw,
+                         int n1, const double* B, int nrhs, double* X,
+                         double tol, int max_iter, void* work,
+                         long long work_bytes, int* iters, double* resid,
+                         void* stream) {
+  int rc = check_solve_args("nta_pcg64", n, ldw, n1, nrhs, work_bytes);
+  if (rc) return rc;
+  if (ldk < n || ldk % 4 != 0 || max_iter < 1 || !K || !W || !WT || !B ||
+      !X || !work)
+    return set_error(NTA_EINVAL, "nta_pcg64: bad ldk=%lld max_iter=%d or "
+                     "null pointer", ldk, max_iter);
+  return pcg_run(K, n, ldk, lam, W, WT, ldw, n1, B, nrhs, X, tol, max_iter,
+                 (char*)work, iters, resid, (hipStream_t)stream);
 }
