@@ -21,7 +21,7 @@ from . import _lib
 class MixedCholesky:
   NB = 64
 
-  def __init__(self, K: torch.Tensor, lam: float, tol: float = 1e-12,
+  def __init__(self, K: torch.Tensor, lam: float, tol: float = 1e-10,
                max_iter: int = 200):
     if not K.is_cuda:
       raise RuntimeError('the mixed-precision solve runs on the GPU')

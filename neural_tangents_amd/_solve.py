@@ -34,7 +34,7 @@ class CholeskySolver:
 
   def __init__(self, A: torch.Tensor, diag_reg: float,
                diag_reg_absolute_scale: bool, mode: str = 'mixed',
-               tol: float = 1e-12, max_iter: int = 200):
+               tol: float = 1e-10, max_iter: int = 200):
     if A.ndim != 2 or A.shape[0] != A.shape[1]:
       raise ValueError(f'k_train_train must be square, got {tuple(A.shape)}')
     if mode not in ('mixed', 'fp64'):

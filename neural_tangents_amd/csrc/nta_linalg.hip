@@ -29,6 +29,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "nta_common.h"
@@ -93,60 +94,67 @@ __device__ __forceinline__ void gemm_tile(const GemmJob& J, int ti, int tj,
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.0f;
 
-  float4 ra[NLD], rb[NLD];
-  auto gload = [&](int k0) {
+  // Two register sets of global loads: stage s + 2 is fetched while stage s
+  // is multiplied and stage s + 1 goes from registers to LDS, so a load has
+  // two stages of MFMA work to arrive.
+  float4 ra[2][NLD], rb[2][NLD];
+  auto gload = [&](int k0, int set) {
 #pragma unroll
     for (int u = 0; u < NLD; ++u) {
       const int f = t + 256 * u;
       const int row = f >> 3, c4 = (f & 7) * 4;
-      ra[u] = (i0 + row < J.M)
-                  ? *(const float4*)(J.A + (long long)(i0 + row) * J.lda +
-                                     k0 + c4)
-                  : make_float4(0.f, 0.f, 0.f, 0.f);
+      ra[set][u] = (i0 + row < J.M)
+                       ? *(const float4*)(J.A + (long long)(i0 + row) * J.lda +
+                                          k0 + c4)
+                       : make_float4(0.f, 0.f, 0.f, 0.f);
       if (BKM) {
         constexpr int NQ = TILE / 4;            // float4 per k-row
         const int kr = f / NQ, n4 = (f % NQ) * 4;
-        rb[u] = (j0 + n4 < J.N)
-                    ? *(const float4*)(J.B + (long long)(k0 + kr) * J.ldb +
-                                       j0 + n4)
-                    : make_float4(0.f, 0.f, 0.f, 0.f);
+        rb[set][u] = (j0 + n4 < J.N)
+                         ? *(const float4*)(J.B + (long long)(k0 + kr) * J.ldb +
+                                            j0 + n4)
+                         : make_float4(0.f, 0.f, 0.f, 0.f);
       } else {
-        rb[u] = (j0 + row < J.N)
-                    ? *(const float4*)(J.B + (long long)(j0 + row) * J.ldb +
-                                       k0 + c4)
-                    : make_float4(0.f, 0.f, 0.f, 0.f);
+        rb[set][u] = (j0 + row < J.N)
+                         ? *(const float4*)(J.B + (long long)(j0 + row) * J.ldb +
+                                            k0 + c4)
+                         : make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
   };
-  auto lstore = [&](int buf) {
+  auto lstore = [&](int buf, int set) {
     float* as = As + buf * GBUF;
     float* bs = Bs + buf * GBUF;
 #pragma unroll
     for (int u = 0; u < NLD; ++u) {
       const int f = t + 256 * u;
       const int row = f >> 3, c4 = (f & 7) * 4;
-      *(float4*)(as + row * GSR + c4) = ra[u];
+      *(float4*)(as + row * GSR + c4) = ra[set][u];
       if (BKM) {
         constexpr int NQ = TILE / 4;
         const int kr = f / NQ, n4 = (f % NQ) * 4;
-        *(float4*)(bs + kr * GSK + n4) = rb[u];
+        *(float4*)(bs + kr * GSK + n4) = rb[set][u];
       } else {
-        *(float4*)(bs + row * GSR + c4) = rb[u];
+        *(float4*)(bs + row * GSR + c4) = rb[set][u];
       }
     }
   };
 
-  if (kb < ke) {
-    gload(kb);
-    lstore(0);
+  const int nst = kb < ke ? (ke - kb) / GK : 0;
+  if (nst > 0) {
+    gload(kb, 0);
+    if (nst > 1) gload(kb + GK, 1);
+    lstore(0, 0);
     __syncthreads();
   }
-  int buf = 0;
-  for (int k0 = kb; k0 < ke; k0 += GK) {
-    const bool more = k0 + GK < ke;
-    if (more) gload(k0 + GK);
-    const float* as = As + buf * GBUF;
-    const float* bs = Bs + buf * GBUF;
+  // The register set and LDS buffer of a stage alternate; the loop is
+  // unrolled by two so that both are compile-time (no dynamic indexing of
+  // the register arrays).
+  auto stage = [&](int st, auto CUR) {
+    constexpr int cur = decltype(CUR)::value;
+    if (st + 2 < nst) gload(kb + (st + 2) * GK, cur);
+    const float* as = As + cur * GBUF;
+    const float* bs = Bs + cur * GBUF;
     float af[MT][16], bf[MT][16];
 #pragma unroll
     for (int mi = 0; mi < MT; ++mi) {
@@ -186,9 +194,12 @@ __device__ __forceinline__ void gemm_tile(const GemmJob& J, int ti, int tj,
         for (int ni = 0; ni < MT; ++ni)
           acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x2f32(
               af[mi][kk], bf[ni][kk], acc[mi][ni], 0, 0, 0);
-    if (more) lstore(buf ^ 1);
+    if (st + 1 < nst) lstore(cur ^ 1, cur ^ 1);
     __syncthreads();
-    buf ^= 1;
+  };
+  for (int st = 0; st < nst; st += 2) {
+    stage(st, std::integral_constant<int, 0>());
+    if (st + 1 < nst) stage(st + 1, std::integral_constant<int, 1>());
   }
 
 #pragma unroll
@@ -224,8 +235,23 @@ __global__ __launch_bounds__(256) void k_gemm(GemmJobs jobs) {
     while (ti * (ti + 1) / 2 > b) --ti;
     tj = b - ti * (ti + 1) / 2;
   } else {
-    ti = b / J.tiles_n;
-    tj = b % J.tiles_n;
+    // Triangular operands give tiles of unequal k-length: dispatch the
+    // longest first (tri 1: long for large tj; tri 3: for large ti), so the
+    // in-order dispatch balances the CUs.
+    const int tm = J.ntiles / J.tiles_n;
+    if (J.tri == 1) {
+      tj = J.tiles_n - 1 - b / tm;
+      ti = b % tm;
+    } else if (J.tri == 3) {
+      ti = tm - 1 - b / J.tiles_n;
+      tj = b % J.tiles_n;
+    } else if (J.tri == 2) {
+      tj = b / tm;
+      ti = b % tm;
+    } else {
+      ti = b / J.tiles_n;
+      tj = b % J.tiles_n;
+    }
   }
   if (J.bkm)
     gemm_tile<TILE, true>(J, ti, tj, As, Bs);
@@ -475,12 +501,15 @@ __global__ __launch_bounds__(64) void k_leaf64(const float* __restrict__ A,
   }
 }
 
-// Split of a block of n rows into n1 + n2 (n1 a multiple of 64; at the top
-// level a multiple of 256 when n >= 512).
+// Split of a block of n rows into n1 + n2 (multiples of 64).  At the top
+// level the trailing block n2 is kept at or below 31 tiles of 128 when it
+// would be just above: its SYRK then has 496 lower tiles, one round on 256
+// CUs x 2 resident workgroups, instead of 528 (two rounds).
 static int split_of(int n, bool top) {
   if (n <= 64) return n;
-  if (top && n >= 512) return 256 * ((n / 256 + 1) / 2);
-  return 64 * ((n / 64 + 1) / 2);
+  int n1 = 64 * ((n / 64 + 1) / 2);
+  if (top && n - n1 > 31 * 128 && n - n1 <= 32 * 128) n1 = n - 31 * 128;
+  return n1;
 }
 
 static void chol_node(float* A, long long lda, float* W, long long ldw, int o,
@@ -759,19 +788,24 @@ __device__ __forceinline__ void block_slot_sums(const double* __restrict__ s,
   __syncthreads();
 }
 
-// P_new = Z + beta P_old, beta = rz_new / rz_old per column (0 on the first
-// step or when rz_old = 0).
+// P_new = Z + beta P_old with the Polak-Ribiere (flexible CG) coefficient
+// beta = (r+ . z+  -  r+ . z) / (r . z) per column, which keeps the
+// iteration robust to a preconditioner that is only approximately
+// symmetric (fp32 products); 0 on the first step or when r . z = 0.
 __global__ __launch_bounds__(256) void k_pcg_p(
     int n, const float* __restrict__ Z, const double* __restrict__ P_old,
     double* __restrict__ P_new, const double* __restrict__ rz_new,
-    const double* __restrict__ rz_old, int nslot, int first) {
-  __shared__ double beta[RHS], num[RHS], den[RHS], scratch[256];
+    const double* __restrict__ rz_old, int nslot,
+    const double* __restrict__ rz_cross, int ncross, int first) {
+  __shared__ double beta[RHS], num[RHS], den[RHS], crs[RHS], scratch[256];
   const int t = threadIdx.x;
   if (!first) {
     block_slot_sums(rz_new, nslot, scratch, num);
     block_slot_sums(rz_old, nslot, scratch, den);
+    block_slot_sums(rz_cross, ncross, scratch, crs);
   }
-  if (t < RHS) beta[t] = (first || den[t] == 0.0) ? 0.0 : num[t] / den[t];
+  if (t < RHS)
+    beta[t] = (first || den[t] == 0.0) ? 0.0 : (num[t] - crs[t]) / den[t];
   __syncthreads();
   const long long e0 = (long long)blockIdx.x * VROWS * RHS;
   for (int e = t; e < VROWS * RHS; e += 256) {
@@ -783,13 +817,14 @@ __global__ __launch_bounds__(256) void k_pcg_p(
 }
 
 // alpha = rz / pq per column; X += alpha P; R -= alpha Q; R32 = R;
-// rr[wg][c] = sum_rows R^2.
+// rr[wg][c] = sum_rows R^2, rzx[wg][c] = sum_rows R . Z (Z: the current
+// preconditioned residual, for the Polak-Ribiere beta).
 __global__ __launch_bounds__(256) void k_pcg_xr(
     int n, const double* __restrict__ pq, int npq,
     const double* __restrict__ rz, int nrz, const double* __restrict__ P,
-    const double* __restrict__ Q, double* __restrict__ X,
-    double* __restrict__ R, float* __restrict__ R32,
-    double* __restrict__ rr) {
+    const double* __restrict__ Q, const float* __restrict__ Z,
+    double* __restrict__ X, double* __restrict__ R, float* __restrict__ R32,
+    double* __restrict__ rr, double* __restrict__ rzx) {
   __shared__ double alpha[RHS], num[RHS], den[RHS];
   __shared__ double red[256];
   const int t = threadIdx.x;
@@ -799,7 +834,7 @@ __global__ __launch_bounds__(256) void k_pcg_xr(
   __syncthreads();
   const int c = t % RHS;
   const long long e0 = (long long)blockIdx.x * VROWS * RHS;
-  double s = 0.0;
+  double s = 0.0, x = 0.0;
   for (int e = t; e < VROWS * RHS; e += 256) {
     const long long g = e0 + e;
     if (g >= (long long)n * RHS) break;
@@ -808,13 +843,23 @@ __global__ __launch_bounds__(256) void k_pcg_xr(
     R[g] = rv;
     R32[g] = (float)rv;
     s += rv * rv;
+    x += rv * (double)Z[g];
   }
+  __syncthreads();
   red[t] = s;
   __syncthreads();
   if (t < RHS) {
     double v = 0.0;
     for (int k = t; k < 256; k += RHS) v += red[k];
     rr[(long long)blockIdx.x * RHS + t] = v;
+  }
+  __syncthreads();
+  red[t] = x;
+  __syncthreads();
+  if (t < RHS) {
+    double v = 0.0;
+    for (int k = t; k < 256; k += RHS) v += red[k];
+    rzx[(long long)blockIdx.x * RHS + t] = v;
   }
 }
 
@@ -873,7 +918,7 @@ __global__ __launch_bounds__(256) void k_pad32(int n, const float* __restrict__ 
 
 // ------------------------------------------------------------ host side
 struct Ws {
-  double *b, *x, *r, *p0, *p1, *q, *pq, *rz0, *rz1, *rr, *bb;
+  double *b, *x, *r, *p0, *p1, *q, *pq, *rz0, *rz1, *rr, *bb, *rzx;
   float *r32, *z32, *y1, *u2, *y2, *v1;
   size_t bytes;
 };
@@ -900,6 +945,7 @@ static Ws carve(char* base, int n) {
   w.rz1 = (double*)take(slots);
   w.rr = (double*)take(slots);
   w.bb = (double*)take(slots);
+  w.rzx = (double*)take(slots);
   w.r32 = (float*)take(v32);
   w.z32 = (float*)take(v32);
   w.y1 = (float*)take(v32);
@@ -1002,11 +1048,13 @@ static int pcg_run(const float* K, int n, long long ldk, double lam,
     double* p_new = pbuf[it & 1];
     const double* rz_new = rzs[(it + 1) & 1];
     hipLaunchKernelGGL(k_pcg_p, dim3(nvec), dim3(256), 0, st, n, w.z32,
-                       p_old, p_new, rz_new, rzs[it & 1], nrz, it == 1);
+                       p_old, p_new, rz_new, rzs[it & 1], nrz, w.rzx, nvec,
+                       it == 1);
     hipLaunchKernelGGL(k_symv_rows, dim3(nsym), dim3(512), 0, st, K, ldk, n,
                        lam, 0, p_new, w.q, nullptr, nullptr, nullptr, w.pq);
     hipLaunchKernelGGL(k_pcg_xr, dim3(nvec), dim3(256), 0, st, n, w.pq, nsym,
-                       rz_new, nrz, p_new, w.q, w.x, w.r, w.r32, w.rr);
+                       rz_new, nrz, p_new, w.q, w.z32, w.x, w.r, w.r32, w.rr,
+                       w.rzx);
     rc = col_sums(w.rr, nvec, rn);
     if (rc) return rc;
     double rel = rel_of(rn);
