@@ -111,6 +111,36 @@ int nta_stage(const float* in_nngp, const float* in_ntk, int n1, int n2,
               float* out_ntk, void* stream);
 
 /*
+ * The first two stages of a Myrtle-type network as one launch on a tile of
+ * pairs: Conv(3x3, SAME) -> Relu -> Conv(3x3, SAME) -> Relu ->
+ * AvgPool(2x2, stride 2) on 32x32 images with 1..4 channels; the first
+ * stage's [32][32][32][32] state is never written.  Same result as
+ * nta_first_stage followed by nta_stage (within FP32 rounding: the patch
+ * Gram is accumulated in another order).
+ * Replaces: the same reference functions as those two entry points, for
+ * stax.serial(Conv, Relu, Conv, Relu, AvgPool) (stax/combinators.py:40-68).
+ * x1: [n1][32][32][C], x2: [n2][32][32][C] (ignored when n2 == 0).
+ * qn1a, qn2a: normalisers of the first ReLU as (sqrt(q), 1/sqrt(q)) pairs,
+ * [n][32][32][2] (nta_qnorm of the nta_first_diag output); qn1b, qn2b: the
+ * same for the second ReLU (nta_qnorm of the nta_stage_diag output).
+ * flags: NTA_NTK or 0.  Outputs: [T][16][16][16][16].
+ * Returns NTA_EUNSUPPORTED for other image shapes.
+ */
+int nta_first_pool_stage(const float* x1, const float* x2, int n1, int n2,
+                         int H, int W, int C, int flags, float w_var1,
+                         float b_var1, const float* qn1a, const float* qn2a,
+                         float w_var2, float b_var2, const float* qn1b,
+                         const float* qn2b, float* out_nngp, float* out_ntk,
+                         void* stream);
+
+/*
+ * out[2k] = sqrt(q[k]), out[2k + 1] = 1 / sqrt(q[k]) (both 0 where
+ * q[k] <= 0), for k < n: the per-position factors of the ReLU map
+ * (elementwise.py:444-455 divides the covariance by sqrt(q1 q2)).
+ */
+int nta_qnorm(const float* q, long long n, float* out, void* stream);
+
+/*
  * Readout head: GlobalAvgPool (or Flatten with NTA_FLATTEN) followed by the
  * composed affine map of the trailing Dense layers:
  *   nngp_out = a * m + b,   ntk_out = c * m + d * mt + e

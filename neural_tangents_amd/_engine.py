@@ -276,19 +276,37 @@ def _state_elems(h, w):
 
 
 def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
-               collect_q=None, head_out=None, ld=0, keep=None):
+               collect_q=None, head_out=None, ld=0, keep=None, qn=None):
   """Runs all stages for one tile.  Self tile when ``n2 == 0`` (then
   ``collect_q`` receives the normaliser of each ReLU stage, computed from the
   self state on the way).  Returns the final (nngp, ntk) state when the plan
   has no head, else writes the head output into ``head_out``.  ``keep``
   (a list) receives the output state of every stage (the checkpoints the
-  adjoint recomputes from)."""
+  adjoint recomputes from).  ``qn`` = (first-ReLU normaliser tables of the
+  tile's x1 and x2 images, then the second ReLU's), from ``nta_qnorm``,
+  selects the fused launch of the first two stages (cross tiles only)."""
   dev = xa.device
   T = n1 if n2 == 0 else n1 * n2
   _, H, W, C = xa.shape
   nngp = ntk = None
   fl_ntk = want_ntk and n2 != 0
+  skip = 0
+  if qn is not None and n2 != 0:
+    s0, s1 = plan.stages[0], plan.stages[1]
+    out = torch.empty((T, s1.hout, s1.hout, s1.wout, s1.wout),
+                      dtype=torch.float32, device=dev)
+    out_t = torch.empty_like(out) if fl_ntk else None
+    _launch('nta_first_pool_stage', T * (2 if fl_ntk else 1) * 4 *
+            out[0].numel(), T, _lib.ptr(xa), _lib.ptr(xb), n1, n2, H, W, C,
+            _lib.NTK if fl_ntk else 0, s0.conv[3], s0.conv[4],
+            _lib.ptr(qn[0]), _lib.ptr(qn[1]), s1.conv[3], s1.conv[4],
+            _lib.ptr(qn[2]), _lib.ptr(qn[3]), _lib.ptr(out),
+            _lib.ptr(out_t), stream)
+    nngp, ntk = out, out_t
+    skip = 2
   for si, s in enumerate(plan.stages):
+    if si < skip:
+      continue
     if s.first:
       fh, fw, valid, w_var, b_var = s.conv
       flags = s.flags(fl_ntk)
@@ -356,9 +374,30 @@ def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
   return None
 
 
-def _per_pair_bytes(plan: Plan, ntk: bool) -> int:
+def _fuses_first_two(plan: Plan, in_shape) -> bool:
+  """Conv(3x3 SAME) -> Relu -> Conv(3x3 SAME) -> Relu -> AvgPool(2x2) at the
+  start of the network on 32x32 images with <= 4 channels: the first two
+  stages run as one launch (nta_first_pool_stage)."""
+  if len(plan.stages) < 2:
+    return False
+  _, H, W, C = in_shape
+  s0, s1 = plan.stages[0], plan.stages[1]
+  def conv33(s):
+    return s.conv is not None and s.conv[:3] == (3, 3, False)
+  return (H == 32 and W == 32 and 1 <= C <= 4 and s0.first and conv33(s0)
+          and s0.relu and s0.act == (1.0, 0.0) and s0.pool is None
+          and conv33(s1) and s1.relu and s1.act == (1.0, 0.0)
+          and s1.pool == (2, 2))
+
+
+def _per_pair_bytes(plan: Plan, ntk: bool, fused: bool = False) -> int:
   m = 0
-  for s in plan.stages:
+  for si, s in enumerate(plan.stages):
+    if fused and si < 2:
+      # the first stage's state and the second conv's output are never
+      # materialised
+      m = max(m, _state_elems(s.hout, s.wout)) if si == 1 else m
+      continue
     m = max(m, _state_elems(s.hc, s.wc), _state_elems(s.hout, s.wout))
   # ping-pong input/output state, nngp (+ ntk)
   return 4 * m * 2 * (2 if ntk else 1)
@@ -475,7 +514,13 @@ def run(plan: Plan, x1, x2, want_nngp, want_ntk, want_cov,
     xb_all = x1
   N1, N2 = x1.shape[0], xb_all.shape[0]
   dev = x1.device
-  per = _per_pair_bytes(plan, want_ntk)
+  fused = _fuses_first_two(plan, x1.shape)
+  qn1 = qn2 = None
+  if fused:
+    qn1 = [_qnorm(q1s[0], stream), _qnorm(q1s[1], stream)]
+    qn2 = qn1 if x2 is None else [_qnorm(q2s[0], stream),
+                                  _qnorm(q2s[1], stream)]
+  per = _per_pair_bytes(plan, want_ntk, fused)
   T = max(1, tile_bytes // per)
   if x2 is None and symmetric and plan.head is not None and T < N1 * N2:
     ti = tj = max(1, min(N1, math.isqrt(T)))
@@ -504,14 +549,18 @@ def run(plan: Plan, x1, x2, want_nngp, want_ntk, want_cov,
       qb = [None if q is None else q[j0:j0 + n2] for q in q2s]
       xa = x1[i0:i0 + n1]
       xb = xb_all[j0:j0 + n2]
+      qn = None
+      if fused:
+        qn = (qn1[0][i0:i0 + n1], qn2[0][j0:j0 + n2],
+              qn1[1][i0:i0 + n1], qn2[1][j0:j0 + n2])
       if plan.head is not None:
         on = nngp[i0:, j0:]
         ot = ntk[i0:, j0:] if want_ntk else None
         _run_chain(plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
-                   head_out=(on, ot), ld=N2)
+                   head_out=(on, ot), ld=N2, qn=qn)
       else:
         st_n, st_t = _run_chain(plan, xa, xb, n1, n2, qa, qb, want_ntk,
-                                stream)
+                                stream, qn=qn)
         sh = st_n.shape[1:]
         nngp[i0:i0 + n1, j0:j0 + n2] = st_n.view((n1, n2) + sh)
         if want_ntk:
@@ -526,6 +575,15 @@ def run(plan: Plan, x1, x2, want_nngp, want_ntk, want_cov,
     for k in ('nngp', 'ntk', 'cov1', 'cov2'):
       if out[k] is not None:
         out[k].mul_(plan.out_scale)
+  return out
+
+
+def _qnorm(q, stream):
+  """(sqrt(q), 1/sqrt(q)) tables [N, H, W, 2] of a normaliser [N, H, W]."""
+  q = q.contiguous()
+  out = torch.empty(q.shape + (2,), dtype=torch.float32, device=q.device)
+  _launch('nta_qnorm', 12 * q.numel(), 0, _lib.ptr(q), q.numel(),
+          _lib.ptr(out), stream)
   return out
 
 

@@ -45,6 +45,9 @@ SIGNATURES = {
                   _LL, _P, _P, _P],
     'nta_diag_chain': [_P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P,
                        _I, _D, _D, _D, _D, _D, _P, _P, _LL, _P],
+    'nta_first_pool_stage': [_P, _P, _I, _I, _I, _I, _I, _I, _F, _F, _P, _P,
+                             _F, _F, _P, _P, _P, _P, _P],
+    'nta_qnorm': [_P, _LL, _P, _P],
     'nta_relu_bwd': [_P, _P, _I, _I, _I, _I, _I, _F, _F, _F, _F, _P, _P, _P,
                      _P, _P, _P, _P, _P, _P, _P, _P],
 }

@@ -75,25 +75,28 @@ __device__ __forceinline__ void relu_acc(float cv, float sqp, float rqp,
   const float rho = cv * rqp;
   const float a = fminf(fabsf(rho), 1.0f);
   const float u = 1.0f - a;
-  const float th = acos_pos(a, u);
+  const float th = acos_pos5(a, u);
   const float pm = rho < 0.0f ? th : kPi - th;
-  const float sn = fsqrt(u * (2.0f - u));
+  const float sn = fsqrt(fmaf(-a, a, 1.0f));
   acc_n = fmaf(fmaf(pm, rho, sn), sqp, acc_n);
   if constexpr (NTK) acc_t = fmaf(pm, t, acc_t);
 }
 
 // Issues the buffer loads of K planes of one tensor: plane k is
 // (hh[k], hp[k]); planes outside [0, S)^2 (and every plane of a tail pair,
-// outside the resource) read as zero.  Each thread loads its 2x2 (w, w')
+// outside the resource) read as zero, and so does every plane when live is
+// false (the loads are issued but touch no memory).  Each thread loads its
+// 2x2 (w, w')
 // block (2W.., 2Wp..): r0 = row 2W, r1 = row 2W + 1.
 template <int S, int K>
 __device__ __forceinline__ void issue_planes(__amdgpu_buffer_rsrc_t rs,
                                              const int* hh, const int* hp,
                                              unsigned tofs, float2 (&r0)[K],
-                                             float2 (&r1)[K]) {
+                                             float2 (&r1)[K],
+                                             bool live = true) {
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const bool ok = (unsigned)hh[k] < (unsigned)S &&
+    const bool ok = live && (unsigned)hh[k] < (unsigned)S &&
                     (unsigned)hp[k] < (unsigned)S;
     const unsigned soff = ok ? (unsigned)((hh[k] * S + hp[k]) * S * S) * 4u
                              : kOutside;
@@ -153,10 +156,9 @@ __device__ __forceinline__ void wpass(__amdgpu_buffer_rsrc_t rs,
 // (w, w')-diagonal 3-tap sums of one LDS plane written by the thread block
 // (zero border): ws[e], e = 2c + d, as in exchange(), with the thread's own
 // elements read back from LDS (paired 8-byte reads).
-template <int S>
+template <int S, int LS = S + 2>
 __device__ __forceinline__ void read_diag3(const float* __restrict__ plane,
                                            int lofs, float (&ws)[4]) {
-  constexpr int LS = S + 2;
   const float* s = plane + lofs;
   const float2 ra = *reinterpret_cast<const float2*>(s - LS - 1);
   const float2 rb = *reinterpret_cast<const float2*>(s - 1);
@@ -174,9 +176,10 @@ __device__ __forceinline__ void read_diag3(const float* __restrict__ plane,
 //   0: issued at the step that uses them;
 //   1: prefetched into registers one step ahead, after the step's exchange;
 //   2: as 1, but issued right after the step's planes are written to LDS
-//      (both tensors share one exchange: two barriers per step).
+//      (both tensors share one exchange: two barriers per step);
+//   3: as 2 with two steps in flight (two register sets, even/odd steps).
 template <int S, int POOL, bool NTK, int PF>
-__global__ __launch_bounds__(256) void k_stage_fast(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_stage_fast(
     const float* __restrict__ in_n, const float* __restrict__ in_t, int n1,
     int n2, int ndiag, int ngroups, int nblk, float wv9, float b_var,
     const float* __restrict__ q1, const float* __restrict__ q2,
@@ -187,7 +190,7 @@ __global__ __launch_bounds__(256) void k_stage_fast(
   constexpr int LS = S + 2;
   constexpr int PL = LS * LS;
   constexpr int NT = NTK ? 2 : 1;
-  constexpr int NB = POOL == 2 ? (PF == 2 ? 6 * NT : 6) : 1;  // LDS planes
+  constexpr int NB = POOL == 2 ? (PF >= 2 ? 6 * NT : 6) : 1;  // LDS planes
   constexpr int SO = S / POOL;
   constexpr long long PER_IN = (long long)S * S * S * S;
   constexpr long long PER_OUT = (long long)SO * SO * SO * SO;
@@ -270,35 +273,33 @@ __global__ __launch_bounds__(256) void k_stage_fast(
       hh[4] = 2 * H;     hp[4] = 2 * H - dm;
       hh[5] = 2 * H + 1; hp[5] = 2 * H + 1 - dm;
     };
-    float2 nx0[NT][6], nx1[NT][6];     // prefetched planes of the next step
-    if constexpr (PF != 0) {
-      int hh[6], hp[6];
-      step_planes(Hs, hh, hp);
-#pragma unroll
-      for (int tt = 0; tt < NT; ++tt)
-        issue_planes<S, 6>(tt ? rt : rn, hh, hp, tofs, nx0[tt], nx1[tt]);
-    }
-    for (int H = Hs; H <= He; ++H) {
+    // One step: emits pooled plane (H, H - D).  With PF >= 2 the planes of
+    // the step arrive in (r0, r1) and the loads of step Hn (if any) are
+    // issued into the same registers once they are in LDS.
+    auto step = [&](int H, float2 (&r0)[NT][6], float2 (&r1)[NT][6],
+                    int Hn) {
       const int Hp = H - D;
-      if constexpr (PF == 2) {
+      if constexpr (PF >= 2) {
         __syncthreads();             // the previous step's reads are done
 #pragma unroll
         for (int tt = 0; tt < NT; ++tt)
 #pragma unroll
           for (int k = 0; k < 6; ++k) {
             float* d = L0 + (tt * 6 + k) * PL + lofs;
-            d[0] = nx0[tt][k].x;
-            d[1] = nx0[tt][k].y;
-            d[LS] = nx1[tt][k].x;
-            d[LS + 1] = nx1[tt][k].y;
+            d[0] = r0[tt][k].x;
+            d[1] = r0[tt][k].y;
+            d[LS] = r1[tt][k].x;
+            d[LS + 1] = r1[tt][k].y;
           }
-        if (H < He) {
+        if (PF == 3 || Hn <= He) {
+          // PF 3 issues past the sweep end too (as empty loads) so that the
+          // count of loads in flight is the same on every path.
           int hh[6], hp[6];
-          step_planes(H + 1, hh, hp);
+          step_planes(Hn, hh, hp);
 #pragma unroll
           for (int tt = 0; tt < NT; ++tt)
-            issue_planes<S, 6>(tt ? rt : rn, hh, hp, tofs, nx0[tt],
-                               nx1[tt]);
+            issue_planes<S, 6>(tt ? rt : rn, hh, hp, tofs, r0[tt], r1[tt],
+                               Hn <= He);
         }
         __syncthreads();
       }
@@ -306,12 +307,12 @@ __global__ __launch_bounds__(256) void k_stage_fast(
 #pragma unroll
       for (int tt = 0; tt < NT; ++tt) {
         float ws[6][4];
-        if constexpr (PF == 2) {
+        if constexpr (PF >= 2) {
 #pragma unroll
           for (int k = 0; k < 6; ++k)
             read_diag3<S>(L0 + (tt * 6 + k) * PL, lofs, ws[k]);
         } else if constexpr (PF == 1) {
-          exchange<S, 6>(nx0[tt], nx1[tt], L0, lofs, ws);
+          exchange<S, 6>(r0[tt], r1[tt], L0, lofs, ws);
         } else {
           int hh[6], hp[6];
           step_planes(H, hh, hp);
@@ -331,13 +332,12 @@ __global__ __launch_bounds__(256) void k_stage_fast(
         }
       }
       if constexpr (PF == 1) {
-        if (H < He) {
+        if (Hn <= He) {
           int hh[6], hp[6];
-          step_planes(H + 1, hh, hp);
+          step_planes(Hn, hh, hp);
 #pragma unroll
           for (int tt = 0; tt < NT; ++tt)
-            issue_planes<S, 6>(tt ? rt : rn, hh, hp, tofs, nx0[tt],
-                               nx1[tt]);
+            issue_planes<S, 6>(tt ? rt : rn, hh, hp, tofs, r0[tt], r1[tt]);
         }
       }
       // (sq, rq) at rows 2H, 2H+1 of image i and 2Hp, 2Hp+1 of image j,
@@ -369,6 +369,35 @@ __global__ __launch_bounds__(256) void k_stage_fast(
         st_state(out_n + o, acc_n * kScale);
         if constexpr (NTK) st_state(out_t + o, acc_t * kScale);
       }
+    };
+    float2 na0[NT][6], na1[NT][6];     // planes in flight (register set A)
+    if constexpr (PF != 0) {
+      int hh[6], hp[6];
+      step_planes(Hs, hh, hp);
+#pragma unroll
+      for (int tt = 0; tt < NT; ++tt)
+        issue_planes<S, 6>(tt ? rt : rn, hh, hp, tofs, na0[tt], na1[tt]);
+    }
+    if constexpr (PF == 3) {
+      // Two steps in flight: set A carries even steps, set B odd ones.
+      float2 nb0[NT][6], nb1[NT][6];
+      {
+        int hh[6], hp[6];
+        step_planes(Hs + 1, hh, hp);
+#pragma unroll
+        for (int tt = 0; tt < NT; ++tt)
+          issue_planes<S, 6>(tt ? rt : rn, hh, hp, tofs, nb0[tt], nb1[tt],
+                             Hs + 1 <= He);
+      }
+      int H = Hs;
+      for (; H + 1 <= He; H += 2) {
+        step(H, na0, na1, H + 2);
+        step(H + 1, nb0, nb1, H + 3);
+      }
+      if (H <= He) step(H, na0, na1, H + 2);
+      __builtin_amdgcn_s_waitcnt(0x0f70);    // vmcnt(0): drain the empty loads
+    } else {
+      for (int H = Hs; H <= He; ++H) step(H, na0, na1, H + 1);
     }
   } else {
     const int d = di - (S - 1);
@@ -469,9 +498,9 @@ __device__ __forceinline__ void first32_epilogue(
     const float rho = cv * (rq1r[r] * qb.y);
     const float a = fminf(fabsf(rho), 1.0f);
     const float u = 1.0f - a;
-    const float th = acos_pos(a, u);
+    const float th = acos_pos5(a, u);
     const float pm = rho < 0.0f ? th : kPi - th;
-    const float sn = fsqrt(u * (2.0f - u));
+    const float sn = fsqrt(fmaf(-a, a, 1.0f));
     // sq1r already carries the 1 / (2 pi) factor.
     st_state(on + m * S + col, fmaf(pm, rho, sn) * (sq1r[r] * qb.x));
     if constexpr (NTK) st_state(ot + m * S + col, (cv * kInv2Pi) * pm);
@@ -639,6 +668,368 @@ __global__ __launch_bounds__(256) void k_first16(
   }
 }
 
+// ------------------------------------------- first two stages, fused
+
+// Conv(3x3 SAME) -> ReLU -> Conv(3x3 SAME) -> ReLU -> AvgPool(2x2) on 32x32
+// images, with the first stage's state never written to memory.  The
+// decomposition is k_stage_fast's (one workgroup per pooled diagonal
+// H - H' = D of one pair, sweeping H); the six first-stage planes a step
+// consumes are computed in place from the two images held in LDS:
+//   plane (h, h') = ReLU(scale1 * P_i(h) P_j(h')^T + b1),
+// P_n(h) the [32 w][9 taps x 4 channels] patch matrix of image n at row h
+// (the Gram of zero-padded patches equals the input-covariance stencil).
+// Each wave owns one 16x16 (w, w') quadrant of every plane: 9
+// v_mfma_f32_16x16x4_f32 (one per tap, the 4 channels across lane groups)
+// and the ReLU epilogue.  The planes then go through the LDS exchange of
+// the second stage (nngp first, ntk second).  Planes on odd first-stage
+// diagonals are computed by both workgroups that need them (1.5x the
+// first-stage work), which costs far less than the 16 MiB per pair of
+// state the unfused pair of stages writes and reads back.
+constexpr int kXS = 34;               // padded image row (floats)
+// Padded channel plane: 34 * 34 rounded up to 16 mod 32 floats, so the
+// lane groups of the four channels read disjoint LDS banks.
+constexpr int kXP = 34 * 34 + 12;
+constexpr int kLS2 = 36;              // exchange plane row (4 rows = 16 banks)
+#ifndef NTA_F12_SCHED
+#define NTA_F12_SCHED 0
+#endif
+constexpr int kPL2 = 34 * kLS2;
+
+// Writes this lane's 4 x K first-stage values (rows w0 + e, column wq of
+// each plane, as offsets wr + e * kLS2) and reads back the (w, w')-diagonal
+// 3-tap sums of the thread's 2x2 block.  Every thread calls it the same
+// number of times.
+template <int K>
+__device__ __forceinline__ void s1_exchange(float* __restrict__ lp,
+                                            const float (&v)[K][4], int wr,
+                                            int lofs, float (&ws)[K][4]) {
+  __syncthreads();                     // the previous reads are done
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) lp[k * kPL2 + wr + e * kLS2] = v[k][e];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) read_diag3<32, kLS2>(lp + k * kPL2, lofs, ws[k]);
+}
+
+template <bool NTK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
+                    int n1, int n2, int C, int nblk, float scale1, float b1,
+                    const float2* __restrict__ qa1,
+                    const float2* __restrict__ qa2, float wv9, float b2,
+                    const float2* __restrict__ qb1,
+                    const float2* __restrict__ qb2, float* __restrict__ out_n,
+                    float* __restrict__ out_t) {
+  constexpr int S = 32, H2 = 16, SO = 16, NDIAG = 2 * H2 - 1;
+  constexpr long long PER_OUT = (long long)SO * SO * SO * SO;
+  constexpr int NT = NTK ? 2 : 1;
+  __shared__ __align__(16) float xs[2 * 4 * kXP];     // images i, j
+  __shared__ __align__(16) float lp[6 * kPL2];        // exchange planes
+  const bool self = n2 == 0;
+  const long long T = self ? (long long)n1 : (long long)n1 * n2;
+  const int L = xcd_remap(blockIdx.x, nblk);
+  if ((long long)L >= T * NDIAG) return;   // whole block exits together
+  const long long p = L / NDIAG;
+  const int di = L - (int)(p * NDIAG);
+  const long long i = self ? p : p / n2;
+  const long long j = self ? p : p - i * n2;
+  const int t = threadIdx.x;
+  // Second-stage role: the thread's 2x2 (w, w') block at (2W, 2Wp).
+  const int W = t / H2;
+  const int Wp = t - W * H2;
+  const int lofs = (2 * W + 1) * kLS2 + 2 * Wp + 1;
+  // First-stage role: quadrant (bw, bwp) of the wave; MFMA output rows
+  // w0 .. w0 + 3 (lane group c4), column wq.
+  const int lane = t & 63, wave = t >> 6;
+  const int m16 = lane & 15, c4 = lane >> 4;
+  const int w0 = 16 * (wave >> 1) + 4 * c4;
+  const int wq = 16 * (wave & 1) + m16;
+  const int wr = (w0 + 1) * kLS2 + wq + 1;
+
+  for (int k = t; k < 6 * kPL2; k += 256) lp[k] = 0.0f;
+  {
+    // Zero-padded, channel-planar images: xs[img][c][1 + h][1 + w].
+    const float* xi = x1 + i * (S * S * C);
+    const float* xj = (self ? x1 : x2) + j * (S * S * C);
+    for (int k = t; k < 2 * 4 * kXS * kXS; k += 256) {
+      const int img = k / (4 * kXS * kXS);
+      const int r = k - img * (4 * kXS * kXS);
+      const int c = r / (kXS * kXS);
+      const int rr = r - c * (kXS * kXS);
+      const int hh = rr / kXS - 1;
+      const int ww = rr - (rr / kXS) * kXS - 1;
+      float v = 0.0f;
+      if (c < C && (unsigned)hh < (unsigned)S && (unsigned)ww < (unsigned)S)
+        v = (img ? xj : xi)[(hh * S + ww) * C + c];
+      xs[img * 4 * kXP + c * kXP + rr] = v;
+    }
+  }
+  __syncthreads();
+  const float* xa = xs + c4 * kXP + 16 * (wave >> 1) + m16;
+  const float* xb = xs + 4 * kXP + c4 * kXP + 16 * (wave & 1) + m16;
+  const float2* qi = qa1 + i * (S * S);
+  const float2* qj = (self ? qa1 : qa2) + j * (S * S);
+  const float2* ri = qb1 + i * (S * S);
+  const float2* rj = (self ? qb1 : qb2) + j * (S * S);
+
+  // This wave's quadrant of first-stage plane (hh, hp): nngp in vn, ntk in
+  // vt (zero outside the grid: the second conv's zero padding), both
+  // 2 pi times the first stage's state (the factor is folded into the
+  // second conv's weight, wv2 = wv9 / (2 pi)).
+  auto s1 = [&](int hh, int hp, float (&vn)[4], float (&vt)[4]) {
+    const bool ok = (unsigned)hh < (unsigned)S && (unsigned)hp < (unsigned)S;
+    if (!ok) hh = hp = 0;              // uniform; the values are dropped
+    floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (ok) {
+      const float* pa = xa + hh * kXS;
+      const float* pb = xb + hp * kXS;
+#pragma unroll
+      for (int kk = 0; kk < 9; ++kk) {
+        const int o = (kk / 3) * kXS + kk % 3;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[o], pb[o], acc, 0, 0,
+                                                   0);
+      }
+    }
+    const float4 q01 = *reinterpret_cast<const float4*>(qi + hh * S + w0);
+    const float4 q23 = *reinterpret_cast<const float4*>(qi + hh * S + w0 + 2);
+    const float2 qb = qj[hp * S + wq];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float4 Q = e < 2 ? q01 : q23;
+      const float sa = (e & 1) ? Q.z : Q.x, ra = (e & 1) ? Q.w : Q.y;
+      const float cv = fmaf(scale1, acc[e], b1);
+      const float rho = cv * (ra * qb.y);
+      const float a = fminf(fabsf(rho), 1.0f);
+      const float u = 1.0f - a;
+      const float th = acos_pos5(a, u);
+      const float pm = rho < 0.0f ? th : kPi - th;
+      const float sn = fsqrt(fmaf(-a, a, 1.0f));
+      vn[e] = ok ? fmaf(pm, rho, sn) * (sa * qb.x) : 0.0f;
+      vt[e] = ok ? cv * pm : 0.0f;
+    }
+  };
+
+  const float wv2 = wv9 * kInv2Pi;
+  const int D = di - (H2 - 1);
+  const int Hs = D > 0 ? D : 0;
+  const int He = D < 0 ? H2 - 1 + D : H2 - 1;
+  const int d0 = 2 * D, dp = 2 * D + 1, dm = 2 * D - 1;
+
+  // Loop-carried state of the sweep.  Steps alternate between two copies
+  // (the loop is unrolled by two) so the carried values stay in place.
+  struct Carry {
+    float c0a[NT][4], c0b[NT][4], cp[NT][4], cm[NT][4];  // (h, h') window
+    float4 qal[3], qah[3];   // first-stage (sq, rq), image i rows 2H + r
+    float2 qbv[3];           // first-stage (sq, rq), image j rows 2Hp + r
+    float4 r0, r1, s0, s1;   // second-stage (sq, rq) rows 2H(+1), 2Hp(+1)
+    floatx4 acc[6];          // patch Grams of the step's six planes
+  };
+  Carry A, B;
+  {
+    const int h0 = 2 * Hs;
+    const int hh[4] = {h0 - 1, h0, h0, h0 - 1};
+    const int hp[4] = {h0 - 1 - d0, h0 - d0, h0 - dp, h0 - 1 - dm};
+    float vn[4][4], vt[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s1(hh[k], hp[k], vn[k], vt[k]);
+    auto init = [&](const float (&v)[4][4], int tt) {
+      float ws[4][4];
+      s1_exchange<4>(lp, v, wr, lofs, ws);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        A.c0a[tt][e] = ws[0][e];
+        A.c0b[tt][e] = ws[1][e];
+        A.cp[tt][e] = ws[2][e];
+        A.cm[tt][e] = ws[3][e];
+      }
+    };
+    init(vn, 0);
+    if constexpr (NTK) init(vt, NT - 1);
+  }
+
+  // Plane k of step H is (2H + ar[k], 2Hp + br[k]) (d0 @ 2H+1, 2H+2; dp @
+  // 2H+1, 2H+2; dm @ 2H, 2H+1).  The normalisers of the rows a step adds
+  // are loaded one step ahead (row indices clamped into the table: values
+  // of rows past the grid are never used).
+  constexpr int ar[6] = {1, 2, 1, 2, 0, 1};
+  constexpr int br[6] = {1, 2, 0, 1, 1, 2};
+  auto ld_a = [&](int row, float4& lo, float4& hi) {
+    row = row < S ? row : S - 1;
+    lo = *reinterpret_cast<const float4*>(qi + row * S + w0);
+    hi = *reinterpret_cast<const float4*>(qi + row * S + w0 + 2);
+  };
+  auto ld_b = [&](int row) { return qj[(row < S ? row : S - 1) * S + wq]; };
+  auto ld_r = [&](const float2* tab, int row, int col) {
+    return *reinterpret_cast<const float4*>(
+        tab + (row < S ? row : S - 1) * S + col);
+  };
+  // First-stage patch Grams of the six planes of step Hn: operands of
+  // padded image rows 2Hn + u (u = 0..4, clamped into the padded image:
+  // such planes are outside the grid and dropped), three taps per row.
+  auto gram6 = [&](int Hn, floatx4 (&acc)[6]) {
+    const int Hpn = Hn - D;
+    float av[5][3], bv[5][3];
+#pragma unroll
+    for (int u = 0; u < 5; ++u) {
+      const int ra = 2 * Hn + u < kXS - 1 ? 2 * Hn + u : kXS - 1;
+      const int rb = 2 * Hpn + u < kXS - 1 ? 2 * Hpn + u : kXS - 1;
+#pragma unroll
+      for (int tx = 0; tx < 3; ++tx) {
+        av[u][tx] = xa[ra * kXS + tx];
+        bv[u][tx] = xb[rb * kXS + tx];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acc[k] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kk = 0; kk < 9; ++kk) {
+      const int ty = kk / 3, tx = kk % 3;
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+            av[ar[k] + ty][tx], bv[br[k] + ty][tx], acc[k], 0, 0, 0);
+    }
+  };
+  {
+    const int Hp = Hs - D;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      ld_a(2 * Hs + r, A.qal[r], A.qah[r]);
+      A.qbv[r] = ld_b(2 * Hp + r);
+    }
+    A.r0 = ld_r(ri, 2 * Hs, 2 * W);
+    A.r1 = ld_r(ri, 2 * Hs + 1, 2 * W);
+    A.s0 = ld_r(rj, 2 * Hp, 2 * Wp);
+    A.s1 = ld_r(rj, 2 * Hp + 1, 2 * Wp);
+    gram6(Hs, A.acc);
+  }
+
+  // First-stage ReLU of the step's planes (MASK: some plane is outside the
+  // grid and reads as zero).
+  auto relu1 = [&](const Carry& in, int H, auto mask, float (&vn)[6][4],
+                   float (&vt)[6][4]) {
+    constexpr bool MASK = decltype(mask)::value;
+    const int Hp = H - D;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const bool ok = 2 * H + ar[k] < S && 2 * Hp + br[k] < S;
+      const float2 qb = in.qbv[br[k]];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float4 Q = e < 2 ? in.qal[ar[k]] : in.qah[ar[k]];
+        const float sa = (e & 1) ? Q.z : Q.x, ra = (e & 1) ? Q.w : Q.y;
+        const float cv = fmaf(scale1, in.acc[k][e], b1);
+        const float rho = cv * (ra * qb.y);
+        const float a = fminf(fabsf(rho), 1.0f);
+        const float th = acos_pos5(a, 1.0f - a);
+        const float pm = rho < 0.0f ? th : kPi - th;
+        const float sn = fsqrt(fmaf(-a, a, 1.0f));
+        const float n = fmaf(pm, rho, sn) * (sa * qb.x);
+        const float t = cv * pm;
+        vn[k][e] = MASK ? (ok ? n : 0.0f) : n;
+        vt[k][e] = MASK ? (ok ? t : 0.0f) : t;
+      }
+    }
+  };
+
+  auto step = [&](int H, const Carry& in, Carry& out) {
+    const int Hp = H - D;
+    // Next step's normaliser rows, in flight during this step.
+    out.qal[0] = in.qal[2];
+    out.qah[0] = in.qah[2];
+    out.qbv[0] = in.qbv[2];
+#pragma unroll
+    for (int r = 1; r < 3; ++r) {
+      ld_a(2 * H + 2 + r, out.qal[r], out.qah[r]);
+      out.qbv[r] = ld_b(2 * Hp + 2 + r);
+    }
+    out.r0 = ld_r(ri, 2 * H + 2, 2 * W);
+    out.r1 = ld_r(ri, 2 * H + 3, 2 * W);
+    out.s0 = ld_r(rj, 2 * Hp + 2, 2 * Wp);
+    out.s1 = ld_r(rj, 2 * Hp + 3, 2 * Wp);
+
+    float vn[6][4], vt[6][4];
+    if (2 * H + 2 < S && 2 * Hp + 2 < S)
+      relu1(in, H, std::false_type(), vn, vt);
+    else
+      relu1(in, H, std::true_type(), vn, vt);
+
+    float pre[NT][4][4];   // [tensor][P00, P11, P10, P01][element]
+    auto advance = [&](const float (&v)[6][4], int tt) {
+      float ws[6][4];
+      s1_exchange<6>(lp, v, wr, lofs, ws);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float z0 = in.c0b[tt][e] + ws[0][e];
+        pre[tt][0][e] = in.c0a[tt][e] + z0;
+        pre[tt][1][e] = z0 + ws[1][e];
+        pre[tt][2][e] = in.cp[tt][e] + ws[2][e] + ws[3][e];
+        pre[tt][3][e] = in.cm[tt][e] + ws[4][e] + ws[5][e];
+        out.c0a[tt][e] = ws[0][e];
+        out.c0b[tt][e] = ws[1][e];
+        out.cp[tt][e] = ws[3][e];
+        out.cm[tt][e] = ws[5][e];
+      }
+    };
+    advance(vn, 0);
+    if constexpr (NTK) advance(vt, NT - 1);
+    // The next step's matrix work overlaps this step's second-stage ReLU
+    // (past the sweep end it computes planes that are never used).
+    gram6(H + 1, out.acc);
+    // Second stage: (sq, rq) at rows 2H, 2H+1 of image i and 2Hp, 2Hp+1 of
+    // image j, columns (2W, 2W+1) and (2Wp, 2Wp+1).  The pool sums
+    // sq_i sq_j J over the 4 x 4 block as sum_x sq_i[x] sum_y sq_j[y] J.
+    float acc_n = 0.0f, acc_t = 0.0f;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {          // row of i: (2H + (x >> 1), 2W + (x & 1))
+      const float4 Ai = (x >> 1) ? in.r1 : in.r0;
+      const float sa = (x & 1) ? Ai.z : Ai.x, ra = (x & 1) ? Ai.w : Ai.y;
+      float inner = 0.0f;
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {        // (2Hp + (y >> 1), 2Wp + (y & 1))
+        const float4 Bj = (y >> 1) ? in.s1 : in.s0;
+        const float sb = (y & 1) ? Bj.z : Bj.x, rb = (y & 1) ? Bj.w : Bj.y;
+        // plane index of (row of i, row of j) = ((x >> 1), (y >> 1)):
+        // P00 (0,0) P11 (1,1) P10 (1,0) P01 (0,1); element (x & 1, y & 1)
+        const int pl = (x >> 1) == (y >> 1) ? (x >> 1) : 2 + (y >> 1);
+        const int e = 2 * (x & 1) + (y & 1);
+        const float cv = fmaf(wv2, pre[0][pl][e], b2);
+        const float tv = NTK ? fmaf(wv2, pre[NT - 1][pl][e], cv) : 0.0f;
+        const float rho = cv * (ra * rb);
+        const float a = fminf(fabsf(rho), 1.0f);
+        const float th = acos_pos5(a, 1.0f - a);
+        const float pm = rho < 0.0f ? th : kPi - th;
+        const float sn = fsqrt(fmaf(-a, a, 1.0f));
+        inner = fmaf(fmaf(pm, rho, sn), sb, inner);
+        if constexpr (NTK) acc_t = fmaf(pm, tv, acc_t);
+      }
+      acc_n = fmaf(inner, sa, acc_n);
+    }
+    const long long o =
+        p * PER_OUT + ((long long)(H * SO + Hp) * SO + W) * SO + Wp;
+    constexpr float kScale = kInv2Pi / 16.0f;
+    st_state(out_n + o, acc_n * kScale);
+    if constexpr (NTK) st_state(out_t + o, acc_t * kScale);
+  };
+
+  int H = Hs;
+  for (; H + 1 <= He; H += 2) {
+    step(H, A, B);
+    step(H + 1, B, A);
+  }
+  if (H <= He) step(H, A, B);
+}
+
+__global__ __launch_bounds__(256) void k_qnorm(const float* __restrict__ q,
+                                               long long n,
+                                               float2* __restrict__ out) {
+  for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < n;
+       k += (long long)gridDim.x * 256)
+    out[k] = qnorm(q[k]);
+}
+
 // ---------------------------------------------------------------- launchers
 
 static inline int pad8(long long n) { return (int)((n + 7) / 8 * 8); }
@@ -746,3 +1137,59 @@ int launch_first_fast(const float* x1, const float* x2, int n1, int n2, int S,
 }
 
 }  // namespace nta
+
+using namespace nta;
+
+// Conv(3x3 SAME) -> ReLU -> Conv(3x3 SAME) -> ReLU -> AvgPool(2x2 / 2) on
+// 32x32 images with C <= 4 channels, as one kernel (k_first_pool32).
+extern "C" int nta_first_pool_stage(
+    const float* x1, const float* x2, int n1, int n2, int H, int W, int C,
+    int flags, float w_var1, float b_var1, const float* qn1a,
+    const float* qn2a, float w_var2, float b_var2, const float* qn1b,
+    const float* qn2b, float* out_nngp, float* out_ntk, void* stream) {
+  if (n1 < 0 || n2 < 0)
+    return set_error(NTA_EINVAL, "nta_first_pool_stage: bad n1=%d n2=%d", n1,
+                     n2);
+  if (H != 32 || W != 32 || C < 1 || C > 4)
+    return set_error(NTA_EUNSUPPORTED, "nta_first_pool_stage: needs 32x32 "
+                     "images with 1..4 channels (got %dx%dx%d)", H, W, C);
+  if (n1 == 0) return NTA_OK;
+  const bool ntk = flags & NTA_NTK;
+  if (!x1 || (n2 > 0 && !x2) || !qn1a || !qn1b ||
+      (n2 > 0 && (!qn2a || !qn2b)) || !out_nngp || (ntk && !out_ntk))
+    return set_error(NTA_EINVAL, "nta_first_pool_stage: null pointer for a "
+                     "requested operand");
+  const long long T = n2 == 0 ? (long long)n1 : (long long)n1 * n2;
+  if (T * 31 > 0x7ffffff0LL)
+    return set_error(NTA_EINVAL, "nta_first_pool_stage: tile too large "
+                     "(%lld pairs)", T);
+  const int nblk = (int)((T * 31 + 7) / 8 * 8);
+  const float scale1 = w_var1 / (9.0f * (float)C);
+  const float wv9 = w_var2 / 9.0f;
+  const float2* a1 = reinterpret_cast<const float2*>(qn1a);
+  const float2* a2 = reinterpret_cast<const float2*>(qn2a);
+  const float2* c1 = reinterpret_cast<const float2*>(qn1b);
+  const float2* c2 = reinterpret_cast<const float2*>(qn2b);
+  if (ntk)
+    hipLaunchKernelGGL(k_first_pool32<true>, dim3(nblk), dim3(256), 0,
+                       (hipStream_t)stream, x1, x2, n1, n2, C, nblk, scale1,
+                       b_var1, a1, a2, wv9, b_var2, c1, c2, out_nngp, out_ntk);
+  else
+    hipLaunchKernelGGL(k_first_pool32<false>, dim3(nblk), dim3(256), 0,
+                       (hipStream_t)stream, x1, x2, n1, n2, C, nblk, scale1,
+                       b_var1, a1, a2, wv9, b_var2, c1, c2, out_nngp, out_ntk);
+  return check_launch("nta_first_pool_stage");
+}
+
+// (sqrt(q), 1 / sqrt(q)) per element, (0, 0) where q <= 0.
+extern "C" int nta_qnorm(const float* q, long long n, float* out,
+                         void* stream) {
+  if (n < 0 || (n > 0 && (!q || !out)))
+    return set_error(NTA_EINVAL, "nta_qnorm: bad n=%lld or null pointer", n);
+  if (n == 0) return NTA_OK;
+  const long long blocks = (n + 255) / 256;
+  const unsigned grid = (unsigned)(blocks < 65536 ? blocks : 65536);
+  hipLaunchKernelGGL(k_qnorm, dim3(grid), dim3(256), 0, (hipStream_t)stream,
+                     q, n, reinterpret_cast<float2*>(out));
+  return check_launch("nta_qnorm");
+}

@@ -48,3 +48,21 @@ def test_invalid_arguments_return_einval_without_gpu_work():
   rc = lib.nta_stage(None, None, 1, 1, 8, 8, 3, 3, 0, 1, 0, 1.0, 0.0, 1.0,
                      0.0, None, None, None, None, None)
   assert rc == _lib.NTA_EINVAL
+
+
+def test_first_pool_stage_rejects_other_shapes_without_gpu_work():
+  lib = _lib.load()
+  rc = lib.nta_first_pool_stage(None, None, 1, 1, 16, 16, 3, 0, 2.0, 0.0,
+                                None, None, 2.0, 0.0, None, None, None, None,
+                                None)
+  assert rc == _lib.NTA_EUNSUPPORTED
+  assert b'nta_first_pool_stage' in lib.nta_last_error()
+  rc = lib.nta_first_pool_stage(None, None, 1, 1, 32, 32, 5, 0, 2.0, 0.0,
+                                None, None, 2.0, 0.0, None, None, None, None,
+                                None)
+  assert rc == _lib.NTA_EUNSUPPORTED
+  rc = lib.nta_first_pool_stage(None, None, 2, 1, 32, 32, 3, 0, 2.0, 0.0,
+                                None, None, 2.0, 0.0, None, None, None, None,
+                                None)
+  assert rc == _lib.NTA_EINVAL
+  assert lib.nta_qnorm(None, -1, None, None) == _lib.NTA_EINVAL

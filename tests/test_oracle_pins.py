@@ -336,3 +336,28 @@ def test_finite_t_limits():
   np.testing.assert_allclose(zero, 0.0, atol=1e-14)
   train_big = po.ensemble_mean_finite_t(K, y, None, t=1e9)
   np.testing.assert_allclose(train_big, y, rtol=1e-7, atol=1e-9)
+
+
+def test_fast_arccos_polynomial_bound():
+  """acos_pos5 (csrc/nta_common.h), evaluated in FP32 with fused
+  multiply-adds, is within 9e-7 rad of arccos on [0, 1] (its coefficients are
+  read from the source)."""
+  import os
+  import re
+  src = open(os.path.join(os.path.dirname(__file__), '..',
+                          'neural_tangents_amd', 'csrc',
+                          'nta_common.h')).read()
+  body = src[src.index('float acos_pos5('):]
+  body = body[:body.index('return r * p;')]
+  coef = [float(x) for x in re.findall(r'(-?\d\.\d+e[-+]\d+)f', body)]
+  assert len(coef) == 6                    # highest degree first
+  a = np.linspace(0.0, 1.0, 400001).astype(np.float32)
+  p = np.full_like(a, np.float32(coef[0]))
+  for c in coef[1:]:                       # fma: one rounding per step
+    p = (p.astype(np.float64) * a + np.float32(c)).astype(np.float32)
+  u = (np.float32(1.0) - a).astype(np.float32)
+  r = np.sqrt(u.astype(np.float64)).astype(np.float32)
+  th = (r.astype(np.float64) * p).astype(np.float32).astype(np.float64)
+  err = np.abs(th - np.arccos(a.astype(np.float64)))
+  assert err.max() <= 9e-7, err.max()
+  assert th[-1] == 0.0                     # arccos(1) = 0 exactly

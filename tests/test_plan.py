@@ -109,3 +109,34 @@ def test_sum_pool_scale_folding():
   assert p.head_coef == (4 * a, 0.0, 4 * a, 4 * a, 0.0)
   p = _plan(conv(), stax.Relu(), stax.SumPool((2, 2), (2, 2)))
   assert p.out_scale == 16.0
+
+
+def test_first_two_stage_fusion_selection():
+  """The fused first-two-stage launch applies to Conv-Relu-Conv-Relu-Pool
+  heads on 32x32 images with <= 4 channels and nowhere else."""
+  for depth in (5, 7):
+    p = _engine.compile_plan(stax.myrtle(depth)[2].specs, (1, 32, 32, 3))
+    assert _engine._fuses_first_two(p, (1, 32, 32, 3))
+    assert not _engine._fuses_first_two(p, (1, 32, 32, 5))
+  # Myrtle-10 has a third Conv before the first pool
+  p = _engine.compile_plan(stax.myrtle(10)[2].specs, (1, 32, 32, 3))
+  assert not _engine._fuses_first_two(p, (1, 32, 32, 3))
+  p = _engine.compile_plan(stax.myrtle(5)[2].specs, (1, 16, 16, 3))
+  assert not _engine._fuses_first_two(p, (1, 16, 16, 3))
+  conv = lambda: stax.Conv(1, (3, 3), padding='SAME')
+  others = [
+      [conv(), stax.Relu(), conv(), stax.Relu(), stax.GlobalAvgPool()],
+      [conv(), stax.LeakyRelu(0.1), conv(), stax.Relu(),
+       stax.AvgPool((2, 2), (2, 2)), stax.GlobalAvgPool()],
+      [conv(), stax.Relu(), stax.Conv(1, (5, 5), padding='SAME'),
+       stax.Relu(), stax.AvgPool((2, 2), (2, 2)), stax.GlobalAvgPool()],
+      [conv(), stax.Relu(), stax.AvgPool((2, 2), (2, 2)), conv(),
+       stax.Relu(), stax.GlobalAvgPool()],
+  ]
+  for layers in others:
+    p = _plan(*layers)
+    assert not _engine._fuses_first_two(p, (1, 32, 32, 3))
+  # the fused tile budget counts only the pooled second-stage state
+  p = _engine.compile_plan(stax.myrtle(5)[2].specs, (1, 32, 32, 3))
+  assert _engine._per_pair_bytes(p, True, True) == 4 * 16 ** 4 * 2 * 2
+  assert _engine._per_pair_bytes(p, True) == 4 * 32 ** 4 * 2 * 2
