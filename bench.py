@@ -28,6 +28,8 @@ WORKLOADS = {
     'myrtle5_32x32_256x256': ((2, 1, 1), 32, 256, 256),
     'myrtle10_32x32_128x512': ((3, 3, 3), 32, 128, 512),
     'myrtle5_16x16_64x64': ((2, 1, 1), 16, 64, 64),
+    # one tile of 256 pairs: the launch the counter summaries are taken on
+    'myrtle5_32x32_16x16': ((2, 1, 1), 32, 16, 16),
 }
 
 
@@ -55,9 +57,7 @@ def canonical_bytes_per_pair(depths, side):
 
 # HBM traffic per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (scaled by a plain copy of known size), per trace label of the engine.
-PMC_FILE = os.path.join(ROOT, 'profiles', 'stage32_hbm_traffic_pmc.json')
-PMC_KERNELS = {'nta_stage[32->16]': 'k_stage_fast<32,2,true>',
-               'nta_stage[32->32]': 'k_stage_fast<32,1,true>'}
+PMC_FILE = os.path.join(ROOT, 'profiles', 'myrtle5_32x32_hbm_traffic_pmc.json')
 
 
 def measured_traffic(label, pairs_per_launch):
@@ -66,7 +66,7 @@ def measured_traffic(label, pairs_per_launch):
   try:
     with open(PMC_FILE) as f:
       d = json.load(f)
-    k = d['kernels'][PMC_KERNELS[label]]
+    k = d['kernels'][label]
   except (OSError, KeyError, ValueError):
     return None
   return k['traffic_bytes_per_launch'] * pairs_per_launch / k[
@@ -147,6 +147,17 @@ def krr_bench(args):
   print(json.dumps(out), flush=True)
 
 
+# How the algorithmic bytes of a launch relate to what it moves.
+SCHEDULE_NOTES = {
+    'nta_first_pool_stage':
+        'first two stages fused (Conv-Relu-Conv-Relu-AvgPool): achieved '
+        'counts the per-layer schedule\'s bytes for those two kernels (the '
+        'first stage state written and read back, the pooled state written); '
+        'the kernel never materialises the first stage, so traffic (measured '
+        'HBM bytes) is far below them and its bound is the vector ALU',
+}
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument('--gpus', type=int, default=1)
@@ -156,6 +167,9 @@ def main():
                   choices=sorted(WORKLOADS))
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--cpu-sample', type=int, default=14)
+  ap.add_argument('--pmc-calibration', action='store_true',
+                  help='first copy 256 MiB with torch (the reference launch '
+                  'that scales FETCH_SIZE / WRITE_SIZE counters)')
   ap.add_argument('--krr', action='store_true',
                   help='kernel-regression solve benchmark instead')
   ap.add_argument('--krr-train', type=int, default=8192)
@@ -174,6 +188,13 @@ def main():
   import torch
   from neural_tangents_amd import _engine, stax
   from neural_tangents_amd.data import blurred_images
+
+  if args.pmc_calibration:
+    src = torch.ones((64 << 20,), dtype=torch.float32, device='cuda')
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
 
   depths, side, n1, n2 = WORKLOADS[args.workload]
   x1n, x2n = blurred_images([n1, n2], side, seed=0)
@@ -238,7 +259,8 @@ def main():
                    'achieved': achieved, 'peak': HBM_PEAK_GBS,
                    'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
                    'traffic': traffic, 'avg_launch_ms': avg_ms,
-                   'alg_bytes_per_launch': avg_bytes},
+                   'alg_bytes_per_launch': avg_bytes,
+                   'note': SCHEDULE_NOTES.get(dom_name, '')},
       'kernels': {n: {'ms_per_step': v['ms'] / args.steps,
                       'launches_per_step': v['launches'] / args.steps,
                       'avg_launch_ms': v['ms'] / v['launches'],

@@ -296,8 +296,13 @@ def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
     out = torch.empty((T, s1.hout, s1.hout, s1.wout, s1.wout),
                       dtype=torch.float32, device=dev)
     out_t = torch.empty_like(out) if fl_ntk else None
-    _launch('nta_first_pool_stage', T * (2 if fl_ntk else 1) * 4 *
-            out[0].numel(), T, _lib.ptr(xa), _lib.ptr(xb), n1, n2, H, W, C,
+    # Algorithmic bytes: those of the two per-layer kernels it replaces
+    # (the first stage's state written and read back, the pooled state
+    # written), the accounting the per-layer schedule is measured against.
+    nt = 2 if fl_ntk else 1
+    alg = T * nt * 4 * (2 * _state_elems(s0.hc, s0.wc) + out[0].numel())
+    _launch('nta_first_pool_stage', alg, T, _lib.ptr(xa), _lib.ptr(xb), n1,
+            n2, H, W, C,
             _lib.NTK if fl_ntk else 0, s0.conv[3], s0.conv[4],
             _lib.ptr(qn[0]), _lib.ptr(qn[1]), s1.conv[3], s1.conv[4],
             _lib.ptr(qn[2]), _lib.ptr(qn[3]), _lib.ptr(out),
