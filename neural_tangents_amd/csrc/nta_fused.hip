@@ -24,6 +24,8 @@ namespace nta {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef _Float16 half4v __attribute__((ext_vector_type(4)));
+typedef _Float16 half8v __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ float2 qnorm(float q) {
   float sq = fsqrt(fmaxf(q, 0.0f));
@@ -678,20 +680,25 @@ __global__ __launch_bounds__(256) void k_first16(
 //   plane (h, h') = ReLU(scale1 * P_i(h) P_j(h')^T + b1),
 // P_n(h) the [32 w][9 taps x 4 channels] patch matrix of image n at row h
 // (the Gram of zero-padded patches equals the input-covariance stencil).
-// Each wave owns one 16x16 (w, w') quadrant of every plane: 9
-// v_mfma_f32_16x16x4_f32 (one per tap, the 4 channels across lane groups)
-// and the ReLU epilogue.  The planes then go through the LDS exchange of
-// the second stage (nngp first, ntk second).  Planes on odd first-stage
-// diagonals are computed by both workgroups that need them (1.5x the
-// first-stage work), which costs far less than the 16 MiB per pair of
-// state the unfused pair of stages writes and reads back.
-constexpr int kXS = 34;               // padded image row (floats)
-// Padded channel plane: 34 * 34 rounded up to 16 mod 32 floats, so the
-// lane groups of the four channels read disjoint LDS banks.
-constexpr int kXP = 34 * 34 + 12;
+// Each wave owns one 16x16 (w, w') quadrant of every plane.  The Gram runs
+// on the FP16 matrix pipe with FP32 accuracy: each image is scaled by a
+// power of two and split as hi + lo (two FP16 numbers), and the three
+// products hi hi + hi lo + lo hi (v_mfma_f32_16x16x32_f16, two per product
+// for the 9 x 4 patch entries) accumulate exactly-multiplied terms in FP32;
+// the dropped lo lo term is below 2^-22 of each product.  That is 6 MFMAs
+// of 16 cycles per plane quadrant where FP32 MFMAs need 9 of 32, and the
+// fused kernel's matrix time (it does not overlap the vector ALU) drops
+// 3x.  The ReLU is the epilogue.  The planes then go through the LDS
+// exchange of the second stage (nngp first, ntk second).  Planes on odd
+// first-stage diagonals are computed by both workgroups that need them
+// (1.5x the first-stage work), which costs far less than the 16 MiB per
+// pair of state the unfused pair of stages writes and reads back.
+constexpr int kXS = 34;               // padded image row (positions)
 constexpr int kLS2 = 36;              // exchange plane row (4 rows = 16 banks)
+// Vector instructions scheduled between consecutive MFMAs of the next step's
+// patch Grams (0: leave the order to the compiler).
 #ifndef NTA_F12_SCHED
-#define NTA_F12_SCHED 0
+#define NTA_F12_SCHED 12
 #endif
 constexpr int kPL2 = 34 * kLS2;
 
@@ -725,8 +732,11 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
   constexpr int S = 32, H2 = 16, SO = 16, NDIAG = 2 * H2 - 1;
   constexpr long long PER_OUT = (long long)SO * SO * SO * SO;
   constexpr int NT = NTK ? 2 : 1;
-  __shared__ __align__(16) float xs[2 * 4 * kXP];     // images i, j
+  // Images i, j as FP16 hi/lo pairs: xh[img][hl][1 + h][1 + w][c] (zero
+  // padded, channels padded to 4).
+  __shared__ __align__(16) _Float16 xh[2 * 2 * kXS * kXS * 4];
   __shared__ __align__(16) float lp[6 * kPL2];        // exchange planes
+  __shared__ unsigned smax[2];
   const bool self = n2 == 0;
   const long long T = self ? (long long)n1 : (long long)n1 * n2;
   const int L = xcd_remap(blockIdx.x, nblk);
@@ -749,26 +759,83 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
   const int wr = (w0 + 1) * kLS2 + wq + 1;
 
   for (int k = t; k < 6 * kPL2; k += 256) lp[k] = 0.0f;
+  if (t < 2) smax[t] = 0u;
+  const float* xi = x1 + i * (S * S * C);
+  const float* xj = (self ? x1 : x2) + j * (S * S * C);
+  __syncthreads();
   {
-    // Zero-padded, channel-planar images: xs[img][c][1 + h][1 + w].
-    const float* xi = x1 + i * (S * S * C);
-    const float* xj = (self ? x1 : x2) + j * (S * S * C);
-    for (int k = t; k < 2 * 4 * kXS * kXS; k += 256) {
-      const int img = k / (4 * kXS * kXS);
-      const int r = k - img * (4 * kXS * kXS);
-      const int c = r / (kXS * kXS);
-      const int rr = r - c * (kXS * kXS);
-      const int hh = rr / kXS - 1;
-      const int ww = rr - (rr / kXS) * kXS - 1;
-      float v = 0.0f;
-      if (c < C && (unsigned)hh < (unsigned)S && (unsigned)ww < (unsigned)S)
-        v = (img ? xj : xi)[(hh * S + ww) * C + c];
-      xs[img * 4 * kXP + c * kXP + rr] = v;
+    // Per-image power-of-two scale 2^-e with max |x| 2^-e in [0.5, 1):
+    // exact, and it keeps the FP16 parts in range.
+    float mi = 0.0f, mj = 0.0f;
+    for (int k = t; k < S * S * C; k += 256) {
+      mi = fmaxf(mi, fabsf(xi[k]));
+      mj = fmaxf(mj, fabsf(xj[k]));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      mi = fmaxf(mi, __shfl_xor(mi, o));
+      mj = fmaxf(mj, __shfl_xor(mj, o));
+    }
+    if (lane == 0) {
+      atomicMax(&smax[0], __float_as_uint(mi));
+      atomicMax(&smax[1], __float_as_uint(mj));
     }
   }
   __syncthreads();
-  const float* xa = xs + c4 * kXP + 16 * (wave >> 1) + m16;
-  const float* xb = xs + 4 * kXP + c4 * kXP + 16 * (wave & 1) + m16;
+  int ei, ej;
+  frexpf(__uint_as_float(smax[0]), &ei);
+  frexpf(__uint_as_float(smax[1]), &ej);
+  // x = 2^e (hi + lo) with hi = fp16(x 2^-e), lo = fp16(x 2^-e - hi): the
+  // patch Gram is 2^(ei+ej) (hi_i hi_j + hi_i lo_j + lo_i hi_j) up to
+  // |lo_i lo_j| <= 2^-22 |x_i x_j|, each FP16 product exact in the FP32
+  // accumulator.
+  const float scale_g = ldexpf(scale1, ei + ej);
+  {
+    const float si = ldexpf(1.0f, -ei), sj = ldexpf(1.0f, -ej);
+    for (int k = t; k < 2 * kXS * kXS * 4; k += 256) {
+      const int img = k / (kXS * kXS * 4);
+      const int r = k - img * (kXS * kXS * 4);
+      const int pos = r >> 2, c = r & 3;
+      const int hh = pos / kXS - 1;
+      const int ww = pos - (pos / kXS) * kXS - 1;
+      float v = 0.0f;
+      if (c < C && (unsigned)hh < (unsigned)S && (unsigned)ww < (unsigned)S)
+        v = img ? xj[(hh * S + ww) * C + c] * sj : xi[(hh * S + ww) * C + c] * si;
+      const _Float16 hv = (_Float16)v;
+      xh[(img * 2) * (kXS * kXS * 4) + r] = hv;
+      xh[(img * 2 + 1) * (kXS * kXS * 4) + r] = (_Float16)(v - (float)hv);
+    }
+  }
+  __syncthreads();
+  // MFMA operands (v_mfma_f32_16x16x32_f16, 8 halves per lane): lane group
+  // c4 carries two patch positions of 4 channels each.  Step 0 holds
+  // positions 2 c4, 2 c4 + 1 (0..7), step 1 position 8 in group 0; the
+  // empty slots point at padded column 0, which is zero on every row.
+  // Byte offsets from the image base, before the row term.
+  constexpr int kRowB = kXS * 4 * 2;            // bytes per padded row
+  constexpr int kHlB = kXS * kXS * 4 * 2;       // bytes of one hi or lo image
+  int oa[2][2], ob[2][2];
+#pragma unroll
+  for (int st = 0; st < 2; ++st)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int pos = st == 0 ? 2 * c4 + h : (c4 == 0 && h == 0 ? 8 : -1);
+      const int ty = pos / 3, tx = pos - 3 * (pos / 3);
+      oa[st][h] = pos < 0 ? 0
+                          : ty * kRowB + (16 * (wave >> 1) + m16 + tx) * 8;
+      ob[st][h] = pos < 0 ? 0
+                          : ty * kRowB + (16 * (wave & 1) + m16 + tx) * 8;
+    }
+  const char* xhb = reinterpret_cast<const char*>(xh);
+  // Operand of image img (0: i, 1: j), part hl, padded row base `row`,
+  // step st: the 8 halves of the lane.
+  auto opnd = [&](int img, int hl, int row, int st) {
+    const char* b = xhb + (img * 2 + hl) * kHlB + row * kRowB;
+    const int* o = img ? ob[st] : oa[st];
+    const half4v lo = *reinterpret_cast<const half4v*>(b + o[0]);
+    const half4v hi = *reinterpret_cast<const half4v*>(b + o[1]);
+    return half8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+  };
   const float2* qi = qa1 + i * (S * S);
   const float2* qj = (self ? qa1 : qa2) + j * (S * S);
   const float2* ri = qb1 + i * (S * S);
@@ -783,13 +850,13 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
     if (!ok) hh = hp = 0;              // uniform; the values are dropped
     floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
     if (ok) {
-      const float* pa = xa + hh * kXS;
-      const float* pb = xb + hp * kXS;
 #pragma unroll
-      for (int kk = 0; kk < 9; ++kk) {
-        const int o = (kk / 3) * kXS + kk % 3;
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[o], pb[o], acc, 0, 0,
-                                                   0);
+      for (int st = 0; st < 2; ++st) {
+        const half8v ah = opnd(0, 0, hh, st), al = opnd(0, 1, hh, st);
+        const half8v bh = opnd(1, 0, hp, st), bl = opnd(1, 1, hp, st);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc, 0, 0, 0);
       }
     }
     const float4 q01 = *reinterpret_cast<const float4*>(qi + hh * S + w0);
@@ -799,7 +866,7 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
     for (int e = 0; e < 4; ++e) {
       const float4 Q = e < 2 ? q01 : q23;
       const float sa = (e & 1) ? Q.z : Q.x, ra = (e & 1) ? Q.w : Q.y;
-      const float cv = fmaf(scale1, acc[e], b1);
+      const float cv = fmaf(scale_g, acc[e], b1);
       const float rho = cv * (ra * qb.y);
       const float a = fminf(fabsf(rho), 1.0f);
       const float u = 1.0f - a;
@@ -865,31 +932,38 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
     return *reinterpret_cast<const float4*>(
         tab + (row < S ? row : S - 1) * S + col);
   };
-  // First-stage patch Grams of the six planes of step Hn: operands of
-  // padded image rows 2Hn + u (u = 0..4, clamped into the padded image:
-  // such planes are outside the grid and dropped), three taps per row.
+  // First-stage patch Grams of the six planes of step Hn (split FP16, 36
+  // MFMAs).  Padded rows 2Hn + r, r = 0..2 (+ 0..2 taps); past the grid
+  // (r = 2 at the last step) they read the rows below, or the lo image or
+  // the words after it: finite values of planes that are dropped.
   auto gram6 = [&](int Hn, floatx4 (&acc)[6]) {
     const int Hpn = Hn - D;
-    float av[5][3], bv[5][3];
-#pragma unroll
-    for (int u = 0; u < 5; ++u) {
-      const int ra = 2 * Hn + u < kXS - 1 ? 2 * Hn + u : kXS - 1;
-      const int rb = 2 * Hpn + u < kXS - 1 ? 2 * Hpn + u : kXS - 1;
-#pragma unroll
-      for (int tx = 0; tx < 3; ++tx) {
-        av[u][tx] = xa[ra * kXS + tx];
-        bv[u][tx] = xb[rb * kXS + tx];
-      }
-    }
 #pragma unroll
     for (int k = 0; k < 6; ++k) acc[k] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int kk = 0; kk < 9; ++kk) {
-      const int ty = kk / 3, tx = kk % 3;
+    for (int st = 0; st < 2; ++st) {
+      half8v ah[3], al[3], bh[3], bl[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        ah[r] = opnd(0, 0, 2 * Hn + r, st);
+        bh[r] = opnd(1, 0, 2 * Hpn + r, st);
+      }
 #pragma unroll
       for (int k = 0; k < 6; ++k)
-        acc[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-            av[ar[k] + ty][tx], bv[br[k] + ty][tx], acc[k], 0, 0, 0);
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[ar[k]], bh[br[k]],
+                                                        acc[k], 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) bl[r] = opnd(1, 1, 2 * Hpn + r, st);
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[ar[k]], bl[br[k]],
+                                                        acc[k], 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) al[r] = opnd(0, 1, 2 * Hn + r, st);
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[ar[k]], bh[br[k]],
+                                                        acc[k], 0, 0, 0);
     }
   };
   {
@@ -920,7 +994,7 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
       for (int e = 0; e < 4; ++e) {
         const float4 Q = e < 2 ? in.qal[ar[k]] : in.qah[ar[k]];
         const float sa = (e & 1) ? Q.z : Q.x, ra = (e & 1) ? Q.w : Q.y;
-        const float cv = fmaf(scale1, in.acc[k][e], b1);
+        const float cv = fmaf(scale_g, in.acc[k][e], b1);
         const float rho = cv * (ra * qb.y);
         const float a = fminf(fabsf(rho), 1.0f);
         const float th = acos_pos5(a, 1.0f - a);
@@ -1007,6 +1081,15 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
       }
       acc_n = fmaf(inner, sa, acc_n);
     }
+#if NTA_F12_SCHED
+    // One MFMA per NTA_F12_SCHED vector instructions of the second-stage
+    // ReLU: the wave keeps issuing while the matrix pipe runs.
+#pragma unroll
+    for (int m = 0; m < 36; ++m) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, NTA_F12_SCHED, 0);
+    }
+#endif
     const long long o =
         p * PER_OUT + ((long long)(H * SO + Hp) * SO + W) * SO + Wp;
     constexpr float kScale = kInv2Pi / 16.0f;

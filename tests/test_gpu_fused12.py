@@ -73,3 +73,21 @@ def test_fused_kernel_matrix_matches_stagewise(monkeypatch):
   _, p = _both(monkeypatch, kfn, x1, x2, ('nngp', 'ntk'))
   _close(f.nngp, p.nngp, 'nngp tiled')
   _close(f.ntk, p.ntk, 'ntk tiled')
+
+
+def test_split_fp16_gram_keeps_fp32_accuracy(monkeypatch):
+  """The fused kernel's patch Gram runs on FP16 MFMAs over a hi/lo split of
+  power-of-two scaled images: inputs spanning six decades, a large overall
+  scale and an all-zero image must match the FP32 stage-by-stage path."""
+  rng = np.random.default_rng(7)
+  x1 = rng.standard_normal((3, 32, 32, 3)) * 10.0 ** rng.uniform(
+      -3, 3, (3, 32, 32, 3))
+  x2 = rng.standard_normal((3, 32, 32, 3)) * 3e4
+  x2[1] = 0.0
+  x1 = torch.from_numpy(x1.astype(np.float32)).to(DEV)
+  x2 = torch.from_numpy(x2.astype(np.float32)).to(DEV)
+  kfn = _net(0.2, 0.1, readout=False)
+  f, p = _both(monkeypatch, kfn, x1, x2, ('nngp', 'ntk'))
+  assert torch.isfinite(f.nngp).all() and torch.isfinite(f.ntk).all()
+  _close(f.nngp, p.nngp, 'nngp')
+  _close(f.ntk, p.ntk, 'ntk')
