@@ -1145,6 +1145,12 @@ static int launch_stage_s(const float* in_n, const float* in_t, int n1,
 #define NTA_STAGE_PREFETCH 2
 #endif
 constexpr int kPrefetch = NTA_STAGE_PREFETCH;
+// 16x16: loads at the step that uses them (three workgroups per CU hide
+// them better than a register prefetch with two).
+#ifndef NTA_STAGE16_PREFETCH
+#define NTA_STAGE16_PREFETCH 0
+#endif
+constexpr int kPrefetch16 = NTA_STAGE16_PREFETCH;
 
 // The fused kernels implement the ReLU map only (k1 = 1, k0 = 0); other
 // ABRelu slopes run the generic kernels.
@@ -1161,7 +1167,7 @@ int launch_stage_fast(const float* in_nngp, const float* in_ntk, int n1,
       case 32: return launch_stage_s<32, 2, kPrefetch>(in_nngp, in_ntk, n1, n2, ntk, wv9,
                                             b_var, q1, q2, out_nngp, out_ntk,
                                             stream);
-      case 16: return launch_stage_s<16, 2, 1>(in_nngp, in_ntk, n1, n2, ntk, wv9,
+      case 16: return launch_stage_s<16, 2, kPrefetch16>(in_nngp, in_ntk, n1, n2, ntk, wv9,
                                             b_var, q1, q2, out_nngp, out_ntk,
                                             stream);
       case 8: return launch_stage_s<8, 2, 1>(in_nngp, in_ntk, n1, n2, ntk, wv9,
