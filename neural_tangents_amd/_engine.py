@@ -35,6 +35,7 @@ class Stage:
   conv: Optional[tuple] = None     # (fh, fw, valid, w_var, b_var)
   relu: bool = False
   act: Tuple[float, float] = (1.0, 0.0)   # ABRelu (k1, k0) = ((a-b)^2, ab)
+  act_kind: int = 0                # 0 ABRelu, _lib.ACT_ERF, _lib.ACT_GELU
   wscale: float = 1.0              # SumPool factor folded into conv W_std^2
   pool: Optional[tuple] = None     # (ph, pw)
   first: bool = False
@@ -53,7 +54,7 @@ class Stage:
       if self.conv[2]:
         f |= _lib.CONV_VALID
     if self.relu:
-      f |= _lib.RELU
+      f |= _lib.RELU | self.act_kind
     if ntk:
       f |= _lib.NTK
     return f
@@ -150,7 +151,7 @@ def compile_plan(specs, in_shape) -> Plan:
       is_gaussian = True
       chans = spec.out_chan
       continue
-    if isinstance(spec, S._ReluSpec) and not is_gaussian:
+    if isinstance(spec, (S._ReluSpec, S._ActSpec)) and not is_gaussian:
       raise ValueError('The input to the activation function must be '
                        'Gaussian, i.e. a random affine transform is '
                        'required before the activation function.')
@@ -163,6 +164,30 @@ def compile_plan(specs, in_shape) -> Plan:
         cur = Stage()
       cur.relu = True
       cur.act = ((spec.a - spec.b) ** 2, spec.a * spec.b)
+      is_gaussian = False
+      continue
+    if isinstance(spec, S._ActSpec):
+      if cur is None or cur.relu or cur.pool is not None:
+        close()
+        cur = Stage()
+      cur.relu = True
+      if spec.kind == 'erf':
+        # a Erf(b x) + c: the input scale b multiplies the pre-activation
+        # (nngp, ntk, cov) by b^2 (elementwise.py:67-68, kernel.py:426-431),
+        # i.e. the stage's conv W_std^2 and b_std^2.
+        if spec.b != 1.0:
+          if cur.conv is None:
+            raise NotImplementedError('Erf with an input scale b != 1 needs '
+                                      'a Conv right before it.')
+          fh, fw, valid, wv, bv = cur.conv
+          b2 = float(spec.b) ** 2
+          cur.conv = (fh, fw, valid, wv * b2, bv * b2)
+          cur.wscale *= b2
+        cur.act_kind = _lib.ACT_ERF
+        cur.act = (float(spec.a) ** 2, float(spec.c) ** 2)
+      else:
+        cur.act_kind = _lib.ACT_GELU
+        cur.act = (0.0, 0.0)
       is_gaussian = False
       continue
     if isinstance(spec, S._AvgPoolSpec):
@@ -390,8 +415,10 @@ def _fuses_first_two(plan: Plan, in_shape) -> bool:
   def conv33(s):
     return s.conv is not None and s.conv[:3] == (3, 3, False)
   return (H == 32 and W == 32 and 1 <= C <= 4 and s0.first and conv33(s0)
-          and s0.relu and s0.act == (1.0, 0.0) and s0.pool is None
+          and s0.relu and s0.act == (1.0, 0.0) and s0.act_kind == 0
+          and s0.pool is None
           and conv33(s1) and s1.relu and s1.act == (1.0, 0.0)
+          and s1.act_kind == 0
           and s1.pool == (2, 2))
 
 
@@ -457,7 +484,7 @@ def _diag_tables(plan: Plan):
       fh, fw, valid, wv, bv = s.conv
       f |= _lib.CONV | (_lib.CONV_VALID if valid else 0)
     if s.relu:
-      f |= _lib.RELU
+      f |= _lib.RELU | s.act_kind
     desc[3 * k:3 * k + 3] = [f, fh, fw]
     wb[4 * k:4 * k + 4] = [wv, bv, s.act[0], s.act[1]]
   return desc, wb

@@ -19,6 +19,8 @@ NTK = 2
 CONV = 4
 CONV_VALID = 8
 FLATTEN = 16
+ACT_ERF = 32
+ACT_GELU = 64
 
 _lock = threading.Lock()
 _lib = None

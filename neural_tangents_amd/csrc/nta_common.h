@@ -75,6 +75,49 @@ __device__ __forceinline__ float abrelu_map(float c, float q1, float q2,
   return fmaf(k1, j, k0 * c);
 }
 
+// Erf(a, b, c) on kernels (elementwise.py:49-108; the input scale b is
+// applied to (c, q1, q2) by the caller): with d = (1 + 2 q1)(1 + 2 q2) - 4 c^2
+// = 1 + 2 (q1 + q2) + 4 (q1 q2 - c^2) > 0,
+//   nngp' = k1 (2/pi) atan2(2 c, sqrt(d)) + k0,
+//   dot_sigma = k1 (4/pi) / sqrt(d)
+// with k1 = a^2, k0 = c_shift^2 (a * Kernel + c, kernel.py:426-439).
+__device__ __forceinline__ float erf_map(float c, float q1, float q2,
+                                         float k1, float k0,
+                                         float* dot_sigma) {
+  const float d = fmaf(4.0f, prod_minus_sq(q1, q2, c),
+                       fmaf(2.0f, q1 + q2, 1.0f));
+  const float s = sqrtf(fmaxf(d, 0.0f));
+  *dot_sigma = k1 * (2.0f / kPi) * 2.0f / s;
+  return fmaf(k1 * (2.0f / kPi), atan2f(2.0f * c, s), k0);
+}
+
+// Gelu on kernels (elementwise.py:197-277): with p = q1 q2,
+// P = (q1 + 1)(q2 + 1), D^2 = P - c^2 = (q1 q2 - c^2) + q1 + q2 + 1 > 0,
+// t = atan2(c, D):
+//   nngp' = ((c^2 + p D^2) / (P D) + c t) / (2 pi) + c / 4,
+//   dot_sigma = c / (2 pi D) (1 / D^2 + (1 - p) / P + 1) + 1/4 + t / (2 pi).
+__device__ __forceinline__ float gelu_map(float c, float q1, float q2,
+                                          float* dot_sigma) {
+  const float p = q1 * q2;
+  const float P = (q1 + 1.0f) * (q2 + 1.0f);
+  const float D2 = fmaxf(prod_minus_sq(q1, q2, c) + (q1 + q2 + 1.0f), 0.0f);
+  const float D = sqrtf(D2);
+  const float t = atan2f(c, D);
+  *dot_sigma = c / (2.0f * kPi * D) * (1.0f / D2 + (1.0f - p) / P + 1.0f) +
+               0.25f + t * kInv2Pi;
+  return (fmaf(c, c, p * D2) / (P * D) + c * t) * kInv2Pi + 0.25f * c;
+}
+
+// The activation of a stage: flags & NTA_ACT_ERF / NTA_ACT_GELU select Erf or
+// Gelu, otherwise ABRelu(k1, k0).
+__device__ __forceinline__ float act_map(int flags, float c, float q1,
+                                         float q2, float k1, float k0,
+                                         float* dot_sigma) {
+  if (flags & NTA_ACT_ERF) return erf_map(c, q1, q2, k1, k0, dot_sigma);
+  if (flags & NTA_ACT_GELU) return gelu_map(c, q1, q2, dot_sigma);
+  return abrelu_map(c, q1, q2, k1, k0, dot_sigma);
+}
+
 // Normalised form of the same map, used by the fused kernels.  With
 // sq = sqrt(q) and rq = 1/sqrt(q) (0 when q == 0) per position,
 //   rho = c rq1 rq2, theta = arccos(rho) (rho clamped to [-1, 1]),

@@ -19,7 +19,7 @@ constexpr int kDiagMaxStages = 48;    // the stage table travels in kernargs
 constexpr int kDiagMaxHW = 3840;       // 4 planes + reduction <= 64 KiB LDS
 
 struct DiagStage {
-  int conv, fh, fw, lo_h, lo_w, hin, win, hout, wout, relu;
+  int conv, fh, fw, lo_h, lo_w, hin, win, hout, wout, relu, act;
   float w_var, b_var, k1, k0;
 };
 struct DiagPlan {
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_diag_chain(
           }
           float ds;
           // elementwise.py:444-455
-          const float nv = abrelu_map(cv, qa, qb, S.k1, S.k0, &ds);
+          const float nv = act_map(S.act, cv, qa, qb, S.k1, S.k0, &ds);
           tv *= ds;
           cv = nv;
         }
@@ -218,6 +218,7 @@ extern "C" int nta_diag_chain(const float* x1, const float* x2, int n1,
     const int f = stage_desc[3 * s];
     S.conv = (f & NTA_CONV) ? 1 : 0;
     S.relu = (f & NTA_RELU) ? 1 : 0;
+    S.act = f & (NTA_ACT_ERF | NTA_ACT_GELU);
     S.fh = S.conv ? stage_desc[3 * s + 1] : 1;
     S.fw = S.conv ? stage_desc[3 * s + 2] : 1;
     S.w_var = stage_wb[4 * s];

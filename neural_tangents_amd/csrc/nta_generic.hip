@@ -132,7 +132,7 @@ __global__ void k_first_stage(const float* __restrict__ x1,
       float qa = q1[((uint64_t)i * Hc + h1) * Wc + w1];
       float qb = (self ? q1 : q2)[((uint64_t)j * Hc + h2) * Wc + w2];
       float ds;
-      v = abrelu_map(v, qa, qb, k1, k0, &ds);
+      v = act_map(flags, v, qa, qb, k1, k0, &ds);
       t *= ds;
     }
     nngp[idx] = v;
@@ -235,7 +235,7 @@ __global__ void k_stage(const float* __restrict__ in_nngp,
               float qa = q1[((uint64_t)i * Hc + h1) * Wc + w1];
               float qb = (self ? q1 : q2)[((uint64_t)j * Hc + h2) * Wc + w2];
               float ds;
-              v = abrelu_map(v, qa, qb, k1, k0, &ds);
+              v = act_map(flags, v, qa, qb, k1, k0, &ds);
               t *= ds;
             }
             acc_n += v;

@@ -95,6 +95,9 @@ def _check_plan(plan):
   if plan.out_scale != 1.0 or any(s.wscale != 1.0 for s in plan.stages):
     raise NotImplementedError('kernel gradients do not support SumPool.')
   for s in plan.stages:
+    if s.act_kind:
+      raise NotImplementedError('kernel gradients support the ABRelu family '
+                                '(Relu, LeakyRelu, Abs) only.')
     if s.conv is None or not s.relu:
       raise NotImplementedError('kernel gradients support stages of Conv + '
                                 'Relu [+ AvgPool] only.')

@@ -55,6 +55,17 @@ class _ReluSpec:
 
 
 @dataclasses.dataclass(frozen=True)
+class _ActSpec:
+  """Erf: a erf(b x) + c; Gelu (``approximate`` only changes apply_fn)."""
+  kind: str
+  a: float = 1.0
+  b: float = 1.0
+  c: float = 0.0
+  approximate: bool = False
+  name: str = ''
+
+
+@dataclasses.dataclass(frozen=True)
 class _AvgPoolSpec:
   window_shape: tuple
   strides: tuple
@@ -168,6 +179,16 @@ def _relu_layer_fns(spec):
   return (lambda rng, s: (tuple(s), ())), apply_fn
 
 
+def _act_layer_fns(spec):
+  if spec.kind == 'erf':
+    def apply_fn(p, x, **kw):
+      return spec.a * torch.erf(spec.b * x) + spec.c
+  else:
+    def apply_fn(p, x, **kw):
+      return F.gelu(x, approximate='tanh' if spec.approximate else 'none')
+  return (lambda rng, s: (tuple(s), ())), apply_fn
+
+
 def _avg_pool_layer_fns(spec: _AvgPoolSpec):
   wh, ww = spec.window_shape
   sh, sw = spec.strides
@@ -225,6 +246,7 @@ _FNS = {
     _ConvSpec: _conv_layer_fns,
     _DenseSpec: _dense_layer_fns,
     _ReluSpec: _relu_layer_fns,
+    _ActSpec: _act_layer_fns,
     _AvgPoolSpec: _avg_pool_layer_fns,
     _GlobalAvgPoolSpec: _gap_layer_fns,
     _FlattenSpec: _flatten_layer_fns,
@@ -414,6 +436,19 @@ def LeakyRelu(alpha: float, do_stabilize: bool = False):
 def Abs(do_stabilize: bool = False):
   """``Abs`` (``elementwise.py:509-520``) = ``ABRelu(-1, 1)``."""
   return _layer(_ReluSpec(-1.0, 1.0, name='Abs'))
+
+
+def Erf(a: float = 1., b: float = 1., c: float = 0.):
+  """``Erf`` (``elementwise.py:49-108``): ``a erf(b x) + c``; on kernels the
+  arcsine map of ``b^2 k`` scaled by ``a^2`` plus ``c^2``."""
+  return _layer(_ActSpec('erf', float(a), float(b), float(c),
+                         name=f'Erf({a}, {b}, {c})'))
+
+
+def Gelu(approximate: bool = False):
+  """``Gelu`` (``elementwise.py:197-277``); ``approximate`` only affects the
+  finite-width ``apply_fn``, as in the reference."""
+  return _layer(_ActSpec('gelu', approximate=bool(approximate), name='Gelu'))
 
 
 def AvgPool(window_shape: Sequence[int],

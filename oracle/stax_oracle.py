@@ -324,6 +324,108 @@ def relu_layer():
   return abrelu_layer(0.0, 1.0)
 
 
+def _scale_kernel(k: OKernel, a: float) -> OKernel:
+  """``Kernel.__mul__`` ``utils/kernel.py:426-431``: every covariance times
+  a^2."""
+  var = k.nngp.dtype.type(a ** 2)
+  return k.replace(nngp=var * k.nngp, cov1=var * k.cov1,
+                   cov2=None if k.cov2 is None else var * k.cov2,
+                   ntk=None if k.ntk is None else var * k.ntk)
+
+
+def _check_gaussian(k: OKernel):
+  if not k.is_gaussian:
+    raise ValueError('The input to the activation function must be '
+                     'Gaussian, i.e. a random affine transform is required '
+                     'before the activation function.')
+
+
+def erf_nngp_ntk(nngp, prod, ntk):
+  """``Erf.nngp_ntk_fn`` ``elementwise.py:80-95`` (prod = outer product of
+  the diagonals of 1 + 2 cov)."""
+  dt = nngp.dtype.type
+  factor = dt(2 / math.pi)
+  square_root = _sqrt(prod - 4 * nngp ** 2)
+  out = factor * np.arctan2(2 * nngp, square_root)
+  if ntk is not None:
+    ntk = ntk * (2 * factor / square_root)
+  return out, ntk
+
+
+def erf_layer(a=1.0, b=1.0, c=0.0):
+  """``Erf(a, b, c)`` ``elementwise.py:49-108``: ``k *= b``, the arcsine
+  map, then ``a * k + c`` (``utils/kernel.py:426-439``: a^2 on every
+  covariance, c^2 added to nngp and cov)."""
+  def kernel_fn(k: OKernel) -> OKernel:
+    _check_gaussian(k)
+    k = _scale_kernel(k, b)
+    dt = k.nngp.dtype.type
+    d1 = 1 + 2 * k.cov1
+    d2 = None if k.cov2 is None else 1 + 2 * k.cov2
+    prod11, prod12, prod22 = get_diagonal_outer_prods(d1, d2,
+                                                      k.diagonal_spatial)
+    nngp, ntk = erf_nngp_ntk(k.nngp, prod12, k.ntk)
+    if k.diagonal_spatial:                    # 97-98, 102-105
+      diag = lambda x: dt(2 / math.pi) * np.arctan2(x, np.sqrt(x + 0.25))
+      cov1 = diag(k.cov1)
+      cov2 = None if k.cov2 is None else diag(k.cov2)
+    else:
+      cov1, _ = erf_nngp_ntk(k.cov1, prod11, None)
+      cov2 = None if k.cov2 is None else erf_nngp_ntk(k.cov2, prod22,
+                                                      None)[0]
+    k = k.replace(cov1=cov1, nngp=nngp, cov2=cov2, ntk=ntk,
+                  is_gaussian=False)
+    k = _scale_kernel(k, a)
+    cc = dt(c ** 2)
+    return k.replace(nngp=k.nngp + cc, cov1=k.cov1 + cc,
+                     cov2=None if k.cov2 is None else k.cov2 + cc)
+  return kernel_fn
+
+
+def gelu_nngp_ntk(nngp, prod, prod_plus_1, ntk):
+  """``Gelu.nngp_ntk_fn`` ``elementwise.py:228-248``."""
+  dt = nngp.dtype.type
+  delta_squared = prod_plus_1 - nngp ** 2
+  delta = _sqrt(delta_squared)
+  angles = np.arctan2(nngp, delta)
+  new = (nngp ** 2 + prod * delta_squared) / (prod_plus_1 * delta)
+  new = new + nngp * angles
+  new = new / dt(2 * math.pi) + dt(0.25) * nngp
+  if ntk is not None:
+    second = dt(0.25) + angles / dt(2 * math.pi)
+    first = 1 / delta_squared + (1 - prod) / prod_plus_1 + 1
+    first = first * nngp / delta / dt(2 * math.pi)
+    ntk = ntk * (first + second)
+  return new, ntk
+
+
+def gelu_layer():
+  """``Gelu`` ``elementwise.py:197-277``."""
+  def kernel_fn(k: OKernel) -> OKernel:
+    _check_gaussian(k)
+    dt = k.nngp.dtype.type
+    c1p = k.cov1 + 1
+    c2p = None if k.cov2 is None else k.cov2 + 1
+    p11p, p12p, p22p = get_diagonal_outer_prods(c1p, c2p, k.diagonal_spatial)
+    p11, p12, p22 = get_diagonal_outer_prods(k.cov1, k.cov2,
+                                             k.diagonal_spatial)
+    nngp, ntk = gelu_nngp_ntk(k.nngp, p12, p12p, k.ntk)
+    if k.diagonal_spatial:                    # 250-257, 261-264
+      def diag(x):
+        sq = np.sqrt(1 + 2 * x)
+        out = x / ((x + 1) * sq) + np.arctan2(x, sq) / 2
+        return (out / dt(math.pi) + dt(0.25)) * x
+      cov1 = diag(k.cov1)
+      cov2 = None if k.cov2 is None else diag(k.cov2)
+    else:
+      cov1, _ = gelu_nngp_ntk(k.cov1, p11, p11p, None)
+      cov2 = None if k.cov2 is None else gelu_nngp_ntk(k.cov2, p22, p22p,
+                                                       None)[0]
+    return k.replace(cov1=cov1, nngp=nngp, cov2=cov2, ntk=ntk,
+                     is_gaussian=False)
+  return kernel_fn
+
+
 # --------------------------------------------------------------------- Pool
 
 def _window_sum_axis(x, axis, window, stride, padding):

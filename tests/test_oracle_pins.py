@@ -10,6 +10,7 @@ import math
 import numpy as np
 import pytest
 import scipy.integrate
+import scipy.special
 import scipy.stats
 
 from conftest import rel_err
@@ -361,3 +362,86 @@ def test_fast_arccos_polynomial_bound():
   err = np.abs(th - np.arccos(a.astype(np.float64)))
   assert err.max() <= 9e-7, err.max()
   assert th[-1] == 0.0                     # arccos(1) = 0 exactly
+
+
+def _gauss2(q1, q2, c, g, n=96):
+  """E[g(u, v)] for (u, v) ~ N(0, [[q1, c], [c, q2]]) by tensor
+  Gauss-Hermite quadrature (probabilists' weights)."""
+  z, w = np.polynomial.hermite_e.hermegauss(n)
+  w = w / np.sqrt(2 * np.pi)
+  z1, z2 = np.meshgrid(z, z, indexing='ij')
+  rho = c / np.sqrt(q1 * q2)
+  u = np.sqrt(q1) * z1
+  v = np.sqrt(q2) * (rho * z1 + np.sqrt(max(1 - rho ** 2, 0.0)) * z2)
+  return float(np.sum(w[:, None] * w[None, :] * g(u, v)))
+
+
+_ACT_POINTS = [(1.0, 1.0, 0.0), (0.7, 1.3, 0.5), (2.0, 0.5, -0.9),
+               (0.3, 0.4, 0.34), (1.5, 1.5, 1.499)]
+
+
+@pytest.mark.parametrize('q1,q2,c', _ACT_POINTS)
+def test_erf_map_matches_gaussian_integrals(q1, q2, c):
+  """Williams' arcsine kernel: E[erf(u) erf(v)] and E[erf'(u) erf'(v)]."""
+  nngp, ntk = so.erf_nngp_ntk(np.array([c]), np.array([(1 + 2 * q1) *
+                                                       (1 + 2 * q2)]),
+                              np.array([1.0]))
+  e = _gauss2(q1, q2, c, lambda u, v: scipy.special.erf(u) *
+              scipy.special.erf(v))
+  d = _gauss2(q1, q2, c, lambda u, v: 4 / np.pi * np.exp(-u ** 2 - v ** 2))
+  assert abs(nngp[0] - e) <= 1e-10, (nngp[0], e)
+  assert abs(ntk[0] - d) <= 1e-10, (ntk[0], d)
+
+
+@pytest.mark.parametrize('q1,q2,c', _ACT_POINTS)
+def test_gelu_map_matches_gaussian_integrals(q1, q2, c):
+  """E[gelu(u) gelu(v)] and E[gelu'(u) gelu'(v)], gelu(x) = x Phi(x)."""
+  Phi = scipy.stats.norm.cdf
+  phi = scipy.stats.norm.pdf
+  nngp, ntk = so.gelu_nngp_ntk(np.array([c]), np.array([q1 * q2]),
+                               np.array([(q1 + 1) * (q2 + 1)]),
+                               np.array([1.0]))
+  e = _gauss2(q1, q2, c, lambda u, v: u * Phi(u) * v * Phi(v))
+  d = _gauss2(q1, q2, c, lambda u, v: (Phi(u) + u * phi(u)) *
+              (Phi(v) + v * phi(v)))
+  assert abs(nngp[0] - e) <= 1e-9, (nngp[0], e)
+  assert abs(ntk[0] - d) <= 1e-9, (ntk[0], d)
+
+
+def test_erf_gelu_layers_scales_and_diagonal_forms():
+  """Erf(a, b, c) = a^2 map(b^2 k) + c^2 on nngp/cov (ntk: a^2 b^2 dsigma);
+  the diagonal_spatial self forms equal the general map at c = q1 = q2."""
+  rng = np.random.default_rng(3)
+  x1 = rng.standard_normal((2, 3, 3, 2))
+  x2 = rng.standard_normal((2, 3, 3, 2))
+  conv = so.conv_layer((3, 3), (1, 1), 'SAME', 1.3, 0.2)
+  base = so.kernel_fn([conv], x1, x2, ('nngp', 'ntk', 'cov1', 'cov2'))
+  a, b, c = 0.7, 1.6, 0.3
+  got = so.kernel_fn([conv, so.erf_layer(a, b, c)], x1, x2,
+                     ('nngp', 'ntk', 'cov1'))
+  q1 = np.einsum('ihhww->ihw', base['cov1'])
+  q2 = np.einsum('ihhww->ihw', base['cov2'])
+  # entry (i, j, h, h', w, w'): q1[i, h, w], q2[j, h', w']
+  P = (1 + 2 * b * b * q1[:, None, :, None, :, None]) * (
+      1 + 2 * b * b * q2[None, :, None, :, None, :])
+  n_ref, t_ref = so.erf_nngp_ntk(b * b * base['nngp'], P,
+                                 b * b * base['ntk'])
+  np.testing.assert_allclose(got['nngp'], a * a * n_ref + c * c, rtol=1e-12)
+  np.testing.assert_allclose(got['ntk'], a * a * t_ref, rtol=1e-12)
+  # the two layouts differ by the conv's spatial-axis reversal: take the
+  # orientation from the conv alone
+  full0 = so.kernel_fn([conv], x1, None, ('cov1',))['cov1']
+  diag0 = so.kernel_fn([conv], x1, None, ('cov1',),
+                       diagonal_spatial=True)['cov1']
+  ext = np.einsum('ihhww->ihw', full0)
+  flip = not np.allclose(ext, diag0, rtol=1e-12)
+  if flip:
+    np.testing.assert_allclose(ext.transpose(0, 2, 1), diag0, rtol=1e-12)
+  for layer, fn in ((so.erf_layer(1.0, 1.0, 0.0), 'erf'),
+                    (so.gelu_layer(), 'gelu')):
+    full = so.kernel_fn([conv, layer], x1, None, ('cov1',))['cov1']
+    diag = so.kernel_fn([conv, layer], x1, None, ('cov1',),
+                        diagonal_spatial=True)['cov1']
+    ext = np.einsum('ihhww->ihw', full)
+    np.testing.assert_allclose(ext.transpose(0, 2, 1) if flip else ext,
+                               diag, rtol=1e-12, err_msg=fn)

@@ -140,3 +140,27 @@ def test_first_two_stage_fusion_selection():
   p = _engine.compile_plan(stax.myrtle(5)[2].specs, (1, 32, 32, 3))
   assert _engine._per_pair_bytes(p, True, True) == 4 * 16 ** 4 * 2 * 2
   assert _engine._per_pair_bytes(p, True) == 4 * 32 ** 4 * 2 * 2
+
+
+def test_erf_gelu_stage_constants():
+  """Erf(a, b, c): flags carry the activation, (k1, k0) = (a^2, c^2) and
+  b^2 multiplies the preceding conv's W_std^2 and b_std^2; Gelu has no
+  constants; neither is fused; b != 1 without a Conv right before raises."""
+  from neural_tangents_amd import _lib
+  p = _plan(stax.Conv(1, (3, 3), padding='SAME', W_std=1.5, b_std=0.5),
+            stax.Erf(0.8, 2.0, 0.3), stax.AvgPool((2, 2), (2, 2)),
+            stax.Conv(1, (3, 3), padding='SAME'), stax.Gelu(),
+            stax.GlobalAvgPool())
+  s0, s1 = p.stages[0], p.stages[2]        # stages[1]: the first pool
+  assert s0.act_kind == _lib.ACT_ERF and s1.act_kind == _lib.ACT_GELU
+  assert s0.act == pytest.approx((0.64, 0.09))
+  assert s0.conv[3] == pytest.approx(1.5 ** 2 * 4.0)
+  assert s0.conv[4] == pytest.approx(0.5 ** 2 * 4.0)
+  assert s0.flags(True) & _lib.ACT_ERF and s1.flags(False) & _lib.ACT_GELU
+  assert not _engine._fuses_first_two(p, (1, 32, 32, 3))
+  with pytest.raises(NotImplementedError):
+    _plan(stax.Conv(1, (3, 3), padding='SAME'), stax.AvgPool((2, 2), (2, 2)),
+          stax.Erf(1.0, 2.0), stax.GlobalAvgPool())
+  with pytest.raises(ValueError):
+    _plan(stax.Conv(1, (3, 3), padding='SAME'), stax.Gelu(), stax.Erf(),
+          stax.GlobalAvgPool())
