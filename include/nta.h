@@ -41,6 +41,7 @@ extern "C" {
 #define NTA_CONV 4        /* apply the 3x3-type stencil (else identity)     */
 #define NTA_CONV_VALID 8  /* VALID padding for the stencil (default SAME)   */
 #define NTA_FLATTEN 16    /* head: Flatten (diagonal trace) instead of GAP  */
+#define NTA_CONV_CIRCULAR 128  /* periodic (wrap-around) padding, SAME size */
 /* Activation of a stage with NTA_RELU set (default: ABRelu with the
  * act_k1 = (a-b)^2, act_k0 = ab constants of the entry point): */
 #define NTA_ACT_ERF 32    /* a Erf(x) + c: act_k1 = a^2, act_k0 = c^2 (the  */

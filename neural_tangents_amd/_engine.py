@@ -36,6 +36,7 @@ class Stage:
   relu: bool = False
   act: Tuple[float, float] = (1.0, 0.0)   # ABRelu (k1, k0) = ((a-b)^2, ab)
   act_kind: int = 0                # 0 ABRelu, _lib.ACT_ERF, _lib.ACT_GELU
+  circular: bool = False           # CIRCULAR (periodic) conv padding
   wscale: float = 1.0              # SumPool factor folded into conv W_std^2
   pool: Optional[tuple] = None     # (ph, pw)
   first: bool = False
@@ -53,6 +54,8 @@ class Stage:
       f |= _lib.CONV
       if self.conv[2]:
         f |= _lib.CONV_VALID
+      if self.circular:
+        f |= _lib.CONV_CIRCULAR
     if self.relu:
       f |= _lib.RELU | self.act_kind
     if ntk:
@@ -128,7 +131,7 @@ def compile_plan(specs, in_shape) -> Plan:
       if any(s != 1 for s in spec.strides):
         raise NotImplementedError('Conv strides other than 1 are not '
                                   'supported by the HIP engine.')
-      if spec.padding not in ('SAME', 'VALID'):
+      if spec.padding not in ('SAME', 'VALID', 'CIRCULAR'):
         raise NotImplementedError(f'Conv padding {spec.padding!r} is not '
                                   'supported by the HIP engine.')
       if len(spec.filter_shape) != 2:
@@ -142,6 +145,7 @@ def compile_plan(specs, in_shape) -> Plan:
       pend = 1.0
       b_var = 0.0 if spec.b_std is None else float(spec.b_std) ** 2
       cur = Stage(conv=(fh, fw, spec.padding == 'VALID', w_var, b_var),
+                  circular=spec.padding == 'CIRCULAR',
                   first=not stages and n_convs == 0,
                   wscale=w_var / float(spec.W_std) ** 2
                   if spec.W_std != 0 else 1.0)
@@ -413,7 +417,8 @@ def _fuses_first_two(plan: Plan, in_shape) -> bool:
   _, H, W, C = in_shape
   s0, s1 = plan.stages[0], plan.stages[1]
   def conv33(s):
-    return s.conv is not None and s.conv[:3] == (3, 3, False)
+    return (s.conv is not None and s.conv[:3] == (3, 3, False)
+            and not s.circular)
   return (H == 32 and W == 32 and 1 <= C <= 4 and s0.first and conv33(s0)
           and s0.relu and s0.act == (1.0, 0.0) and s0.act_kind == 0
           and s0.pool is None
@@ -483,6 +488,7 @@ def _diag_tables(plan: Plan):
     if s.conv is not None:
       fh, fw, valid, wv, bv = s.conv
       f |= _lib.CONV | (_lib.CONV_VALID if valid else 0)
+      f |= _lib.CONV_CIRCULAR if s.circular else 0
     if s.relu:
       f |= _lib.RELU | s.act_kind
     desc[3 * k:3 * k + 3] = [f, fh, fw]

@@ -21,6 +21,7 @@ CONV_VALID = 8
 FLATTEN = 16
 ACT_ERF = 32
 ACT_GELU = 64
+CONV_CIRCULAR = 128
 
 _lock = threading.Lock()
 _lib = None

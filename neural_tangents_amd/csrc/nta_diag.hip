@@ -19,7 +19,7 @@ constexpr int kDiagMaxStages = 48;    // the stage table travels in kernargs
 constexpr int kDiagMaxHW = 3840;       // 4 planes + reduction <= 64 KiB LDS
 
 struct DiagStage {
-  int conv, fh, fw, lo_h, lo_w, hin, win, hout, wout, relu, act;
+  int conv, fh, fw, lo_h, lo_w, hin, win, hout, wout, relu, act, circ;
   float w_var, b_var, k1, k0;
 };
 struct DiagPlan {
@@ -94,10 +94,12 @@ __global__ __launch_bounds__(256) void k_diag_chain(
         if (S.conv) {
           float an = 0.0f, at = 0.0f;
           for (int a = 0; a < S.fh; ++a) {
-            const int hh = h + a - S.lo_h;
+            int hh = h + a - S.lo_h;
+            if (S.circ) hh = ((hh % S.hin) + S.hin) % S.hin;
             if ((unsigned)hh >= (unsigned)S.hin) continue;
             for (int b = 0; b < S.fw; ++b) {
-              const int ww = w + b - S.lo_w;
+              int ww = w + b - S.lo_w;
+              if (S.circ) ww = ((ww % S.win) + S.win) % S.win;
               if ((unsigned)ww >= (unsigned)S.win) continue;
               an += sn[cur][hh * S.win + ww];
               if (ntk && have_t) at += st[cur][hh * S.win + ww];
@@ -219,6 +221,7 @@ extern "C" int nta_diag_chain(const float* x1, const float* x2, int n1,
     S.conv = (f & NTA_CONV) ? 1 : 0;
     S.relu = (f & NTA_RELU) ? 1 : 0;
     S.act = f & (NTA_ACT_ERF | NTA_ACT_GELU);
+    S.circ = (S.conv && (f & NTA_CONV_CIRCULAR)) ? 1 : 0;
     S.fh = S.conv ? stage_desc[3 * s + 1] : 1;
     S.fw = S.conv ? stage_desc[3 * s + 2] : 1;
     S.w_var = stage_wb[4 * s];

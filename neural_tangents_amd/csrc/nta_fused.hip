@@ -1159,7 +1159,8 @@ int launch_stage_fast(const float* in_nngp, const float* in_ntk, int n1,
                       float b_var, float k1, float k0, const float* q1,
                       const float* q2, float* out_nngp, float* out_ntk,
                       hipStream_t stream) {
-  if (k1 != 1.0f || k0 != 0.0f || (flags & (NTA_ACT_ERF | NTA_ACT_GELU)))
+  if (k1 != 1.0f || k0 != 0.0f ||
+      (flags & (NTA_ACT_ERF | NTA_ACT_GELU | NTA_CONV_CIRCULAR)))
     return kNotHandled;
   const bool ntk = flags & NTA_NTK;
   const float wv9 = w_var / 9.0f;
@@ -1195,7 +1196,8 @@ int launch_first_fast(const float* x1, const float* x2, int n1, int n2, int S,
                       int C, int flags, float w_var, float b_var, float k1,
                       float k0, const float* q1, const float* q2,
                       float* nngp, float* ntk, hipStream_t stream) {
-  if (k1 != 1.0f || k0 != 0.0f || (flags & (NTA_ACT_ERF | NTA_ACT_GELU)))
+  if (k1 != 1.0f || k0 != 0.0f ||
+      (flags & (NTA_ACT_ERF | NTA_ACT_GELU | NTA_CONV_CIRCULAR)))
     return kNotHandled;
   if (C > 4 || (S != 32 && S != 16)) return kNotHandled;
   const long long T = n2 == 0 ? (long long)n1 : (long long)n1 * n2;

@@ -42,7 +42,17 @@ struct StencilGeom {
   int fh, fw;    // filter
   int oh, ow;    // input offset of tap 0 (SAME: -(f-1)/2, VALID: 0)
   int Hc, Wc;    // conv output size
+  int circ;      // CIRCULAR: taps wrap around instead of reading zeros
 };
+
+// Input index of a tap, or -1 for a zero (padding) tap.
+__device__ __forceinline__ int tap_index(int v, int n, int circ) {
+  if (circ) {
+    v %= n;
+    return v < 0 ? v + n : v;
+  }
+  return (unsigned)v < (unsigned)n ? v : -1;
+}
 
 static StencilGeom make_geom(int H, int W, int fh, int fw, int flags) {
   StencilGeom g;
@@ -59,6 +69,7 @@ static StencilGeom make_geom(int H, int W, int fh, int fw, int flags) {
     g.Hc = H;
     g.Wc = W;
   }
+  g.circ = (flags & NTA_CONV_CIRCULAR) ? 1 : 0;
   return g;
 }
 
@@ -76,11 +87,11 @@ __global__ void k_first_diag(const float* __restrict__ x, int n, int H, int W,
     const float* xi = x + (uint64_t)img * H * W * C;
     float acc = 0.f;
     for (int k = 0; k < g.fh; ++k) {
-      int hh = h + k + g.oh;
-      if (hh < 0 || hh >= H) continue;
+      int hh = tap_index(h + k + g.oh, H, g.circ);
+      if (hh < 0) continue;
       for (int l = 0; l < g.fw; ++l) {
-        int ww = w + l + g.ow;
-        if (ww < 0 || ww >= W) continue;
+        int ww = tap_index(w + l + g.ow, W, g.circ);
+        if (ww < 0) continue;
         const float* px = xi + ((uint64_t)hh * W + ww) * C;
         for (int c = 0; c < C; ++c) acc = fmaf(px[c], px[c], acc);
       }
@@ -116,11 +127,13 @@ __global__ void k_first_stage(const float* __restrict__ x1,
     const float* xb = (self ? x1 : x2) + (uint64_t)j * H * W * C;
     float acc = 0.f;
     for (int k = 0; k < g.fh; ++k) {
-      int ha = h1 + k + g.oh, hb = h2 + k + g.oh;
-      if (ha < 0 || ha >= H || hb < 0 || hb >= H) continue;
+      int ha = tap_index(h1 + k + g.oh, H, g.circ);
+      int hb = tap_index(h2 + k + g.oh, H, g.circ);
+      if (ha < 0 || hb < 0) continue;
       for (int l = 0; l < g.fw; ++l) {
-        int wa = w1 + l + g.ow, wb = w2 + l + g.ow;
-        if (wa < 0 || wa >= W || wb < 0 || wb >= W) continue;
+        int wa = tap_index(w1 + l + g.ow, W, g.circ);
+        int wb = tap_index(w2 + l + g.ow, W, g.circ);
+        if (wa < 0 || wb < 0) continue;
         const float* pa = xa + ((uint64_t)ha * W + wa) * C;
         const float* pb = xb + ((uint64_t)hb * W + wb) * C;
         for (int c = 0; c < C; ++c) acc = fmaf(pa[c], pb[c], acc);
@@ -156,11 +169,11 @@ __global__ void k_stage_diag(const float* __restrict__ R, int n, int H, int W,
     if (conv) {
       float acc = 0.f;
       for (int k = 0; k < g.fh; ++k) {
-        int hh = h + k + g.oh;
-        if (hh < 0 || hh >= H) continue;
+        int hh = tap_index(h + k + g.oh, H, g.circ);
+        if (hh < 0) continue;
         for (int l = 0; l < g.fw; ++l) {
-          int ww = w + l + g.ow;
-          if (ww < 0 || ww >= W) continue;
+          int ww = tap_index(w + l + g.ow, W, g.circ);
+          if (ww < 0) continue;
           acc += r[(((uint64_t)hh * H + hh) * W + ww) * W + ww];
         }
       }
@@ -214,11 +227,13 @@ __global__ void k_stage(const float* __restrict__ in_nngp,
             if (conv) {
               float sv = 0.f, stt = 0.f;
               for (int k = 0; k < g.fh; ++k) {
-                int ha = h1 + k + g.oh, hb = h2 + k + g.oh;
-                if (ha < 0 || ha >= H || hb < 0 || hb >= H) continue;
+                int ha = tap_index(h1 + k + g.oh, H, g.circ);
+                int hb = tap_index(h2 + k + g.oh, H, g.circ);
+                if (ha < 0 || hb < 0) continue;
                 for (int l = 0; l < g.fw; ++l) {
-                  int wa = w1 + l + g.ow, wb = w2 + l + g.ow;
-                  if (wa < 0 || wa >= W || wb < 0 || wb >= W) continue;
+                  int wa = tap_index(w1 + l + g.ow, W, g.circ);
+                  int wb = tap_index(w2 + l + g.ow, W, g.circ);
+                  if (wa < 0 || wb < 0) continue;
                   uint64_t o = (((uint64_t)ha * H + hb) * W + wa) * W + wb;
                   sv += sn[o];
                   if (want_ntk) stt += st[o];
@@ -316,6 +331,9 @@ static int check_filter(int H, int W, int fh, int fw, int flags,
   if (fh <= 0 || fw <= 0)
     return set_error(NTA_EINVAL, "%s: filter sizes must be positive, got "
                      "%dx%d", who, fh, fw);
+  if ((flags & NTA_CONV_VALID) && (flags & NTA_CONV_CIRCULAR))
+    return set_error(NTA_EINVAL, "%s: VALID and CIRCULAR padding both set",
+                     who);
   if ((flags & NTA_CONV_VALID) && (fh > H || fw > W))
     return set_error(NTA_EINVAL, "%s: VALID %dx%d filter larger than %dx%d "
                      "input", who, fh, fw, H, W);

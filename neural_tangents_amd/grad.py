@@ -103,7 +103,7 @@ def _check_plan(plan):
                                 'Relu [+ AvgPool] only.')
     if s.hin != s.win:
       raise NotImplementedError('kernel gradients need square images.')
-    if s.conv[2]:
+    if s.conv[2] or s.circular:
       raise NotImplementedError('kernel gradients need SAME convolutions.')
 
 

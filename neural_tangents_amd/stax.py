@@ -127,7 +127,7 @@ def _conv_layer_fns(spec: _ConvSpec):
     b = None if spec.b_std is None else torch.randn((spec.out_chan,),
                                                     generator=g,
                                                     dtype=torch.float64)
-    if spec.padding == 'SAME':
+    if spec.padding in ('SAME', 'CIRCULAR'):
       ho, wo = -(-h // spec.strides[0]), -(-w // spec.strides[1])
     else:
       ho = (h - fh) // spec.strides[0] + 1
@@ -138,10 +138,11 @@ def _conv_layer_fns(spec: _ConvSpec):
     W, b = params
     x = inputs.permute(0, 3, 1, 2)
     W = W.to(x.device, x.dtype)
-    if spec.padding == 'SAME':
+    if spec.padding in ('SAME', 'CIRCULAR'):
       ph = _same_pads(x.shape[2], fh, spec.strides[0])
       pw = _same_pads(x.shape[3], fw, spec.strides[1])
-      x = F.pad(x, (pw[0], pw[1], ph[0], ph[1]))
+      x = F.pad(x, (pw[0], pw[1], ph[0], ph[1]),
+                mode='circular' if spec.padding == 'CIRCULAR' else 'constant')
     fan_in = fh * fw * W.shape[2]
     y = F.conv2d(x, W.permute(3, 2, 0, 1), stride=tuple(spec.strides))
     y = y * (spec.W_std / math.sqrt(fan_in))

@@ -168,13 +168,17 @@ def _diag_conv_last2(x, f, stride, padding):
   ``out[.., p, p'] = (1/f) sum_k xpad[.., p*s + k, p'*s + k]``."""
   n = x.shape[-1]
   assert x.shape[-2] == n
-  if padding == 'SAME':
+  if padding in ('SAME', 'CIRCULAR'):
     lo, hi = same_pads(n, f, stride)
   elif padding == 'VALID':
     lo = hi = 0
   else:
     raise NotImplementedError(padding)
-  xp = np.pad(x, [(0, 0)] * (x.ndim - 2) + [(lo, hi), (lo, hi)])
+  # CIRCULAR: the SAME pads filled by wrapping (``_same_pad_for_filter_shape``
+  # with mode 'wrap', ``linear.py:2977-3021``, applied at 3158-3164), then a
+  # VALID pass.
+  xp = np.pad(x, [(0, 0)] * (x.ndim - 2) + [(lo, hi), (lo, hi)],
+              mode='wrap' if padding == 'CIRCULAR' else 'constant')
   np_ = n + lo + hi
   out_n = (np_ - f) // stride + 1
   acc = np.zeros(x.shape[:-2] + (out_n, out_n), x.dtype)
@@ -209,6 +213,12 @@ def conv_kernel_diagonal_spatial(lhs, filter_shape, strides, padding,
     return lhs
   window = (1,) * batch_ndim + tuple(filter_shape)
   st = (1,) * batch_ndim + tuple(strides)
+  if padding == 'CIRCULAR':                 # 3422-3426
+    pads = [(0, 0)] * batch_ndim + [
+        same_pads(lhs.shape[batch_ndim + a], filter_shape[a], strides[a])
+        for a in range(len(filter_shape))]
+    lhs = np.pad(lhs, pads, mode='wrap')
+    padding = 'VALID'
   out = _reduce_window_sum(lhs, window, st, padding)
   return out / lhs.dtype.type(np.prod(filter_shape))
 

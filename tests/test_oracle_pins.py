@@ -24,7 +24,8 @@ def _brute_conv(nngp, fh, fw, padding):
   """Direct 4-D loop: out[h,h',w,w'] = mean over taps (k, l) of
   in[h+k+o, h'+k+o, w+l+o, w'+l+o], zero outside; natural axis order."""
   n1, n2, H, _, W, _ = nngp.shape
-  if padding == 'SAME':
+  circ = padding == 'CIRCULAR'               # taps wrap around
+  if padding in ('SAME', 'CIRCULAR'):
     oh, ow, Hc, Wc = -((fh - 1) // 2), -((fw - 1) // 2), H, W
   else:
     oh, ow, Hc, Wc = 0, 0, H - fh + 1, W - fw + 1
@@ -36,10 +37,14 @@ def _brute_conv(nngp, fh, fw, padding):
           acc = 0.0
           for k in range(fh):
             a, ap = h + k + oh, hp + k + oh
+            if circ:
+              a, ap = a % H, ap % H
             if not (0 <= a < H and 0 <= ap < H):
               continue
             for l in range(fw):
               b, bp = w + l + ow, wp + l + ow
+              if circ:
+                b, bp = b % W, bp % W
               if not (0 <= b < W and 0 <= bp < W):
                 continue
               acc = acc + nngp[:, :, a, ap, b, bp]
@@ -48,7 +53,10 @@ def _brute_conv(nngp, fh, fw, padding):
 
 
 @pytest.mark.parametrize('fh,fw,padding', [(3, 3, 'SAME'), (2, 3, 'SAME'),
-                                           (3, 2, 'VALID'), (1, 3, 'SAME')])
+                                           (3, 2, 'VALID'), (1, 3, 'SAME'),
+                                           (3, 3, 'CIRCULAR'),
+                                           (2, 3, 'CIRCULAR'),
+                                           (5, 3, 'CIRCULAR')])
 def test_conv_stencil_bruteforce(fh, fw, padding):
   rng = np.random.default_rng(0)
   x = rng.standard_normal((2, 3, 5, 5, 4, 4))
@@ -445,3 +453,19 @@ def test_erf_gelu_layers_scales_and_diagonal_forms():
     ext = np.einsum('ihhww->ihw', full)
     np.testing.assert_allclose(ext.transpose(0, 2, 1) if flip else ext,
                                diag, rtol=1e-12, err_msg=fn)
+
+
+def test_circular_full_vs_diagonal_spatial():
+  """A Flatten-readout network with CIRCULAR convs: the spatial diagonal of
+  the full-spatial kernel equals the diagonal-spatial computation (a check
+  of the wrap-padded window sums against the wrap-padded 4-D stencil)."""
+  rng = np.random.default_rng(5)
+  x1 = rng.standard_normal((2, 5, 4, 2))
+  x2 = rng.standard_normal((3, 5, 4, 2))
+  layers = [so.conv_layer((3, 3), (1, 1), 'CIRCULAR', 1.2, 0.1),
+            so.relu_layer(), so.conv_layer((3, 2), (1, 1), 'CIRCULAR'),
+            so.relu_layer(), so.flatten_layer()]
+  full = so.kernel_fn(layers, x1, x2, ('nngp', 'ntk'))
+  diag = so.kernel_fn(layers, x1, x2, ('nngp', 'ntk'), diagonal_spatial=True)
+  for g in ('nngp', 'ntk'):
+    np.testing.assert_allclose(full[g], diag[g], rtol=1e-12)

@@ -47,7 +47,8 @@ def test_dense_chain_coefficients():
     ((stax.Relu(),), ValueError),
     ((stax.AvgPool((2, 2), (2, 2)),), NotImplementedError),
     ((stax.Conv(1, (3, 3), (2, 2), 'SAME'),), NotImplementedError),
-    ((stax.Conv(1, (3, 3), padding='CIRCULAR'),), NotImplementedError),
+    ((stax.Conv(1, (3, 3), (1, 2), padding='CIRCULAR'),),
+     NotImplementedError),
     ((stax.Conv(1, (3, 3), padding='SAME'), stax.AvgPool((3, 3), (2, 2))),
      NotImplementedError),
     ((stax.Conv(1, (3, 3), padding='SAME'), stax.GlobalAvgPool(),
@@ -164,3 +165,14 @@ def test_erf_gelu_stage_constants():
   with pytest.raises(ValueError):
     _plan(stax.Conv(1, (3, 3), padding='SAME'), stax.Gelu(), stax.Erf(),
           stax.GlobalAvgPool())
+
+
+def test_circular_padding_flags_and_no_fusion():
+  from neural_tangents_amd import _lib
+  conv = lambda: stax.Conv(1, (3, 3), padding='CIRCULAR')
+  p = _plan(conv(), stax.Relu(), conv(), stax.Relu(),
+            stax.AvgPool((2, 2), (2, 2)), stax.GlobalAvgPool())
+  assert all(s.circular for s in p.stages if s.conv is not None)
+  assert p.stages[0].flags(True) & _lib.CONV_CIRCULAR
+  assert not p.stages[0].flags(True) & _lib.CONV_VALID
+  assert not _engine._fuses_first_two(p, (1, 32, 32, 3))
