@@ -107,9 +107,20 @@ def krr_bench(args):
   dev = torch.device('cuda', 0)
   kernel_fn = stax.myrtle(5)[2]
   t0 = time.perf_counter()
-  k_dd = kernel_fn(torch.from_numpy(xtr).to(dev), None, 'ntk')
-  k_td = kernel_fn(torch.from_numpy(xte).to(dev),
-                   torch.from_numpy(xtr).to(dev), 'ntk')
+  xtr_d = torch.from_numpy(xtr).to(dev)
+  # K_dd by row blocks on and above the block diagonal (mirrored), so that
+  # progress is printed every block.
+  blk = 2048
+  k_dd = torch.empty((ntr, ntr), dtype=torch.float32, device=dev)
+  for i0 in range(0, ntr, blk):
+    i1 = min(ntr, i0 + blk)
+    k_dd[i0:i1, i0:] = kernel_fn(xtr_d[i0:i1], xtr_d[i0:], 'ntk')
+    k_dd[i0:, i0:i1] = k_dd[i0:i1, i0:].T
+    torch.cuda.synchronize()
+    print(f'K_dd rows {i0}-{i1}: {time.perf_counter() - t0:.1f} s',
+          file=sys.stderr, flush=True)
+  k_dd = 0.5 * (k_dd + k_dd.T)        # exactly symmetric
+  k_td = kernel_fn(torch.from_numpy(xte).to(dev), xtr_d, 'ntk')
   torch.cuda.synchronize()
   t_kernel = time.perf_counter() - t0
   print(f'kernels: {t_kernel:.1f} s', file=sys.stderr, flush=True)
