@@ -758,7 +758,9 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
   const int wq = 16 * (wave & 1) + m16;
   const int wr = (w0 + 1) * kLS2 + wq + 1;
 
-  for (int k = t; k < 6 * kPL2; k += 256) lp[k] = 0.0f;
+  static_assert((6 * kPL2) % 4 == 0, "exchange planes in float4 units");
+  for (int k = t; k < 6 * kPL2 / 4; k += 256)
+    reinterpret_cast<float4*>(lp)[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (t < 2) smax[t] = 0u;
   const float* xi = x1 + i * (S * S * C);
   const float* xj = (self ? x1 : x2) + j * (S * S * C);
@@ -791,19 +793,28 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
   // accumulator.
   const float scale_g = ldexpf(scale1, ei + ej);
   {
-    const float si = ldexpf(1.0f, -ei), sj = ldexpf(1.0f, -ej);
-    for (int k = t; k < 2 * kXS * kXS * 4; k += 256) {
-      const int img = k / (kXS * kXS * 4);
-      const int r = k - img * (kXS * kXS * 4);
-      const int pos = r >> 2, c = r & 3;
+    // One padded position (4 channels: one 8-byte hi and one lo store) per
+    // thread and pass.
+    for (int k = t; k < 2 * kXS * kXS; k += 256) {
+      const int img = k >= kXS * kXS;
+      const int pos = k - img * (kXS * kXS);
       const int hh = pos / kXS - 1;
       const int ww = pos - (pos / kXS) * kXS - 1;
-      float v = 0.0f;
-      if (c < C && (unsigned)hh < (unsigned)S && (unsigned)ww < (unsigned)S)
-        v = img ? xj[(hh * S + ww) * C + c] * sj : xi[(hh * S + ww) * C + c] * si;
-      const _Float16 hv = (_Float16)v;
-      xh[(img * 2) * (kXS * kXS * 4) + r] = hv;
-      xh[(img * 2 + 1) * (kXS * kXS * 4) + r] = (_Float16)(v - (float)hv);
+      const float sc = ldexpf(1.0f, img ? -ej : -ei);
+      const float* src = (img ? xj : xi) + (hh * S + ww) * C;
+      const bool in = (unsigned)hh < (unsigned)S && (unsigned)ww < (unsigned)S;
+      half4v hv4, lv4;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float v = (in && c < C) ? src[c] * sc : 0.0f;
+        const _Float16 hv = (_Float16)v;
+        hv4[c] = hv;
+        lv4[c] = (_Float16)(v - (float)hv);
+      }
+      *reinterpret_cast<half4v*>(xh + ((img * 2) * (kXS * kXS) + pos) * 4) =
+          hv4;
+      *reinterpret_cast<half4v*>(xh + ((img * 2 + 1) * (kXS * kXS) + pos) *
+                                          4) = lv4;
     }
   }
   __syncthreads();
