@@ -674,9 +674,11 @@ __global__ __launch_bounds__(256) void k_first16(
 
 // Conv(3x3 SAME) -> ReLU -> Conv(3x3 SAME) -> ReLU -> AvgPool(2x2) on 32x32
 // images, with the first stage's state never written to memory.  The
-// decomposition is k_stage_fast's (one workgroup per pooled diagonal
-// H - H' = D of one pair, sweeping H); the six first-stage planes a step
-// consumes are computed in place from the two images held in LDS:
+// decomposition is k_stage_fast's (a sweep in H along each pooled diagonal
+// H - H' = D of one pair), with two diagonals of 16 steps in all per
+// workgroup (D = s and s - 16) so that the images are prepared once per
+// 16 steps; the six first-stage planes a step consumes are computed in
+// place from the two images held in LDS:
 //   plane (h, h') = ReLU(scale1 * P_i(h) P_j(h')^T + b1),
 // P_n(h) the [32 w][9 taps x 4 channels] patch matrix of image n at row h
 // (the Gram of zero-padded patches equals the input-covariance stencil).
@@ -729,7 +731,7 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
                     const float2* __restrict__ qb1,
                     const float2* __restrict__ qb2, float* __restrict__ out_n,
                     float* __restrict__ out_t) {
-  constexpr int S = 32, H2 = 16, SO = 16, NDIAG = 2 * H2 - 1;
+  constexpr int S = 32, H2 = 16, SO = 16;
   constexpr long long PER_OUT = (long long)SO * SO * SO * SO;
   constexpr int NT = NTK ? 2 : 1;
   // Images i, j as FP16 hi/lo pairs: xh[img][hl][1 + h][1 + w][c] (zero
@@ -739,10 +741,13 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
   __shared__ unsigned smax[2];
   const bool self = n2 == 0;
   const long long T = self ? (long long)n1 : (long long)n1 * n2;
+  // Workgroup (p, slot): slot 0 sweeps diagonal 0, slot s > 0 sweeps s and
+  // then s - 16: 16 steps each, and the images are prepared once per pair
+  // and 16 diagonals instead of once per diagonal.
   const int L = xcd_remap(blockIdx.x, nblk);
-  if ((long long)L >= T * NDIAG) return;   // whole block exits together
-  const long long p = L / NDIAG;
-  const int di = L - (int)(p * NDIAG);
+  if ((long long)L >= T * H2) return;      // whole block exits together
+  const long long p = L / H2;
+  const int slot = L - (int)(p * H2);
   const long long i = self ? p : p / n2;
   const long long j = self ? p : p - i * n2;
   const int t = threadIdx.x;
@@ -890,7 +895,9 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
   };
 
   const float wv2 = wv9 * kInv2Pi;
-  const int D = di - (H2 - 1);
+  const int nsweep = slot == 0 ? 1 : 2;
+  for (int sw = 0; sw < nsweep; ++sw) {
+  const int D = sw == 0 ? slot : slot - H2;
   const int Hs = D > 0 ? D : 0;
   const int He = D < 0 ? H2 - 1 + D : H2 - 1;
   const int d0 = 2 * D, dp = 2 * D + 1, dm = 2 * D - 1;
@@ -1114,6 +1121,7 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
     step(H + 1, B, A);
   }
   if (H <= He) step(H, A, B);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_qnorm(const float* __restrict__ q,
@@ -1264,10 +1272,10 @@ extern "C" int nta_first_pool_stage(
     return set_error(NTA_EINVAL, "nta_first_pool_stage: null pointer for a "
                      "requested operand");
   const long long T = n2 == 0 ? (long long)n1 : (long long)n1 * n2;
-  if (T * 31 > 0x7ffffff0LL)
+  if (T * 16 > 0x7ffffff0LL)
     return set_error(NTA_EINVAL, "nta_first_pool_stage: tile too large "
                      "(%lld pairs)", T);
-  const int nblk = (int)((T * 31 + 7) / 8 * 8);
+  const int nblk = (int)((T * 16 + 7) / 8 * 8);
   const float scale1 = w_var1 / (9.0f * (float)C);
   const float wv9 = w_var2 / 9.0f;
   const float2* a1 = reinterpret_cast<const float2*>(qn1a);
