@@ -675,10 +675,10 @@ __global__ __launch_bounds__(256) void k_first16(
 // Conv(3x3 SAME) -> ReLU -> Conv(3x3 SAME) -> ReLU -> AvgPool(2x2) on 32x32
 // images, with the first stage's state never written to memory.  The
 // decomposition is k_stage_fast's (a sweep in H along each pooled diagonal
-// H - H' = D of one pair), with two diagonals of 16 steps in all per
-// workgroup (D = s and s - 16) so that the images are prepared once per
-// 16 steps; the six first-stage planes a step consumes are computed in
-// place from the two images held in LDS:
+// H - H' = D of one pair), with several diagonals of 32 steps in all per
+// workgroup (f12_diag) so that the images are prepared once per 32 steps;
+// the six first-stage planes a step consumes are computed in place from the
+// two images held in LDS:
 //   plane (h, h') = ReLU(scale1 * P_i(h) P_j(h')^T + b1),
 // P_n(h) the [32 w][9 taps x 4 channels] patch matrix of image n at row h
 // (the Gram of zero-padded patches equals the input-covariance stencil).
@@ -722,6 +722,24 @@ __device__ __forceinline__ void s1_exchange(float* __restrict__ lp,
   for (int k = 0; k < K; ++k) read_diag3<32, kLS2>(lp + k * kPL2, lofs, ws[k]);
 }
 
+// Diagonals per workgroup.  With kF12Slots = 16: slot 0 sweeps D = 0, slot
+// s > 0 sweeps s and s - 16 (16 steps).  With 8: slot 0 sweeps 0, 8, -8,
+// slot s > 0 sweeps s, s - 16, s + 8, s - 8 (32 steps).
+#ifndef NTA_F12_SLOTS
+#define NTA_F12_SLOTS 8
+#endif
+constexpr int kF12Slots = NTA_F12_SLOTS;
+static_assert(kF12Slots == 8 || kF12Slots == 16, "8 or 16 slots per pair");
+__device__ __forceinline__ int f12_nsweep(int slot) {
+  return kF12Slots == 16 ? (slot == 0 ? 1 : 2) : (slot == 0 ? 3 : 4);
+}
+__device__ __forceinline__ int f12_diag(int slot, int sw) {
+  if (kF12Slots == 16) return sw == 0 ? slot : slot - 16;
+  if (slot == 0) return sw == 0 ? 0 : (sw == 1 ? 8 : -8);
+  return sw == 0 ? slot : (sw == 1 ? slot - 16 : (sw == 2 ? slot + 8
+                                                          : slot - 8));
+}
+
 template <bool NTK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
@@ -745,9 +763,9 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
   // then s - 16: 16 steps each, and the images are prepared once per pair
   // and 16 diagonals instead of once per diagonal.
   const int L = xcd_remap(blockIdx.x, nblk);
-  if ((long long)L >= T * H2) return;      // whole block exits together
-  const long long p = L / H2;
-  const int slot = L - (int)(p * H2);
+  if ((long long)L >= T * kF12Slots) return;   // whole block exits together
+  const long long p = L / kF12Slots;
+  const int slot = L - (int)(p * kF12Slots);
   const long long i = self ? p : p / n2;
   const long long j = self ? p : p - i * n2;
   const int t = threadIdx.x;
@@ -895,9 +913,9 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
   };
 
   const float wv2 = wv9 * kInv2Pi;
-  const int nsweep = slot == 0 ? 1 : 2;
+  const int nsweep = f12_nsweep(slot);
   for (int sw = 0; sw < nsweep; ++sw) {
-  const int D = sw == 0 ? slot : slot - H2;
+  const int D = f12_diag(slot, sw);
   const int Hs = D > 0 ? D : 0;
   const int He = D < 0 ? H2 - 1 + D : H2 - 1;
   const int d0 = 2 * D, dp = 2 * D + 1, dm = 2 * D - 1;
@@ -1272,10 +1290,10 @@ extern "C" int nta_first_pool_stage(
     return set_error(NTA_EINVAL, "nta_first_pool_stage: null pointer for a "
                      "requested operand");
   const long long T = n2 == 0 ? (long long)n1 : (long long)n1 * n2;
-  if (T * 16 > 0x7ffffff0LL)
+  if (T * kF12Slots > 0x7ffffff0LL)
     return set_error(NTA_EINVAL, "nta_first_pool_stage: tile too large "
                      "(%lld pairs)", T);
-  const int nblk = (int)((T * 16 + 7) / 8 * 8);
+  const int nblk = (int)((T * kF12Slots + 7) / 8 * 8);
   const float scale1 = w_var1 / (9.0f * (float)C);
   const float wv9 = w_var2 / 9.0f;
   const float2* a1 = reinterpret_cast<const float2*>(qn1a);
