@@ -675,8 +675,8 @@ __global__ __launch_bounds__(256) void k_first16(
 // Conv(3x3 SAME) -> ReLU -> Conv(3x3 SAME) -> ReLU -> AvgPool(2x2) on 32x32
 // images, with the first stage's state never written to memory.  The
 // decomposition is k_stage_fast's (a sweep in H along each pooled diagonal
-// H - H' = D of one pair), with several diagonals of 32 steps in all per
-// workgroup (f12_diag) so that the images are prepared once per 32 steps;
+// H - H' = D of one pair), with several diagonals of 64 steps in all per
+// workgroup (f12_diag) so that the images are prepared once per 64 steps;
 // the six first-stage planes a step consumes are computed in place from the
 // two images held in LDS:
 //   plane (h, h') = ReLU(scale1 * P_i(h) P_j(h')^T + b1),
@@ -722,22 +722,41 @@ __device__ __forceinline__ void s1_exchange(float* __restrict__ lp,
   for (int k = 0; k < K; ++k) read_diag3<32, kLS2>(lp + k * kPL2, lofs, ws[k]);
 }
 
-// Diagonals per workgroup.  With kF12Slots = 16: slot 0 sweeps D = 0, slot
-// s > 0 sweeps s and s - 16 (16 steps).  With 8: slot 0 sweeps 0, 8, -8,
-// slot s > 0 sweeps s, s - 16, s + 8, s - 8 (32 steps).
+// Workgroups ("slots") per pair and the diagonals each sweeps.  With 16:
+// slot 0 sweeps D = 0, slot s > 0 sweeps s and s - 16 (16 steps).  With 8:
+// slot 0 sweeps 0, 8, -8, slot s > 0 sweeps s, s - 16, s + 8, s - 8 (32
+// steps).  With 4 (default; 2 and 1 measure the same): the diagonals of two
+// of those groups (64 steps), so the images are prepared once per 64 steps.
 #ifndef NTA_F12_SLOTS
-#define NTA_F12_SLOTS 8
+#define NTA_F12_SLOTS 4
 #endif
 constexpr int kF12Slots = NTA_F12_SLOTS;
-static_assert(kF12Slots == 8 || kF12Slots == 16, "8 or 16 slots per pair");
-__device__ __forceinline__ int f12_nsweep(int slot) {
-  return kF12Slots == 16 ? (slot == 0 ? 1 : 2) : (slot == 0 ? 3 : 4);
+static_assert(kF12Slots == 1 || kF12Slots == 2 || kF12Slots == 4 ||
+              kF12Slots == 8 || kF12Slots == 16,
+              "1, 2, 4, 8 or 16 slots per pair");
+__device__ __forceinline__ int f12_nsweep8(int slot) {
+  return slot == 0 ? 3 : 4;
 }
-__device__ __forceinline__ int f12_diag(int slot, int sw) {
-  if (kF12Slots == 16) return sw == 0 ? slot : slot - 16;
+__device__ __forceinline__ int f12_diag8(int slot, int sw) {
   if (slot == 0) return sw == 0 ? 0 : (sw == 1 ? 8 : -8);
   return sw == 0 ? slot : (sw == 1 ? slot - 16 : (sw == 2 ? slot + 8
                                                           : slot - 8));
+}
+// With fewer slots, slot t sweeps the diagonals of the 8 / kF12Slots
+// consecutive 8-slot groups starting at t * 8 / kF12Slots.
+__device__ __forceinline__ int f12_nsweep(int slot) {
+  if (kF12Slots == 16) return slot == 0 ? 1 : 2;
+  constexpr int G = 8 / (kF12Slots < 8 ? kF12Slots : 8);
+  int n = 0;
+  for (int g = 0; g < G; ++g) n += f12_nsweep8(slot * G + g);
+  return n;
+}
+__device__ __forceinline__ int f12_diag(int slot, int sw) {
+  if (kF12Slots == 16) return sw == 0 ? slot : slot - 16;
+  constexpr int G = 8 / (kF12Slots < 8 ? kF12Slots : 8);
+  int g = slot * G;
+  while (sw >= f12_nsweep8(g)) sw -= f12_nsweep8(g++);
+  return f12_diag8(g, sw);
 }
 
 template <bool NTK>
