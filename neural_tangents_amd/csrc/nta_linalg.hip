@@ -12,8 +12,9 @@
 //   W22 = node(A22)                       (recursion)
 //   W21 = -W22 T
 // All products run on one FP32 MFMA GEMM kernel (v_mfma_f32_32x32x2f32,
-// 128x128 tiles) that skips the zero blocks of triangular operands; 64x64
-// leaves are factored and inverted by one wavefront.  At the top level the
+// 128x128 or 64x64 tiles) that skips the zero blocks of triangular
+// operands; 128x128 nodes run as one workgroup in LDS (k_leaf128), whose
+// 64x64 halves are factored and inverted by one wavefront.  At the top level the
 // last two products are skipped: W keeps W11 = L11^-1, W22 = L22^-1 on its
 // diagonal blocks and S = L21 below them, and the preconditioner
 // M^-1 = (L L^T)^-1 is applied as two block triangular solves built from
@@ -36,6 +37,7 @@
 
 namespace nta {
 
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 // ---------------------------------------------------------------- GEMM
@@ -260,8 +262,17 @@ __global__ __launch_bounds__(256) void k_gemm(GemmJobs jobs) {
 }
 
 // Jobs of up to 512 x 512 outputs use 64x64 tiles (more workgroups, no
-// MFMA work on empty quadrants); larger ones 128x128.
-static int tile_for(int M, int N) { return (M <= 512 && N <= 512) ? 64 : 128; }
+// MFMA work on empty quadrants); larger ones 128x128 unless that leaves
+// fewer than NTA_CHOL_T64_BELOW workgroups (one 128 tile per CU is
+// resident, so a short grid idles most of the 256 CUs).
+#ifndef NTA_CHOL_T64_BELOW
+#define NTA_CHOL_T64_BELOW 512
+#endif
+static int tile_for(int M, int N) {
+  if (M <= 512 && N <= 512) return 64;
+  const int t128 = ((M + 127) / 128) * ((N + 127) / 128);
+  return t128 < NTA_CHOL_T64_BELOW ? 64 : 128;
+}
 
 static GemmJob make_job(const float* A, long long lda, const float* B,
                         long long ldb, float* C, long long ldc, int M, int N,
@@ -305,200 +316,378 @@ static void launch_gemm(const GemmJob& a, const GemmJob* b, int tile,
     hipLaunchKernelGGL(k_gemm<128>, dim3(grid), dim3(256), 0, st, jobs);
 }
 
-// ---------------------------------------------------------------- leaf
-// Factors the 64x64 diagonal block of A at (k0, k0) (lower triangle read)
-// and writes its inverse factor W = L^-1 (64x64, zeros above the diagonal)
-// at (k0, k0) of W (whose upper-right 32x32 block is left untouched, zero).
-// One wavefront, in 2 x 2 blocks of 32:
+// ---------------------------------------------------------------- leaves
+// A 64x64 diagonal block is factored and inverted in LDS by one wavefront,
+// in 2 x 2 blocks of 32:
 //   1. the 64 x 32 panel [A11; A21] is factored in registers (lane i holds
-//      row i; column j's scaled entries are broadcast with readlane), giving
-//      L11 (lanes 0-31) and L21 (lanes 32-63) together;
+//      row i), giving L11 (lanes 0-31) and L21 (lanes 32-63) together;
 //   2. A22 -= L21 L21^T on v_mfma_f32_32x32x2f32;
 //   3. A22 is factored the same way (lanes 0-31);
-//   4. L11^-1 (lanes 0-31) and L22^-1 (lanes 32-63) by forward
-//      substitution, one column per lane;
+//   4. L11^-1 and L22^-1: their 16x16 diagonal blocks by forward
+//      substitution (one column of one block per lane), the blocks below
+//      them by two 16x16 MFMA products;
 //   5. W21 = -L22^-1 (L21 L11^-1) on MFMA.
-// *info receives the 1-based global index of the first non-positive pivot
-// (the factorisation continues with a unit pivot so the output stays
-// finite).
+// k_leaf64 runs this on a 64 block; k_leaf128 runs it twice inside a 128
+// block, with the products that join the two halves on four wavefronts, so
+// a 128 block is one launch instead of five.
 constexpr int LP = 33;                      // LDS row stride of 32x32 tiles
-
 constexpr int LQ = 36;                      // 16-byte aligned LDS rows
+constexpr int XS = 65;                      // LDS row stride of 64x64 blocks
 
 // Right-looking factorisation of the 32 columns held in a[] (lane i: row i
-// of the panel; lanes < 32 are the diagonal block).  The scaled column of
-// each step goes through lcol (LDS) and comes back as 4-wide broadcast
-// reads.  Pivots use the hardware reciprocal square root: the factor only
+// of the panel; lanes < 32 are the diagonal block), four columns at a time.
+// Inside a block of four the few entries L[j + v][j + u] the lanes need are
+// broadcast with v_readlane; the block's columns then go through LDS once
+// (lane k writes L[k][j .. j + 3] as a float4) and every lane applies the
+// rank-4 update to its columns k >= j + 4.  Eight LDS exchanges per panel
+// instead of 32 keep the dependent chain short.  lc: 2 x 32 float4 of LDS.
+// Pivots use the hardware reciprocal square root: the factor only
 // preconditions.  Returns the 1-based column of the first non-positive
 // pivot (0 if none).
 __device__ __forceinline__ int panel32(float (&a)[32], int i,
-                                       float* __restrict__ lcol) {
+                                       float4* __restrict__ lc) {
   int bad = 0;
 #pragma unroll
-  for (int j = 0; j < 32; ++j) {
-    float piv = __int_as_float(
-        __builtin_amdgcn_readlane(__float_as_int(a[j]), j));
-    if (!(piv > 0.0f)) {
-      if (bad == 0) bad = j + 1;
-      piv = 1.0f;
-    }
-    const float rs = __builtin_amdgcn_rsqf(piv);
-    float l = (i == j) ? piv * rs : a[j] * rs;
-    if (i < j) l = 0.0f;
-    a[j] = l;
-    lcol[i] = l;
-    __builtin_amdgcn_s_waitcnt(0xc07f);      // lgkmcnt(0): write visible
-    __builtin_amdgcn_wave_barrier();
+  for (int j = 0; j < 32; j += 4) {
 #pragma unroll
-    for (int k = (j + 1) & ~3; k < 32; k += 4) {
-      const float4 v = *(const float4*)(lcol + k);
-      const float vv[4] = {v.x, v.y, v.z, v.w};
+    for (int u = 0; u < 4; ++u) {
+      const int c = j + u;
+      float piv = __int_as_float(
+          __builtin_amdgcn_readlane(__float_as_int(a[c]), c));
+      if (!(piv > 0.0f)) {
+        if (bad == 0) bad = c + 1;
+        piv = 1.0f;
+      }
+      const float rs = __builtin_amdgcn_rsqf(piv);
+      float l = (i == c) ? piv * rs : a[c] * rs;
+      if (i < c) l = 0.0f;
+      a[c] = l;
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (k + u > j) a[k + u] = fmaf(-l, vv[u], a[k + u]);
+      for (int v = u + 1; v < 4; ++v) {
+        const float lk = __int_as_float(
+            __builtin_amdgcn_readlane(__float_as_int(l), j + v));
+        a[j + v] = fmaf(-l, lk, a[j + v]);
+      }
     }
-    __builtin_amdgcn_wave_barrier();
+    if (j + 4 < 32) {
+      float4* buf = lc + 32 * ((j >> 2) & 1);
+      if (i < 32) buf[i] = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
+      __builtin_amdgcn_s_waitcnt(0xc07f);    // lgkmcnt(0): writes visible
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int k = j + 4; k < 32; ++k) {
+        const float4 v = buf[k];
+        float x = a[k];
+        x = fmaf(-a[j], v.x, x);
+        x = fmaf(-a[j + 1], v.y, x);
+        x = fmaf(-a[j + 2], v.z, x);
+        x = fmaf(-a[j + 3], v.w, x);
+        a[k] = x;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
   }
   return bad;
 }
 
-// D (32x32, MFMA layout) = X Y for 32x32 LDS tiles; ta / tb: read X / Y
-// transposed (element (r, c) of the operand at [c * LP + r]).
+// D (32x32, MFMA layout: register e of lane l is row 8 (e / 4) + 4 (l / 32)
+// + e % 4, column l % 32) = X Y over K, for LDS operands with row strides SX,
+// SY; TA / TB: read X / Y transposed (element (r, c) of the operand at
+// [c * S + r]).
+template <int SX, int SY, int K, bool TA, bool TB>
 __device__ __forceinline__ floatx16 mm32(const float* __restrict__ X,
-                                         bool ta, const float* __restrict__ Y,
-                                         bool tb, int lane) {
-  const int r = lane & 31, h = lane >> 5;
+                                         const float* __restrict__ Y,
+                                         int lane) {
+  const int r = lane & 31, h = (lane >> 5) & 1;
   floatx16 acc;
 #pragma unroll
   for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
 #pragma unroll
-  for (int kk = 0; kk < 16; ++kk) {
+  for (int kk = 0; kk < K / 2; ++kk) {
     const int k = 2 * kk + h;
-    const float av = ta ? X[k * LP + r] : X[r * LP + k];
-    const float bv = tb ? Y[r * LP + k] : Y[k * LP + r];
+    const float av = TA ? X[k * SX + r] : X[r * SX + k];
+    const float bv = TB ? Y[r * SY + k] : Y[k * SY + r];
     acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
   }
   return acc;
 }
 
+__device__ __forceinline__ int mfma_row(int e, int lane) {
+  return 8 * (e >> 2) + 4 * ((lane >> 5) & 1) + (e & 3);
+}
+
+struct LeafLds {
+  float L11T[32 * LQ], L22T[32 * LQ];       // L11, L22 transposed
+  float L21[32 * LP], W11[32 * LP], W22[32 * LP], T[32 * LP];
+  float dg[64];
+  float4 lc[2 * 32];
+};
+
+// Factors the SPD block held in X (64x64, LDS, row stride XS; lower triangle
+// read) and overwrites X with W = L^-1 (lower; zeros above the diagonal).
+// The work runs on threads 0-63 (wavefront 0); every thread of the
+// workgroup must call (the phases are separated by workgroup barriers).
+// Returns, on wavefront 0, the 1-based index of the first non-positive
+// pivot in the block (0 if none); the factorisation continues with a unit
+// pivot so the output stays finite.
+__device__ __forceinline__ int chol_inv64(float* __restrict__ X, int t,
+                                          LeafLds& s) {
+  const bool act = t < 64;
+  const int i = t & 63, r = i & 31, h = i >> 5;
+  float a[32];
+  int bad = 0;
+  if (act) {
+#pragma unroll
+    for (int c = 0; c < 32; ++c)
+      a[c] = (h == 1 || c <= i) ? X[i * XS + c] : 0.0f;
+    bad = panel32(a, i, s.lc);
+    if (h == 0) {
+#pragma unroll
+      for (int c = 0; c < 32; ++c) s.L11T[c * LQ + r] = a[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 32; ++c) s.L21[r * LP + c] = a[c];
+    }
+  }
+  __syncthreads();
+  if (act) {
+    if (h == 0) s.dg[r] = __builtin_amdgcn_rcpf(s.L11T[r * LQ + r]);
+    const floatx16 d = mm32<LP, LP, 32, false, true>(s.L21, s.L21, i);
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+      X[(32 + mfma_row(e, i)) * XS + 32 + r] -= d[e];
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int c = 0; c < 32; ++c)
+      a[c] = (h == 0 && c <= r) ? X[(32 + r) * XS + 32 + c] : 0.0f;
+    const int bad2 = panel32(a, h ? 64 : r, s.lc);
+    if (bad == 0 && bad2) bad = 32 + bad2;
+    if (h == 0) {
+#pragma unroll
+      for (int c = 0; c < 32; ++c) s.L22T[c * LQ + r] = a[c];
+    }
+  }
+  __syncthreads();
+  if (act && h == 0) s.dg[32 + r] = __builtin_amdgcn_rcpf(s.L22T[r * LQ + r]);
+  __syncthreads();
+  if (act) {
+    // The 16x16 diagonal blocks of L11 and L22 are inverted per lane:
+    // lane i takes column c = i % 16 of block b = i / 16 (b = 0, 1: upper-
+    // left / lower-right block of L11; 2, 3: of L22), right-looking: once
+    // w[k] is final, column k (a row of the transposed copy, broadcast
+    // float4s that do not depend on w) updates the rows below.
+    const int b = i >> 4, c = i & 15, o = 16 * (b & 1);
+    const float* Lt = (b < 2 ? s.L11T : s.L22T) + o * LQ + o;
+    const float* dv = s.dg + 32 * (b >> 1) + o;
+    float w[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) w[k] = (k == c) ? 1.0f : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      w[k] *= dv[k];
+#pragma unroll
+      for (int m = (k + 1) & ~3; m < 16; m += 4) {
+        const float4 lv = *(const float4*)(Lt + k * LQ + m);
+        if (m > k) w[m] = fmaf(-lv.x, w[k], w[m]);
+        if (m + 1 > k) w[m + 1] = fmaf(-lv.y, w[k], w[m + 1]);
+        if (m + 2 > k) w[m + 2] = fmaf(-lv.z, w[k], w[m + 2]);
+        if (m + 3 > k) w[m + 3] = fmaf(-lv.w, w[k], w[m + 3]);
+      }
+    }
+    float* Wb = b < 2 ? s.W11 : s.W22;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      Wb[(o + k) * LP + o + c] = w[k];
+      if (o == 0) Wb[k * LP + 16 + c] = 0.0f;   // upper-right 16 block
+    }
+  }
+  __syncthreads();
+  // The lower-left 16 blocks: with L = [P 0; Q R], L^-1 has -R^-1 Q P^-1
+  // there; Y = Q P^-1 (into T), then -R^-1 Y, on v_mfma_f32_16x16x4f32
+  // (lane l: A[l % 16][l / 16], B[l / 16][l % 16]; register v of the result
+  // is row 4 (l / 16) + v, column l % 16).
+  if (act) {
+    const int q = i & 15, kq = i >> 4;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const float* Lt = u ? s.L22T : s.L11T;
+      const float* Wu = u ? s.W22 : s.W11;
+      floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int k = 4 * kk + kq;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(
+            Lt[k * LQ + 16 + q], Wu[k * LP + q], acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        s.T[(16 * u + 4 * kq + v) * LP + q] = acc[v];
+    }
+  }
+  __syncthreads();
+  if (act) {
+    const int q = i & 15, kq = i >> 4;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      float* Wu = u ? s.W22 : s.W11;
+      floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int k = 4 * kk + kq;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(
+            Wu[(16 + q) * LP + 16 + k], s.T[(16 * u + k) * LP + q], acc, 0,
+            0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        Wu[(16 + 4 * kq + v) * LP + q] = -acc[v];
+    }
+  }
+  __syncthreads();
+  if (act) {
+    // T = L21 W11
+    const floatx16 d = mm32<LP, LP, 32, false, false>(s.L21, s.W11, i);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) s.T[mfma_row(e, i) * LP + r] = d[e];
+  }
+  __syncthreads();
+  if (act) {
+    const floatx16 w21 = mm32<LP, LP, 32, false, false>(s.W22, s.T, i);
+#pragma unroll
+    for (int e = i; e < 32 * 32; e += 64) {
+      const int rr = e >> 5, cc = e & 31;
+      X[rr * XS + cc] = s.W11[rr * LP + cc];
+      X[rr * XS + 32 + cc] = 0.0f;
+      X[(32 + rr) * XS + 32 + cc] = s.W22[rr * LP + cc];
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+      X[(32 + mfma_row(e, i)) * XS + r] = -w21[e];
+  }
+  __syncthreads();
+  return bad;
+}
+
+// Global (row stride ld) <-> LDS (row stride XS) copies of a 64x64 block by
+// NT threads.
+template <int NT>
+__device__ __forceinline__ void block64_in(float* __restrict__ X,
+                                           const float* __restrict__ G,
+                                           long long ld, int t) {
+#pragma unroll
+  for (int f = t; f < 64 * 16; f += NT) {
+    const int row = f >> 4, c4 = (f & 15) * 4;
+    const float4 v = *(const float4*)(G + (long long)row * ld + c4);
+    float* x = X + row * XS + c4;
+    x[0] = v.x;
+    x[1] = v.y;
+    x[2] = v.z;
+    x[3] = v.w;
+  }
+}
+
+// Writes a 64x64 LDS block to global memory: all of it (full), or only its
+// 32-blocks on and below the diagonal (the upper-right one is skipped).
+template <int NT>
+__device__ __forceinline__ void block64_out(float* __restrict__ G,
+                                            long long ld,
+                                            const float* __restrict__ X,
+                                            bool full, int t) {
+#pragma unroll
+  for (int f = t; f < 64 * 16; f += NT) {
+    const int row = f >> 4, c4 = (f & 15) * 4;
+    if (!full && c4 >= 32 && row < 32) continue;
+    const float* x = X + row * XS + c4;
+    *(float4*)(G + (long long)row * ld + c4) =
+        make_float4(x[0], x[1], x[2], x[3]);
+  }
+}
+
+// Factors and inverts the 64x64 diagonal block of A at (k0, k0) (lower
+// triangle read) into W = L^-1 at (k0, k0) of W (lower; the upper-right
+// 32x32 block of W is left untouched).  *info receives the 1-based global
+// index of the first non-positive pivot.
 __global__ __launch_bounds__(64) void k_leaf64(const float* __restrict__ A,
                                                long long lda,
                                                float* __restrict__ W,
                                                long long ldw, int k0,
                                                int* __restrict__ info) {
-  __shared__ __align__(16) float L11[32 * LQ], L22[32 * LQ];
-  __shared__ float L21[32 * LP], A22[32 * LP];
-  __shared__ float W11[32 * LP], W22[32 * LP], T[32 * LP];
-  __shared__ __align__(16) float lcol[64];
-  __shared__ float dg[64];
-  const int i = threadIdx.x;
-  const int r = i & 31, h = i >> 5;
-  // A22 to LDS (the MFMA update below touches all of it).
+  __shared__ float X[64 * XS];
+  __shared__ __align__(16) LeafLds s;
+  const int t = threadIdx.x;
+  block64_in<64>(X, A + (long long)k0 * lda + k0, lda, t);
+  __syncthreads();
+  const int bad = chol_inv64(X, t, s);
+  if (t == 0 && bad != 0 && *info == 0) *info = k0 + bad;
+  block64_out<64>(W + (long long)k0 * ldw + k0, ldw, X, false, t);
+}
+
+// The 128x128 diagonal block at (k0, k0) as one recursion node:
+// W11 = L11^-1, S = A21 W11^T, A22 -= S S^T, T = S W11, W22 = L22^-1,
+// W21 = -W22 T, all in LDS (four regions of 64x64).  Writes the lower
+// 32-blocks of W = L^-1 at (k0, k0) (its upper-right 32-blocks are left
+// untouched); A is only read.
+__global__ __launch_bounds__(256) void k_leaf128(const float* __restrict__ A,
+                                                 long long lda,
+                                                 float* __restrict__ W,
+                                                 long long ldw, int k0,
+                                                 int* __restrict__ info) {
+  __shared__ float X11[64 * XS], X21[64 * XS], X22[64 * XS], XT[64 * XS];
+  __shared__ __align__(16) LeafLds s;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const float* A0 = A + (long long)k0 * lda + k0;
+  block64_in<256>(X11, A0, lda, t);
+  block64_in<256>(X21, A0 + 64 * lda, lda, t);
+  block64_in<256>(X22, A0 + 64 * lda + 64, lda, t);
+  __syncthreads();
+  int bad = chol_inv64(X11, t, s);
+  // wave w: 32x32 tile (w / 2, w % 2) of the 64x64 products
+  const int ib = w >> 1, jb = w & 1;
+  {
+    // S = A21 W11^T (W11 lower: its upper-right block is zero)
+    const floatx16 d = mm32<XS, XS, 64, false, true>(
+        X21 + 32 * ib * XS, X11 + 32 * jb * XS, lane);
+    __syncthreads();                           // every read of A21 done
 #pragma unroll
-  for (int e = i; e < 32 * 32; e += 64) {
-    const int rr = e >> 5, cc = e & 31;
-    A22[rr * LP + cc] =
-        A[(long long)(k0 + 32 + rr) * lda + k0 + 32 + cc];
-  }
-  // 1. panel [A11; A21]
-  float a[32];
-  const float* row = A + (long long)(k0 + i) * lda + k0;
-#pragma unroll
-  for (int c = 0; c < 32; c += 4) {
-    const float4 v = *(const float4*)(row + c);
-    a[c] = (i >= 32 || c <= i) ? v.x : 0.0f;
-    a[c + 1] = (i >= 32 || c + 1 <= i) ? v.y : 0.0f;
-    a[c + 2] = (i >= 32 || c + 2 <= i) ? v.z : 0.0f;
-    a[c + 3] = (i >= 32 || c + 3 <= i) ? v.w : 0.0f;
-  }
-  int bad = panel32(a, i, lcol);
-  if (bad) bad += k0;
-  if (h == 0) {
-#pragma unroll
-    for (int c = 0; c < 32; c += 4)
-      *(float4*)(L11 + r * LQ + c) = make_float4(a[c], a[c + 1], a[c + 2],
-                                                a[c + 3]);
-  } else {
-#pragma unroll
-    for (int c = 0; c < 32; ++c) L21[r * LP + c] = a[c];
+    for (int e = 0; e < 16; ++e)
+      X21[(32 * ib + mfma_row(e, lane)) * XS + 32 * jb + (lane & 31)] = d[e];
   }
   __syncthreads();
-  if (h == 0) dg[r] = __builtin_amdgcn_rcpf(L11[r * LQ + r]);
-  // 2. A22 -= L21 L21^T
   {
-    const floatx16 d = mm32(L21, false, L21, true, i);
+    // T = S W11; A22 -= S S^T on the three lower tiles
+    const floatx16 d = mm32<XS, XS, 64, false, false>(
+        X21 + 32 * ib * XS, X11 + 32 * jb, lane);
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const int rr = 8 * (e >> 2) + 4 * h + (e & 3);
-      A22[rr * LP + r] -= d[e];
+    for (int e = 0; e < 16; ++e)
+      XT[(32 * ib + mfma_row(e, lane)) * XS + 32 * jb + (lane & 31)] = d[e];
+    if (jb <= ib) {
+      const floatx16 g = mm32<XS, XS, 64, false, true>(
+          X21 + 32 * ib * XS, X21 + 32 * jb * XS, lane);
+#pragma unroll
+      for (int e = 0; e < 16; ++e)
+        X22[(32 * ib + mfma_row(e, lane)) * XS + 32 * jb + (lane & 31)] -=
+            g[e];
     }
   }
   __syncthreads();
-  // 3. A22 = L22 L22^T (lanes 0-31 hold its rows)
-#pragma unroll
-  for (int c = 0; c < 32; ++c)
-    a[c] = (h == 0 && c <= r) ? A22[r * LP + c] : 0.0f;
+  const int bad2 = chol_inv64(X22, t, s);
+  if (bad == 0 && bad2 != 0) bad = 64 + bad2;
+  if (t == 0 && bad != 0 && *info == 0) *info = k0 + bad;
   {
-    int bad2 = panel32(a, h ? 64 : r, lcol);
-    if (bad == 0 && bad2) bad = k0 + 32 + bad2;
-  }
-  if (h == 0) {
+    // W21 = -W22 T into the S slot (S is no longer read)
+    const floatx16 d = mm32<XS, XS, 64, false, false>(
+        X22 + 32 * ib * XS, XT + 32 * jb, lane);
 #pragma unroll
-    for (int c = 0; c < 32; c += 4)
-      *(float4*)(L22 + r * LQ + c) = make_float4(a[c], a[c + 1], a[c + 2],
-                                                a[c + 3]);
-  }
-  if (bad != 0 && i == 0 && *info == 0) *info = bad;
-  __syncthreads();
-  if (h == 0) dg[32 + r] = __builtin_amdgcn_rcpf(L22[r * LQ + r]);
-  __syncthreads();
-  // 4. column r of L11^-1 (h = 0) or of L22^-1 (h = 1); two interleaved
-  // partial sums per row halve the dependent-FMA chain.
-  {
-    const float* Ls = h ? L22 : L11;
-    const float* dv = dg + 32 * h;
-    float w[32];
-#pragma unroll
-    for (int k = 0; k < 32; ++k) {
-      float s0 = (k == r) ? 1.0f : 0.0f, s1 = 0.0f;
-#pragma unroll
-      for (int m = 0; m < k; m += 4) {
-        const float4 lv = *(const float4*)(Ls + k * LQ + m);
-        s0 = fmaf(-lv.x, w[m], s0);
-        if (m + 1 < k) s1 = fmaf(-lv.y, w[m + 1], s1);
-        if (m + 2 < k) s0 = fmaf(-lv.z, w[m + 2], s0);
-        if (m + 3 < k) s1 = fmaf(-lv.w, w[m + 3], s1);
-      }
-      w[k] = (s0 + s1) * dv[k];
-    }
-    float* Wd = h ? W22 : W11;
-#pragma unroll
-    for (int k = 0; k < 32; ++k) Wd[k * LP + r] = w[k];
+    for (int e = 0; e < 16; ++e)
+      X21[(32 * ib + mfma_row(e, lane)) * XS + 32 * jb + (lane & 31)] = -d[e];
   }
   __syncthreads();
-  // 5. T = L21 W11, W21 = -W22 T
-  {
-    const floatx16 d = mm32(L21, false, W11, false, i);
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const int rr = 8 * (e >> 2) + 4 * h + (e & 3);
-      T[rr * LP + r] = d[e];
-    }
-  }
-  __syncthreads();
-  const floatx16 w21 = mm32(W22, false, T, false, i);
-  // Output: W11, W22 (lower, zeros above) and W21.
-#pragma unroll
-  for (int e = i; e < 32 * 32; e += 64) {
-    const int rr = e >> 5, cc = e & 31;
-    W[(long long)(k0 + rr) * ldw + k0 + cc] = W11[rr * LP + cc];
-    W[(long long)(k0 + 32 + rr) * ldw + k0 + 32 + cc] = W22[rr * LP + cc];
-  }
-#pragma unroll
-  for (int e = 0; e < 16; ++e) {
-    const int rr = 8 * (e >> 2) + 4 * h + (e & 3);
-    W[(long long)(k0 + 32 + rr) * ldw + k0 + r] = -w21[e];
-  }
+  float* W0 = W + (long long)k0 * ldw + k0;
+  block64_out<256>(W0, ldw, X11, false, t);
+  block64_out<256>(W0 + 64 * ldw, ldw, X21, true, t);
+  block64_out<256>(W0 + 64 * ldw + 64, ldw, X22, false, t);
 }
 
 // Split of a block of n rows into n1 + n2 (multiples of 64).  At the top
@@ -517,6 +706,11 @@ static void chol_node(float* A, long long lda, float* W, long long ldw, int o,
   if (n <= 64) {
     hipLaunchKernelGGL(k_leaf64, dim3(1), dim3(64), 0, st, A, lda, W, ldw, o,
                        info);
+    return;
+  }
+  if (n == 128 && !top) {
+    hipLaunchKernelGGL(k_leaf128, dim3(1), dim3(256), 0, st, A, lda, W, ldw,
+                       o, info);
     return;
   }
   const int n1 = split_of(n, top), n2 = n - n1, r2 = o + n1;
@@ -582,7 +776,6 @@ constexpr int RHS = 16;
 constexpr int VROWS = 256;        // rows per workgroup of the vector kernels
 
 typedef double doublex4 __attribute__((ext_vector_type(4)));
-typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // Column range of the k-loop of row stripe [i0, i0 + 16) for a lower (tri =
 // 1: columns <= row) or upper (tri = 2: columns >= row) triangular operand.
@@ -730,12 +923,13 @@ __global__ __launch_bounds__(512) void k_symv_rows(
 // waves take the 16-column chunks round-robin and are added through LDS in
 // a fixed order.  tri: A is lower (1) or upper (2) triangular.  With r64
 // given, rzp[wg][c] = sum_rows r64 * Out (fp64).
-__global__ __launch_bounds__(512) void k_gemv_rows(
+constexpr int GV_WAVES = 8;
+__global__ __launch_bounds__(64 * GV_WAVES) void k_gemv_rows(
     const float* __restrict__ A, long long lda, int m, int k, int tri,
     const float* __restrict__ X, float* __restrict__ Out, float sa,
     const float* __restrict__ V, float sv, const double* __restrict__ r64,
     double* __restrict__ rzp) {
-  __shared__ floatx4 red[8][64];
+  __shared__ floatx4 red[GV_WAVES][64];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int r = lane & 15, g = lane >> 4;
   const int i0 = blockIdx.x * 16;
@@ -744,7 +938,7 @@ __global__ __launch_bounds__(512) void k_gemv_rows(
   floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
   row_stripe_product(acc, A + (long long)(i0 + r) * lda + 4 * g,
                      X + (long long)(4 * g) * RHS + r, kb / 16 + w, ke / 16,
-                     8);
+                     GV_WAVES);
   red[w][lane] = acc;
   __syncthreads();
   if (w != 0) return;
@@ -753,7 +947,7 @@ __global__ __launch_bounds__(512) void k_gemv_rows(
   for (int v = 0; v < 4; ++v) {
     float o = acc[v];
 #pragma unroll
-    for (int u = 1; u < 8; ++u) o += red[u][lane][v];
+    for (int u = 1; u < GV_WAVES; ++u) o += red[u][lane][v];
     const long long e = (long long)(i0 + 4 * g + v) * RHS + r;
     o *= sa;
     if (V) o = fmaf(sv, V[e], o);
@@ -959,8 +1153,8 @@ static Ws carve(char* base, int n) {
 static void gemv(const float* A, long long lda, int m, int k, int tri,
                  const float* X, float* out, float sa, const float* V,
                  float sv, const double* r64, double* rzp, hipStream_t st) {
-  hipLaunchKernelGGL(k_gemv_rows, dim3(m / 16), dim3(512), 0, st, A, lda, m,
-                     k, tri, X, out, sa, V, sv, r64, rzp);
+  hipLaunchKernelGGL(k_gemv_rows, dim3(m / 16), dim3(64 * GV_WAVES), 0, st,
+                     A, lda, m, k, tri, X, out, sa, V, sv, r64, rzp);
 }
 
 // z = (L L^T)^-1 r (fp32) from W (W11, L21, W22) and WT = W^T; rz partials
