@@ -174,13 +174,26 @@ __device__ __forceinline__ void read_diag3(const float* __restrict__ plane,
   ws[3] = rb.y + re.x + rf.y;    // (2W, 2Wp) + (2W+1, 2Wp+1) + (2W+2, 2Wp+2)
 }
 
+// Slots of the 16x16 pooled stage: the 15 pooled diagonals D = -7..7 (8 - |D|
+// steps each, 64 in all) in four groups of 16 steps, one workgroup each.
+__device__ __forceinline__ int stage16_nsweep(int slot) {
+  return slot == 0 ? 3 : 4;
+}
+__device__ __forceinline__ int stage16_diag(int slot, int k) {
+  // 0: {0, 4, -4}; 1: {1, -1, 7, -7}; 2: {2, 3, 5, 6}; 3: {-2, -3, -5, -6}
+  if (slot == 0) return k == 0 ? 0 : (k == 1 ? 4 : -4);
+  if (slot == 1) return k == 0 ? 1 : (k == 1 ? -1 : (k == 2 ? 7 : -7));
+  const int v = k == 0 ? 2 : (k == 1 ? 3 : (k == 2 ? 5 : 6));
+  return slot == 2 ? v : -v;
+}
+
 // PF selects how the planes of a step are fetched (POOL == 2):
 //   0: issued at the step that uses them;
 //   1: prefetched into registers one step ahead, after the step's exchange;
 //   2: as 1, but issued right after the step's planes are written to LDS
 //      (both tensors share one exchange: two barriers per step);
 //   3: as 2 with two steps in flight (two register sets, even/odd steps).
-template <int S, int POOL, bool NTK, int PF>
+template <int S, int POOL, bool NTK, int PF, int SL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_stage_fast(
     const float* __restrict__ in_n, const float* __restrict__ in_t, int n1,
     int n2, int ndiag, int ngroups, int nblk, float wv9, float b_var,
@@ -244,7 +257,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const float4* lq2v = reinterpret_cast<const float4*>(lq2);
 
   if constexpr (POOL == 2) {
-    const int D = di - (H2 - 1);
+    // SL > 0: the workgroup sweeps the diagonals of slot di (shared
+    // normaliser tables, fewer workgroups); else diagonal D = di - (H2 - 1).
+    const int nsw = SL ? stage16_nsweep(di) : 1;
+    for (int sw = 0; sw < nsw; ++sw) {
+    const int D = SL ? stage16_diag(di, sw) : di - (H2 - 1);
     const int Hs = D > 0 ? D : 0;
     const int He = D < 0 ? H2 - 1 + D : H2 - 1;
     const int d0 = 2 * D, dp = 2 * D + 1, dm = 2 * D - 1;
@@ -400,6 +417,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       __builtin_amdgcn_s_waitcnt(0x0f70);    // vmcnt(0): drain the empty loads
     } else {
       for (int H = Hs; H <= He; ++H) step(H, na0, na1, H + 1);
+    }
     }
   } else {
     const int d = di - (S - 1);
@@ -1173,24 +1191,28 @@ __global__ __launch_bounds__(256) void k_qnorm(const float* __restrict__ q,
 
 static inline int pad8(long long n) { return (int)((n + 7) / 8 * 8); }
 
-template <int S, int POOL, int PF>
+template <int S, int POOL, int PF, int SL = 0>
 static int launch_stage_s(const float* in_n, const float* in_t, int n1,
                           int n2, bool ntk, float wv9, float b_var,
                           const float* q1, const float* q2, float* out_n,
                           float* out_t, hipStream_t st) {
+  static_assert(SL == 0 || (S == 16 && POOL == 2 && SL == 4),
+                "diagonal slots are laid out for the 16x16 pooled stage");
   constexpr int JPW = 256 / ((S / 2) * (S / 2));
   const long long T = n2 == 0 ? (long long)n1 : (long long)n1 * n2;
-  const int ndiag = POOL == 2 ? S - 1 : 2 * S - 1;
+  const int ndiag = SL ? SL : (POOL == 2 ? S - 1 : 2 * S - 1);
   const long long ngroups = (T + JPW - 1) / JPW;
   if (ngroups * ndiag > 0x7ffffff0LL)
     return set_error(NTA_EINVAL, "nta_stage: tile too large (%lld pairs)", T);
   const int nblk = pad8(ngroups * ndiag);
   if (ntk)
-    hipLaunchKernelGGL((k_stage_fast<S, POOL, true, PF>), dim3(nblk), dim3(256),
+    hipLaunchKernelGGL((k_stage_fast<S, POOL, true, PF, SL>), dim3(nblk),
+                       dim3(256),
                        0, st, in_n, in_t, n1, n2, ndiag, (int)ngroups, nblk,
                        wv9, b_var, q1, q2, out_n, out_t);
   else
-    hipLaunchKernelGGL((k_stage_fast<S, POOL, false, PF>), dim3(nblk), dim3(256),
+    hipLaunchKernelGGL((k_stage_fast<S, POOL, false, PF, SL>), dim3(nblk),
+                       dim3(256),
                        0, st, in_n, in_t, n1, n2, ndiag, (int)ngroups, nblk,
                        wv9, b_var, q1, q2, out_n, out_t);
   return check_launch("nta_stage");
@@ -1207,6 +1229,12 @@ constexpr int kPrefetch = NTA_STAGE_PREFETCH;
 #define NTA_STAGE16_PREFETCH 0
 #endif
 constexpr int kPrefetch16 = NTA_STAGE16_PREFETCH;
+// 16x16: four workgroups per group of pairs, each sweeping several
+// diagonals (0: one diagonal per workgroup).
+#ifndef NTA_STAGE16_SLOTS
+#define NTA_STAGE16_SLOTS 4
+#endif
+constexpr int kSlots16 = NTA_STAGE16_SLOTS;
 
 // The fused kernels implement the ReLU map only (k1 = 1, k0 = 0); other
 // ABRelu slopes run the generic kernels.
@@ -1225,7 +1253,7 @@ int launch_stage_fast(const float* in_nngp, const float* in_ntk, int n1,
       case 32: return launch_stage_s<32, 2, kPrefetch>(in_nngp, in_ntk, n1, n2, ntk, wv9,
                                             b_var, q1, q2, out_nngp, out_ntk,
                                             stream);
-      case 16: return launch_stage_s<16, 2, kPrefetch16>(in_nngp, in_ntk, n1, n2, ntk, wv9,
+      case 16: return launch_stage_s<16, 2, kPrefetch16, kSlots16>(in_nngp, in_ntk, n1, n2, ntk, wv9,
                                             b_var, q1, q2, out_nngp, out_ntk,
                                             stream);
       case 8: return launch_stage_s<8, 2, 1>(in_nngp, in_ntk, n1, n2, ntk, wv9,
