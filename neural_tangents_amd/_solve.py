@@ -15,7 +15,7 @@ def add_diagonal_regularizer(A: torch.Tensor, diag_reg: float,
   """``predict.py:1224-1232`` (returns a new tensor)."""
   n = A.shape[0]
   if not diag_reg_absolute_scale:
-    diag_reg = diag_reg * torch.trace(A.to(torch.float64)) / n
+    diag_reg = diag_reg * torch.diagonal(A).to(torch.float64).sum() / n
   out = A.clone()
   out.diagonal().add_(diag_reg)
   return out
@@ -26,7 +26,8 @@ def regularizer_value(A: torch.Tensor, diag_reg: float,
   """The FP64 shift lambda added to the diagonal."""
   if diag_reg_absolute_scale:
     return float(diag_reg)
-  return float(diag_reg) * float(torch.trace(A.to(torch.float64))) / A.shape[0]
+  trace = float(torch.diagonal(A).to(torch.float64).sum())
+  return float(diag_reg) * trace / A.shape[0]
 
 
 class CholeskySolver:
