@@ -878,33 +878,41 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
     }
   }
   __syncthreads();
-  // MFMA operands (v_mfma_f32_16x16x32_f16, 8 halves per lane): lane group
-  // c4 carries two patch positions of 4 channels each.  Step 0 holds
-  // positions 2 c4, 2 c4 + 1 (0..7), step 1 position 8 in group 0; the
-  // empty slots point at padded column 0, which is zero on every row.
-  // Byte offsets from the image base, before the row term.
+  // MFMA operands (v_mfma_f32_16x16x32_f16, 8 halves per lane; A and B
+  // share the K order).  The 9 x 4 patch entries (tap (ty, tx), 4 channels)
+  // are laid out so that every lane reads contiguous halves:
+  //   step 0: lane group c4 < 3 carries taps (c4, 0) and (c4, 1), 16
+  //     contiguous bytes of its row;
+  //   step 1: lane group c4 < 3 carries tap (c4, 2); image i also reads the
+  //     next 8 bytes, image j has zeros there, so they add nothing.
+  // Lane group 3 reads zeros: padded column 33 and the next row's column 0
+  // (step 0), padded column 0 (step 1; its upper halves meet image j's
+  // zeros).  Byte offsets from the image base, before the row term.
   constexpr int kRowB = kXS * 4 * 2;            // bytes per padded row
   constexpr int kHlB = kXS * kXS * 4 * 2;       // bytes of one hi or lo image
-  int oa[2][2], ob[2][2];
+  int oa[2], ob[2];
 #pragma unroll
-  for (int st = 0; st < 2; ++st)
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int pos = st == 0 ? 2 * c4 + h : (c4 == 0 && h == 0 ? 8 : -1);
-      const int ty = pos / 3, tx = pos - 3 * (pos / 3);
-      oa[st][h] = pos < 0 ? 0
-                          : ty * kRowB + (16 * (wave >> 1) + m16 + tx) * 8;
-      ob[st][h] = pos < 0 ? 0
-                          : ty * kRowB + (16 * (wave & 1) + m16 + tx) * 8;
-    }
+  for (int st = 0; st < 2; ++st) {
+    const int tx = st == 0 ? 0 : 2;
+    oa[st] = c4 < 3 ? c4 * kRowB + (16 * (wave >> 1) + m16 + tx) * 8
+                    : (st == 0 ? (kXS - 1) * 8 : 0);
+    ob[st] = c4 < 3 ? c4 * kRowB + (16 * (wave & 1) + m16 + tx) * 8
+                    : (st == 0 ? (kXS - 1) * 8 : 0);
+  }
   const char* xhb = reinterpret_cast<const char*>(xh);
-  // Operand of image img (0: i, 1: j), part hl, padded row base `row`,
-  // step st: the 8 halves of the lane.
-  auto opnd = [&](int img, int hl, int row, int st) {
-    const char* b = xhb + (img * 2 + hl) * kHlB + row * kRowB;
-    const int* o = img ? ob[st] : oa[st];
-    const half4v lo = *reinterpret_cast<const half4v*>(b + o[0]);
-    const half4v hi = *reinterpret_cast<const half4v*>(b + o[1]);
+  // Operands of image img (0: i, 1: j), part hl, padded row base `row`.
+  auto opnd8 = [&](int img, int hl, int row) {
+    const char* b = xhb + (img * 2 + hl) * kHlB + row * kRowB + (img ? ob[0] : oa[0]);
+    const half4v lo = *reinterpret_cast<const half4v*>(b);
+    const half4v hi = *reinterpret_cast<const half4v*>(b + 8);
+    return half8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+  };
+  auto opnd4 = [&](int img, int hl, int row) {
+    const char* b = xhb + (img * 2 + hl) * kHlB + row * kRowB + (img ? ob[1] : oa[1]);
+    const half4v lo = *reinterpret_cast<const half4v*>(b);
+    const _Float16 z = 0;
+    if (img == 1) return half8v{lo[0], lo[1], lo[2], lo[3], z, z, z, z};
+    const half4v hi = *reinterpret_cast<const half4v*>(b + 8);
     return half8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
   };
   const float2* qi = qa1 + i * (S * S);
@@ -921,14 +929,16 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
     if (!ok) hh = hp = 0;              // uniform; the values are dropped
     floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
     if (ok) {
-#pragma unroll
-      for (int st = 0; st < 2; ++st) {
-        const half8v ah = opnd(0, 0, hh, st), al = opnd(0, 1, hh, st);
-        const half8v bh = opnd(1, 0, hp, st), bl = opnd(1, 1, hp, st);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc, 0, 0, 0);
-      }
+      const half8v ah = opnd8(0, 0, hh), al = opnd8(0, 1, hh);
+      const half8v bh = opnd8(1, 0, hp), bl = opnd8(1, 1, hp);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc, 0, 0, 0);
+      const half8v ch = opnd4(0, 0, hh), cl = opnd4(0, 1, hh);
+      const half8v dh = opnd4(1, 0, hp), dl = opnd4(1, 1, hp);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ch, dh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ch, dl, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(cl, dh, acc, 0, 0, 0);
     }
     const float4 q01 = *reinterpret_cast<const float4*>(qi + hh * S + w0);
     const float4 q23 = *reinterpret_cast<const float4*>(qi + hh * S + w0 + 2);
@@ -1013,26 +1023,49 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
     const int Hpn = Hn - D;
 #pragma unroll
     for (int k = 0; k < 6; ++k) acc[k] = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int st = 0; st < 2; ++st) {
+    {
       half8v ah[3], al[3], bh[3], bl[3];
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
-        ah[r] = opnd(0, 0, 2 * Hn + r, st);
-        bh[r] = opnd(1, 0, 2 * Hpn + r, st);
+        ah[r] = opnd8(0, 0, 2 * Hn + r);
+        bh[r] = opnd8(1, 0, 2 * Hpn + r);
       }
 #pragma unroll
       for (int k = 0; k < 6; ++k)
         acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[ar[k]], bh[br[k]],
                                                         acc[k], 0, 0, 0);
 #pragma unroll
-      for (int r = 0; r < 3; ++r) bl[r] = opnd(1, 1, 2 * Hpn + r, st);
+      for (int r = 0; r < 3; ++r) bl[r] = opnd8(1, 1, 2 * Hpn + r);
 #pragma unroll
       for (int k = 0; k < 6; ++k)
         acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[ar[k]], bl[br[k]],
                                                         acc[k], 0, 0, 0);
 #pragma unroll
-      for (int r = 0; r < 3; ++r) al[r] = opnd(0, 1, 2 * Hn + r, st);
+      for (int r = 0; r < 3; ++r) al[r] = opnd8(0, 1, 2 * Hn + r);
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[ar[k]], bh[br[k]],
+                                                        acc[k], 0, 0, 0);
+    }
+    {
+      half8v ah[3], al[3], bh[3], bl[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        ah[r] = opnd4(0, 0, 2 * Hn + r);
+        bh[r] = opnd4(1, 0, 2 * Hpn + r);
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[ar[k]], bh[br[k]],
+                                                        acc[k], 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) bl[r] = opnd4(1, 1, 2 * Hpn + r);
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[ar[k]], bl[br[k]],
+                                                        acc[k], 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) al[r] = opnd4(0, 1, 2 * Hn + r);
 #pragma unroll
       for (int k = 0; k < 6; ++k)
         acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[ar[k]], bh[br[k]],
