@@ -718,7 +718,7 @@ constexpr int kLS2 = 36;              // exchange plane row (4 rows = 16 banks)
 // Vector instructions scheduled between consecutive MFMAs of the next step's
 // patch Grams (0: leave the order to the compiler).
 #ifndef NTA_F12_SCHED
-#define NTA_F12_SCHED 12
+#define NTA_F12_SCHED 16
 #endif
 constexpr int kPL2 = 34 * kLS2;
 
