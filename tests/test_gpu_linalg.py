@@ -39,7 +39,7 @@ def _factor(K32):
   return W, WT, int(info.item())
 
 
-@pytest.mark.parametrize('n', [64, 128, 192, 320, 640, 1024, 1216])
+@pytest.mark.parametrize('n', [64, 128, 192, 256, 320, 640, 1024, 1216])
 def test_factor_blocks_match_fp64(n):
   """W holds L11^-1, L22^-1 on the diagonal blocks and L21 below them."""
   K = _spd(n, seed=n)
@@ -64,6 +64,27 @@ def test_factor_blocks_match_fp64(n):
   # the upper triangles stay zero
   assert float(Wd[:n1, :n1].triu(1).abs().max()) == 0.0
   assert float(Wd[n1:, n1:].triu(1).abs().max()) == 0.0
+
+
+def test_factor_large_node_uses_128_tiles():
+  """n = 6144: the top node's products run on 128x128 tiles (the smaller
+  nodes on 64x64 tiles and the 128x128 leaves), checked block by block
+  against torch FP64 on the GPU."""
+  n = 6144
+  g = torch.Generator(device=DEV).manual_seed(11)
+  X = torch.randn(n, n, generator=g, device=DEV, dtype=torch.float64)
+  K = X @ X.T / n + 0.5
+  K.diagonal().add_(1e-2)
+  W, WT, info = _factor(K.float().cpu())
+  assert info == 0
+  n1 = _lib.value('nta_chol_split', n)
+  L = torch.linalg.cholesky(K.float().double())
+  Wd = W.double()
+  for blk, want in ((Wd[:n1, :n1], torch.linalg.inv(L[:n1, :n1])),
+                    (Wd[n1:, n1:], torch.linalg.inv(L[n1:, n1:])),
+                    (Wd[n1:, :n1], L[n1:, :n1])):
+    err = (blk - want).abs().max() / want.abs().max()
+    assert err <= 2e-4, (n, float(err))
 
 
 @pytest.mark.parametrize('n,nrhs', [(64, 3), (640, 10), (1024, 16),
