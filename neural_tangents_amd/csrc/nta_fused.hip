@@ -1016,9 +1016,11 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
         tab + (row < S ? row : S - 1) * S + col);
   };
   // First-stage patch Grams of the six planes of step Hn (split FP16, 36
-  // MFMAs).  Padded rows 2Hn + r, r = 0..2 (+ 0..2 taps); past the grid
-  // (r = 2 at the last step) they read the rows below, or the lo image or
-  // the words after it: finite values of planes that are dropped.
+  // MFMAs).  Padded rows 2Hn + r, r = 0..2 (+ 0..2 taps); past the sweep
+  // end (Hn = He + 1) they read the rows below, the lo image, the other
+  // image or the LDS words after them: values of planes that are dropped
+  // (every plane has its own accumulator, so they never reach a plane that
+  // is used).
   auto gram6 = [&](int Hn, floatx4 (&acc)[6]) {
     const int Hpn = Hn - D;
 #pragma unroll
