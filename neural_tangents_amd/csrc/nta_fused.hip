@@ -693,8 +693,8 @@ __global__ __launch_bounds__(256) void k_first16(
 // Conv(3x3 SAME) -> ReLU -> Conv(3x3 SAME) -> ReLU -> AvgPool(2x2) on 32x32
 // images, with the first stage's state never written to memory.  The
 // decomposition is k_stage_fast's (a sweep in H along each pooled diagonal
-// H - H' = D of one pair), with several diagonals of 64 steps in all per
-// workgroup (f12_diag) so that the images are prepared once per 64 steps;
+// H - H' = D of one pair), with several diagonals of 128 steps in all per
+// workgroup (f12_diag) so that the images are prepared once per 128 steps;
 // the six first-stage planes a step consumes are computed in place from the
 // two images held in LDS:
 //   plane (h, h') = ReLU(scale1 * P_i(h) P_j(h')^T + b1),
@@ -743,10 +743,11 @@ __device__ __forceinline__ void s1_exchange(float* __restrict__ lp,
 // Workgroups ("slots") per pair and the diagonals each sweeps.  With 16:
 // slot 0 sweeps D = 0, slot s > 0 sweeps s and s - 16 (16 steps).  With 8:
 // slot 0 sweeps 0, 8, -8, slot s > 0 sweeps s, s - 16, s + 8, s - 8 (32
-// steps).  With 4 (default; 2 and 1 measure the same): the diagonals of two
-// of those groups (64 steps), so the images are prepared once per 64 steps.
+// steps).  With 4, 2 (default) or 1: the diagonals of 2, 4 or 8 of those
+// groups (64, 128 or 256 steps), so the images are prepared once per 128
+// steps by default (measured: 4 slots 0.7% slower, 1 the same, 8 3% slower).
 #ifndef NTA_F12_SLOTS
-#define NTA_F12_SLOTS 4
+#define NTA_F12_SLOTS 2
 #endif
 constexpr int kF12Slots = NTA_F12_SLOTS;
 static_assert(kF12Slots == 1 || kF12Slots == 2 || kF12Slots == 4 ||
