@@ -2,7 +2,8 @@
 ``torch.autograd.Function``.
 
 The reference has no adjoint code: ``jax.grad`` traces its ``kernel_fn``
-(SURVEY a12).  Here the reverse pass of each fused stage runs on the GPU:
+(``linear.py:1325-1424``, ``elementwise.py:1283-1310`` for the singular-point
+rules).  Here the reverse pass of each fused stage runs on the GPU:
 
 * ``nta_relu_bwd`` (``csrc/nta_adjoint.hip``): the adjoint of the ReLU
   arc-cosine map and of the average pool, per pre-pool element, including
