@@ -1,4 +1,5 @@
-"""Benchmark: Myrtle-5 full-spatial NNGP+NTK on one MI355X.
+"""Benchmark: Myrtle-5 full-spatial NNGP+NTK on one MI355X, plus the
+kernel-regression, Myrtle-10 and gradient forms.
 
 One step = one ``kernel_fn(x1, x2, ('nngp', 'ntk'))`` over the whole
 workload (Myrtle-5, 32x32x3, x1 and x2 each [256, 32, 32, 3] fp32 blurred
@@ -7,10 +8,24 @@ chains.  Inputs are resident in HBM before the timed region.
 
 Prints ONE JSON line.  ``value`` is pairs/s; ``roofline`` describes the
 dominant kernel, timed with HIP events on its launch stream over the timed
-steps; ``cpu_baseline`` is the FP32 oracle (the reference algorithm restated
-in numpy, one thread) on a bounded sample of the same workload.
+steps.  Extra keys, all measured after the timed steps:
+
+* ``krr``: the kernel-regression form (Myrtle-5 32x32 NTK of 8,192 train and
+  1,024 test images, ``diag_reg = 1e-4``): mixed solve (FP32 MFMA Cholesky +
+  FP64 refinement) against the plain FP64 solve on the same kernel, both
+  times, the speed-up and the relative errors of alpha and the test
+  predictions against FP64;
+* ``myrtle10``: the Myrtle-10 workload (x1 [128], x2 [512]) pairs/s and
+  per-kernel GB/s;
+* ``grad``: wall time of the differentiation form (loss and gradients with
+  respect to 100 support images, W_std and b_std against 2,048 targets);
+* ``cpu_baseline``: the oracle (the reference algorithm restated in numpy,
+  FP32) on a bounded block of the workload, one thread per host core; the
+  same leg evaluates the FP64 oracle on a block of the benchmarked output
+  that spans pair-tile boundaries and reports ``rel_err_vs_oracle``.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+       [--no-krr] [--no-m10] [--no-grad]
 """
 import argparse
 import json
@@ -31,6 +46,10 @@ WORKLOADS = {
     # one tile of 256 pairs: the launch the counter summaries are taken on
     'myrtle5_32x32_16x16': ((2, 1, 1), 32, 16, 16),
 }
+
+
+def log(msg):
+  print(msg, file=sys.stderr, flush=True)
 
 
 def canonical_bytes_per_pair(depths, side):
@@ -58,43 +77,133 @@ def canonical_bytes_per_pair(depths, side):
 # HBM traffic per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (scaled by a plain copy of known size), per trace label of the engine.
 PMC_FILE = os.path.join(ROOT, 'profiles', 'myrtle5_32x32_hbm_traffic_pmc.json')
+# Vector-ALU counters of the fused first kernel (rocprofv3 --pmc).
+VALU_FILE = os.path.join(ROOT, 'profiles', 'myrtle5_32x32_valu_pmc.json')
+
+
+def _load_json(path):
+  try:
+    with open(path) as f:
+      return json.load(f)
+  except (OSError, ValueError):
+    return None
 
 
 def measured_traffic(label, pairs_per_launch):
   """HBM bytes per launch of ``label`` from the committed counter summary
   (scaled to the launch's pair count), or None."""
+  d = _load_json(PMC_FILE)
   try:
-    with open(PMC_FILE) as f:
-      d = json.load(f)
     k = d['kernels'][label]
-  except (OSError, KeyError, ValueError):
+  except (TypeError, KeyError):
     return None
   return k['traffic_bytes_per_launch'] * pairs_per_launch / k[
       'pairs_per_launch']
 
 
-def cpu_baseline(depths, x1, x2, n_sample):
-  """Times the oracle (FP32, one thread) on an n_sample x n_sample block."""
+def host_threads():
+  """Host threads for the CPU leg: the process's CPU affinity, capped by
+  OMP_NUM_THREADS when set (the GPU box's CPU share)."""
+  n = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') \
+      else (os.cpu_count() or 1)
+  omp = os.environ.get('OMP_NUM_THREADS')
+  if omp and omp.isdigit() and int(omp) > 0:
+    n = min(n, int(omp))
+  return max(1, n)
+
+
+def cpu_leg(depths, x1, x2, got, rows, cols, err_rows, err_cols):
+  """The CPU leg: times the oracle (FP32, numpy, one thread per host core,
+  each thread an independent row block of the sample against the sample's
+  columns) and evaluates the FP64 oracle on the (err_rows x err_cols) block
+  of the benchmarked output ``got`` = (nngp, ntk) host arrays."""
   import numpy as np
+  from concurrent.futures import ThreadPoolExecutor
   from threadpoolctl import threadpool_limits
   from oracle import stax_oracle as so
   layers = so.myrtle_layers(tuple(depths))
-  a, b = x1[:n_sample], x2[:n_sample]
+  nthr = host_threads()
+  xb = x2[:cols]
+
+  def block(i0, i1, dtype=np.float32, xa=x1, xc=xb):
+    return so.kernel_fn(layers, xa[i0:i1], xc, ('nngp', 'ntk'), dtype=dtype)
+
+  per = max(1, rows // nthr)
+  chunks = [(i, min(rows, i + per)) for i in range(0, rows, per)]
   with threadpool_limits(1):
-    t0 = time.perf_counter()
-    so.kernel_fn(layers, a, b, ('nngp', 'ntk'), dtype=np.float32)
-    dt = time.perf_counter() - t0
-  pairs = n_sample * n_sample
-  return {'value': pairs / dt, 'unit': 'pairs/s', 'cores': 1,
+    with ThreadPoolExecutor(nthr) as ex:
+      t0 = time.perf_counter()
+      list(ex.map(lambda c: block(*c), chunks))
+      dt = time.perf_counter() - t0
+      # FP64 oracle on the error block (one row per task)
+      ea, eb = x1[err_rows], x2[err_cols]
+      refs = list(ex.map(lambda i: block(i, i + 1, np.float64, ea, eb),
+                         range(len(err_rows))))
+  pairs = rows * cols
+  base = {'value': pairs / dt, 'unit': 'pairs/s', 'cores': nthr,
           'kind': 'port',
-          'sample': f'{n_sample}x{n_sample} block of the workload '
-                    f'({pairs} pairs + {2 * n_sample} self chains), '
-                    f'{dt:.1f} s'}
+          'sample': f'{rows}x{cols} block of the workload ({pairs} pairs + '
+                    f'{rows + cols} self chains; {len(chunks)} row blocks '
+                    f'on {nthr} threads, each with its own x2 self chains), '
+                    f'FP32 numpy, {dt:.1f} s'}
+  err = {}
+  for gi, g in enumerate(('nngp', 'ntk')):
+    ref = np.concatenate([r[g] for r in refs], axis=0)
+    blk = got[gi][np.ix_(err_rows, err_cols)].astype(np.float64)
+    err[g] = float(np.linalg.norm(blk - ref) / np.linalg.norm(ref))
+  return base, err
 
 
-def krr_bench(args):
+def tile_rows(kernel_fn, x1, x2):
+  """Row-tile size of the engine's pair tiling for this call."""
+  from neural_tangents_amd import _engine
+  plan = kernel_fn.plan(x1.shape)
+  fused = _engine._fuses_first_two(plan, x1.shape)
+  per = _engine._per_pair_bytes(plan, True, fused)
+  T = max(1, _engine.DEFAULT_TILE_BYTES // per)
+  tj = min(x2.shape[0], T)
+  return max(1, min(x1.shape[0], T // tj))
+
+
+def timed_kernel_fn(kernel_fn, x1, x2, steps, warmup):
+  """Runs kernel_fn (nngp, ntk) warmup + steps times; returns (seconds of
+  the timed steps, per-entry-point launch statistics, last output)."""
+  import torch
+  from neural_tangents_amd import _engine
+  for _ in range(warmup):
+    kernel_fn(x1, x2, ('nngp', 'ntk'))
+  torch.cuda.synchronize()
+  _engine.TRACE = []
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(steps):
+    k = kernel_fn(x1, x2, ('nngp', 'ntk'))
+  torch.cuda.synchronize()
+  elapsed = time.perf_counter() - t0
+  trace, _engine.TRACE = _engine.TRACE, None
+  per = {}
+  for name, s, e, nbytes, pairs in trace:
+    d = per.setdefault(name, {'ms': 0.0, 'launches': 0, 'bytes': 0,
+                              'pairs': 0})
+    d['ms'] += s.elapsed_time(e)
+    d['launches'] += 1
+    d['bytes'] += nbytes
+    d['pairs'] += pairs
+  return elapsed, per, k
+
+
+def kernel_table(per, steps):
+  return {n: {'ms_per_step': v['ms'] / steps,
+              'launches_per_step': v['launches'] / steps,
+              'avg_launch_ms': v['ms'] / v['launches'],
+              'gbs': v['bytes'] / (v['ms'] * 1e-3) / 1e9
+              if v['ms'] > 0 else None}
+          for n, v in per.items()}
+
+
+def krr_result(ntr, nte, reps):
   """Kernel-regression solve (infinite-time gradient_descent_mse_ensemble):
-  Myrtle-5 32x32 NTK of n_train synthetic images, one-hot targets (-0.1),
+  Myrtle-5 32x32 NTK of ntr synthetic images, one-hot targets (-0.1),
   diag_reg = 1e-4 (relative to the mean trace).  Times the plain FP64
   Cholesky solve (rocSOLVER through torch) and the mixed solve (FP32 MFMA
   Cholesky + FP64-residual PCG), both on the same kernel, and reports the
@@ -102,33 +211,24 @@ def krr_bench(args):
   import torch
   from neural_tangents_amd import _solve, stax
   from neural_tangents_amd.data import krr_workload
-  ntr, nte = args.krr_train, args.krr_test
   xtr, xte, ytr, _ = krr_workload(ntr, nte, seed=0)
   dev = torch.device('cuda', 0)
   kernel_fn = stax.myrtle(5)[2]
   t0 = time.perf_counter()
   xtr_d = torch.from_numpy(xtr).to(dev)
-  # K_dd by row blocks on and above the block diagonal (mirrored), so that
-  # progress is printed every block.
-  blk = 2048
-  k_dd = torch.empty((ntr, ntr), dtype=torch.float32, device=dev)
-  for i0 in range(0, ntr, blk):
-    i1 = min(ntr, i0 + blk)
-    k_dd[i0:i1, i0:] = kernel_fn(xtr_d[i0:i1], xtr_d[i0:], 'ntk')
-    k_dd[i0:, i0:i1] = k_dd[i0:i1, i0:].T
-    torch.cuda.synchronize()
-    print(f'K_dd rows {i0}-{i1}: {time.perf_counter() - t0:.1f} s',
-          file=sys.stderr, flush=True)
-  k_dd = 0.5 * (k_dd + k_dd.T)        # exactly symmetric
+  log(f'krr: train kernel {ntr}x{ntr} (upper triangle) ...')
+  k_dd = kernel_fn(xtr_d, None, 'ntk')
+  torch.cuda.synchronize()
+  log(f'krr: train kernel {time.perf_counter() - t0:.1f} s')
   k_td = kernel_fn(torch.from_numpy(xte).to(dev), xtr_d, 'ntk')
   torch.cuda.synchronize()
   t_kernel = time.perf_counter() - t0
-  print(f'kernels: {t_kernel:.1f} s', file=sys.stderr, flush=True)
+  log(f'krr: kernels {t_kernel:.1f} s')
   y = torch.from_numpy(ytr).to(dev, torch.float64)
   res = {}
   for mode in ('fp64', 'mixed'):
     times = []
-    for rep in range(args.krr_reps):
+    for _ in range(reps):
       torch.cuda.synchronize()
       t = time.perf_counter()
       s = _solve.CholeskySolver(k_dd, 1e-4, False, mode)
@@ -138,24 +238,89 @@ def krr_bench(args):
       times.append(time.perf_counter() - t)
     res[mode] = (sorted(times)[len(times) // 2], alpha, f,
                  getattr(s, 'iterations', 0), getattr(s, 'last_residual',
-                                                      None))
+                                                      None), s)
   t64, a64, f64 = res['fp64'][:3]
-  tmx, amx, fmx, iters, resid = res['mixed']
+  tmx, amx, fmx, iters, resid, smx = res['mixed']
   err_a = float(torch.linalg.norm(amx - a64) / torch.linalg.norm(a64))
   err_f = float(torch.linalg.norm(fmx - f64) / torch.linalg.norm(f64))
-  out = {
+  n = ntr
+  chol_flop = n ** 3 / 3
+  log(f'krr: fp64 {t64 * 1e3:.2f} ms, mixed {tmx * 1e3:.2f} ms')
+  return {
       'metric': 'KRR solve time vs plain FP64 at <=1e-6 rel err',
       'value': t64 / tmx, 'unit': 'x speed-up (plain FP64 / mixed)',
-      'n_gpus': 1, 'higher_is_better': True, 'dtype': 'fp32+fp64',
-      'data': 'synthetic (blurred Gaussian images, seed 0)',
       'config': {'workload': f'krr_myrtle5_32x32_train{ntr}_test{nte}',
-                 'diag_reg': 1e-4, 'get': 'ntk'},
+                 'diag_reg': 1e-4, 'get': 'ntk', 'targets': 10},
       'plain_fp64_ms': t64 * 1e3, 'mixed_ms': tmx * 1e3,
+      'plain_fp64_tflops': chol_flop / t64 / 1e12,
+      'plain_fp64_note': 'torch.linalg.cholesky_ex + cholesky_solve in FP64 '
+                         '(rocSOLVER); tflops counts n^3/3 over the whole '
+                         'solve + predict time',
       'pcg_iterations': iters, 'final_rel_residual': resid,
+      'fell_back_to_fp64': bool(smx.impl.fell_back),
       'rel_err_alpha_vs_fp64': err_a, 'rel_err_ftest_vs_fp64': err_f,
       'kernel_seconds': t_kernel,
   }
-  print(json.dumps(out), flush=True)
+
+
+def m10_result(steps):
+  """Myrtle-10 (Conv+Relu blocks [3, 3, 3]) at 32x32, x1 [128], x2 [512]:
+  pairs/s and per-kernel GB/s (algorithmic bytes over HIP-event time)."""
+  import torch
+  from neural_tangents_amd import stax
+  from neural_tangents_amd.data import blurred_images
+  depths, side, n1, n2 = WORKLOADS['myrtle10_32x32_128x512']
+  a, b = blurred_images([n1, n2], side, seed=0)
+  dev = torch.device('cuda', 0)
+  x1, x2 = torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)
+  kernel_fn = stax.myrtle(depths=list(depths))[2]
+  elapsed, per, _ = timed_kernel_fn(kernel_fn, x1, x2, steps, 1)
+  value = n1 * n2 * steps / elapsed
+  bpp = canonical_bytes_per_pair(depths, side)
+  log(f'myrtle10: {value:.0f} pairs/s')
+  return {'workload': 'myrtle10_32x32_128x512', 'pairs_per_s': value,
+          'ms_per_step': elapsed / steps * 1e3, 'steps': steps,
+          'path_gbs_canonical': bpp * value / 1e9,
+          'path_pct_hbm_peak': 100.0 * bpp * value / 1e9 / HBM_PEAK_GBS,
+          'kernels': kernel_table(per, steps)}
+
+
+def grad_result():
+  """Differentiation form: loss 0.5 mean((K_ts (K_ss + lambda I)^-1 Y_s -
+  Y_t)^2) of the Myrtle-5 32x32 NTK with 100 support and 2,048 target
+  images, gradients with respect to the support images, W_std and b_std
+  (one forward + backward, wall time)."""
+  import torch
+  from neural_tangents_amd import grad
+  from neural_tangents_amd.data import blurred_images, one_hot_targets
+  dev = torch.device('cuda', 0)
+  xs, xt = blurred_images([100, 2048], 32, seed=5)
+  ys = torch.from_numpy(one_hot_targets(100, seed=6)).to(dev)
+  yt = torch.from_numpy(one_hot_targets(2048, seed=7)).to(dev)
+  builder = grad.myrtle_builder((2, 1, 1))
+
+  def run(ns, nt):
+    xs_t = torch.tensor(xs[:ns], device=dev, requires_grad=True)
+    W = torch.tensor(2 ** 0.5, dtype=torch.float64, device=dev,
+                     requires_grad=True)
+    b = torch.tensor(0.1, dtype=torch.float64, device=dev,
+                     requires_grad=True)
+    loss = grad.krr_loss(xs_t, ys[:ns], torch.tensor(xt[:nt], device=dev),
+                         yt[:nt], W, b, builder, diag_reg=1e-4)
+    loss.backward()
+    torch.cuda.synchronize()
+    return float(loss)
+
+  run(4, 8)                                    # warm-up
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  loss = run(100, 2048)
+  dt = time.perf_counter() - t0
+  log(f'grad: {dt:.1f} s')
+  pairs = 100 * 2048 + 100 * 101 // 2
+  return {'workload': 'grad_myrtle5_32x32_support100_target2048',
+          'seconds': dt, 'loss': loss, 'kernel_pairs': pairs,
+          'pairs_per_s_fwd_bwd': pairs / dt}
 
 
 # How the algorithmic bytes of a launch relate to what it moves.
@@ -165,24 +330,35 @@ SCHEDULE_NOTES = {
         'counts the per-layer schedule\'s bytes for those two kernels (the '
         'first stage state written and read back, the pooled state written); '
         'the kernel never materialises the first stage, so traffic (measured '
-        'HBM bytes) is far below them and its bound is the vector ALU',
+        'HBM bytes) is far below them; its bound is the vector ALU '
+        '(valu_busy_frac, from rocprofv3 counters)',
 }
+
+
+def valu_busy(kernel):
+  d = _load_json(VALU_FILE)
+  try:
+    return d['kernels'][kernel]['valu_busy_frac']
+  except (TypeError, KeyError):
+    return None
 
 
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument('--gpus', type=int, default=1)
-  ap.add_argument('--steps', type=int, default=3)
-  ap.add_argument('--warmup', type=int, default=1)
+  ap.add_argument('--steps', type=int, default=10)
+  ap.add_argument('--warmup', type=int, default=2)
   ap.add_argument('--workload', default='myrtle5_32x32_256x256',
                   choices=sorted(WORKLOADS))
   ap.add_argument('--no-cpu', action='store_true')
-  ap.add_argument('--cpu-sample', type=int, default=14)
+  ap.add_argument('--no-krr', action='store_true')
+  ap.add_argument('--no-m10', action='store_true')
+  ap.add_argument('--no-grad', action='store_true')
   ap.add_argument('--pmc-calibration', action='store_true',
                   help='first copy 256 MiB with torch (the reference launch '
                   'that scales FETCH_SIZE / WRITE_SIZE counters)')
   ap.add_argument('--krr', action='store_true',
-                  help='kernel-regression solve benchmark instead')
+                  help='only the kernel-regression solve benchmark')
   ap.add_argument('--krr-train', type=int, default=8192)
   ap.add_argument('--krr-test', type=int, default=1024)
   ap.add_argument('--krr-reps', type=int, default=3)
@@ -193,11 +369,16 @@ def main():
           'design notes).', file=sys.stderr)
     sys.exit(2)
   if args.krr:
-    krr_bench(args)
+    out = krr_result(args.krr_train, args.krr_test, args.krr_reps)
+    out.update({'n_gpus': 1, 'higher_is_better': True,
+                'dtype': 'fp32+fp64',
+                'data': 'synthetic (blurred Gaussian images, seed 0)'})
+    print(json.dumps(out), flush=True)
     return
 
+  import numpy as np
   import torch
-  from neural_tangents_amd import _engine, stax
+  from neural_tangents_amd import stax
   from neural_tangents_amd.data import blurred_images
 
   if args.pmc_calibration:
@@ -214,28 +395,9 @@ def main():
   x2 = torch.from_numpy(x2n).to(dev)
   _, _, kernel_fn = stax.myrtle(depths=list(depths))
 
-  for _ in range(args.warmup):
-    kernel_fn(x1, x2, ('nngp', 'ntk'))
-  torch.cuda.synchronize()
-
-  _engine.TRACE = []
-  torch.cuda.synchronize()
-  t0 = time.perf_counter()
-  for _ in range(args.steps):
-    k = kernel_fn(x1, x2, ('nngp', 'ntk'))
-  torch.cuda.synchronize()
-  elapsed = time.perf_counter() - t0
-  trace, _engine.TRACE = _engine.TRACE, None
-
-  # Per-entry-point launch statistics (HIP events on the launch stream).
-  per = {}
-  for name, s, e, nbytes, pairs in trace:
-    d = per.setdefault(name, {'ms': 0.0, 'launches': 0, 'bytes': 0,
-                              'pairs': 0})
-    d['ms'] += s.elapsed_time(e)
-    d['launches'] += 1
-    d['bytes'] += nbytes
-    d['pairs'] += pairs
+  log(f'{args.workload}: {args.warmup} warm-up + {args.steps} timed steps')
+  elapsed, per, k = timed_kernel_fn(kernel_fn, x1, x2, args.steps,
+                                    args.warmup)
   dom_name = max(per, key=lambda n: per[n]['ms'])
   dom = per[dom_name]
   avg_ms = dom['ms'] / dom['launches']
@@ -247,6 +409,9 @@ def main():
   ms_per_step = elapsed / args.steps * 1e3
   value = pairs * args.steps / elapsed
   bpp = canonical_bytes_per_pair(depths, side)
+  log(f'{args.workload}: {value:.0f} pairs/s')
+  fused = dom_name == 'nta_first_pool_stage'
+  vb = valu_busy(dom_name) if fused else None
 
   out = {
       'metric': 'Myrtle NTK pairs/sec + %HBM peak',
@@ -266,22 +431,41 @@ def main():
                  'get': ['nngp', 'ntk'], 'parallelism': 'single GPU'},
       'path_gbs_canonical': bpp * value / 1e9,
       'path_pct_hbm_peak': 100.0 * bpp * value / 1e9 / HBM_PEAK_GBS,
-      'roofline': {'bound': 'hbm', 'kernel': dom_name,
+      'roofline': {'bound': 'valu' if fused else 'hbm', 'kernel': dom_name,
                    'achieved': achieved, 'peak': HBM_PEAK_GBS,
                    'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
                    'traffic': traffic, 'avg_launch_ms': avg_ms,
                    'alg_bytes_per_launch': avg_bytes,
+                   'valu_busy_frac': vb,
                    'note': SCHEDULE_NOTES.get(dom_name, '')},
-      'kernels': {n: {'ms_per_step': v['ms'] / args.steps,
-                      'launches_per_step': v['launches'] / args.steps,
-                      'avg_launch_ms': v['ms'] / v['launches'],
-                      'gbs': v['bytes'] / (v['ms'] * 1e-3) / 1e9
-                      if v['ms'] > 0 else None}
-                  for n, v in per.items()},
+      'kernels': kernel_table(per, args.steps),
   }
+  got = (k.nngp.cpu().numpy(), k.ntk.cpu().numpy())
   del k
+  ti = tile_rows(kernel_fn, x1, x2)
+
+  if not args.no_m10 and args.workload == 'myrtle5_32x32_256x256':
+    out['myrtle10'] = m10_result(3)
+    torch.cuda.empty_cache()
+  if not args.no_krr and args.workload == 'myrtle5_32x32_256x256':
+    out['krr'] = krr_result(args.krr_train, args.krr_test, args.krr_reps)
+    torch.cuda.empty_cache()
+  if not args.no_grad and args.workload == 'myrtle5_32x32_256x256':
+    out['grad'] = grad_result()
+    torch.cuda.empty_cache()
   if not args.no_cpu:
-    out['cpu_baseline'] = cpu_baseline(depths, x1n, x2n, args.cpu_sample)
+    # Error block: rows on both sides of the first row-tile boundary and in
+    # the last (tail) row tile; first, middle and last columns.
+    er = sorted({0, ti - 1, ti, n1 - ti // 2, n1 - 1})
+    ec = sorted({0, n2 // 2, n2 - 2, n2 - 1})
+    rows = 2 * host_threads()
+    log(f'cpu leg: {rows}x48 FP32 sample, FP64 error block {er} x {ec}')
+    base, err = cpu_leg(depths, x1n, x2n, got, rows, 48, er, ec)
+    out['cpu_baseline'] = base
+    out['rel_err_vs_oracle'] = {
+        'nngp': err['nngp'], 'ntk': err['ntk'], 'oracle': 'FP64 numpy',
+        'rows': er, 'cols': ec, 'row_tile': ti,
+        'note': 'rows span the first pair-tile boundary and the tail tile'}
   print(json.dumps(out), flush=True)
 
 

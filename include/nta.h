@@ -265,7 +265,8 @@ int nta_chol_precond(const float* W, const float* WT, int n, long long ldw,
  * *iters receives the iteration count and *resid the
  * largest relative residual of the returned X, recomputed from scratch
  * (host pointers, may be NULL).  Synchronises the stream once per
- * iteration to test convergence.
+ * iteration to test convergence (reading 16 doubles reduced on the
+ * device).
  * Replaces: cho_solve (predict.py:1254) of the regularised train kernel.
  */
 int nta_pcg64(const float* K, int n, long long ldk, double lam,
@@ -273,6 +274,20 @@ int nta_pcg64(const float* K, int n, long long ldk, double lam,
               const double* B, int nrhs, double* X, double tol, int max_iter,
               void* work, long long work_bytes, int* iters, double* resid,
               void* stream);
+
+/*
+ * nta_pcg64 for a kernel matrix held in FP64 (K [n][ldk] fp64, ldk even):
+ * the residual operator (K + lam I) reads the FP64 entries, so the solution
+ * is that of the FP64 system even where K is not representable in fp32;
+ * W, WT are the FP32 factor of fp32(K) + shift I as for nta_pcg64.
+ * Replaces: cho_solve (predict.py:1254) of the regularised train kernel
+ * given in float64.
+ */
+int nta_pcg64_k64(const double* K, int n, long long ldk, double lam,
+                  const float* W, const float* WT, long long ldw, int n1,
+                  const double* B, int nrhs, double* X, double tol,
+                  int max_iter, void* work, long long work_bytes, int* iters,
+                  double* resid, void* stream);
 
 #ifdef __cplusplus
 }

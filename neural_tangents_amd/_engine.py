@@ -116,9 +116,10 @@ def compile_plan(specs, in_shape) -> Plan:
   for spec in specs:
     if head is not None:
       if isinstance(spec, S._DenseSpec):
-        dense.append(spec)
+        dense.append((spec, chans))
         is_gaussian = True
-        chans = spec.out_dim
+        chans = spec.out_dim * (spec.s[1] if spec.parameterization ==
+                                'standard' else 1)
         continue
       if isinstance(spec, S._IdentitySpec):
         continue
@@ -263,11 +264,19 @@ def compile_plan(specs, in_shape) -> Plan:
     a = pend * (float(h * h * w * w) if gsum else 1.0)
     ntk_t = a
     pend = 1.0
-  for spec in dense:
+  for spec, fan_in in dense:
     wv = float(spec.W_std) ** 2
     bv = 0.0 if spec.b_std is None else float(spec.b_std) ** 2
-    a, b = wv * a, wv * b + bv
-    ntk_m, ntk_t, ntk_1 = a + wv * ntk_m, wv * ntk_t, b + wv * ntk_1
+    if spec.parameterization == 'standard':
+      # ntk' = (fan_in / s0) nngp + W^2 ntk (+ 1 with a bias), from the
+      # layer's input nngp (linear.py:913-918)
+      iw = fan_in / spec.s[0]
+      ntk_m, ntk_t = iw * a + wv * ntk_m, wv * ntk_t
+      ntk_1 = iw * b + wv * ntk_1 + (1.0 if spec.b_std is not None else 0.0)
+      a, b = wv * a, wv * b + bv
+    else:
+      a, b = wv * a, wv * b + bv
+      ntk_m, ntk_t, ntk_1 = a + wv * ntk_m, wv * ntk_t, b + wv * ntk_1
   coef = (a, b, ntk_m, ntk_t, ntk_1)
   diag = head == 'flatten' and all(s.pool is None for s in stages)
   return Plan(stages=stages, head=head, head_coef=coef, n_convs=n_convs,

@@ -3,15 +3,22 @@ FP64 residuals, preconditioned conjugate gradients.
 
 For ``A = K + lambda I`` (``predict.py:1224-1258``):
 
-1. ``nta_chol_f32`` factors ``K32 + shift I`` in FP32 by recursive blocking
-   on fp32 MFMA and inverts the factor on the way; the preconditioner
-   ``(L L^T)^-1`` is applied from the blocks it leaves (``nta_chol_precond``).
+1. ``nta_chol_f32`` factors ``fp32(K) + shift I`` in FP32 by recursive
+   blocking on fp32 MFMA and inverts the factor on the way; the
+   preconditioner ``(L L^T)^-1`` is applied from the blocks it leaves
+   (``nta_chol_precond``).
 2. ``nta_pcg64`` runs conjugate gradients on ``A`` with FP64 vectors:
-   ``A p`` reads the fp32 kernel (exact in FP64), accumulates in FP64 and
-   adds the FP64 ``lambda``, so the converged solution is the FP64 solution
-   of the regularised system; the FP32 factor only steers convergence.
+   ``A p`` reads the kernel as given -- fp32 (exact in FP64) or, for a
+   float64 kernel, its FP64 entries (``nta_pcg64_k64``) -- accumulates in
+   FP64 and adds the FP64 ``lambda``, so the converged solution is the FP64
+   solution of the regularised system; the FP32 factor only steers
+   convergence.
+3. A solve whose final relative residual (recomputed from scratch) stays
+   above ``max(100 tol, 1e-8)`` falls back to the FP64 Cholesky solve with a
+   ``RuntimeWarning``; a result is never returned unverified.
 """
 import ctypes
+import warnings
 
 import torch
 
@@ -33,10 +40,13 @@ class MixedCholesky:
     self.max_iter = max_iter
     self.iterations = 0
     self.last_residual = None
+    self.fell_back = False
+    self._L64 = None
     npad = -(-n // self.NB) * self.NB
     self.npad = npad
     self.n1 = _lib.value('nta_chol_split', npad)
     stream = _lib.stream_handle(self.device)
+    self.k64 = K.dtype == torch.float64
     if npad == n and K.dtype == torch.float32 and K.is_contiguous():
       K32 = K
     else:
@@ -44,7 +54,20 @@ class MixedCholesky:
       K32[:n, :n] = K.to(torch.float32)
       if npad > n:                                # identity padding
         K32.diagonal()[n:] = 1.0
-    self.K32 = K32                               # kept for FP64 residuals
+    if self.k64:
+      # The residual operator reads the FP64 entries: fp32(K) would change
+      # the system by up to cond(A) 2^-24 in the solution.
+      if npad == n and K.is_contiguous():
+        Kop = K
+      else:
+        Kop = torch.zeros((npad, npad), dtype=torch.float64,
+                          device=self.device)
+        Kop[:n, :n] = K
+        if npad > n:
+          Kop.diagonal()[n:] = 1.0
+    else:
+      Kop = K32
+    self.Kop = Kop                               # FP64 residual operator
     # The preconditioner factors K32 + shift I.  When FP32 breaks down (a
     # non-positive pivot: the kernel is too ill-conditioned for single
     # precision at this lambda), the shift grows by 10x; the FP64-residual
@@ -118,6 +141,10 @@ class MixedCholesky:
     self.last_residual = res
     return torch.cat(chunks, dim=1) if len(chunks) > 1 else chunks[0]
 
+  def accept_residual(self):
+    """Largest final relative residual a refined solve may return."""
+    return max(100.0 * self.tol, 1e-8)
+
   def _solve16(self, b):
     B = self._pad(b).contiguous()
     k = B.shape[1]
@@ -125,10 +152,32 @@ class MixedCholesky:
     ws = self._workspace(k)
     iters = ctypes.c_int(0)
     resid = ctypes.c_double(0.0)
-    _lib.call('nta_pcg64', _lib.ptr(self.K32), self.npad, self.npad,
+    _lib.call('nta_pcg64_k64' if self.k64 else 'nta_pcg64',
+              _lib.ptr(self.Kop), self.npad, self.npad,
               self.lam, _lib.ptr(self.W), _lib.ptr(self.WT), self.npad,
               self.n1, _lib.ptr(B), k,
               _lib.ptr(X), self.tol, self.max_iter, _lib.ptr(ws), ws.numel(),
               ctypes.byref(iters), ctypes.byref(resid),
               _lib.stream_handle(self.device))
+    if not resid.value <= self.accept_residual():
+      warnings.warn(
+          f'mixed-precision solve stopped at relative residual '
+          f'{resid.value:.3g} after {iters.value} iterations (limit '
+          f'{self.accept_residual():.3g}); using the FP64 Cholesky solve '
+          'instead.', RuntimeWarning, stacklevel=3)
+      self.fell_back = True
+      return self._fp64_solve(b), iters.value, resid.value
     return X[:self.n], iters.value, resid.value
+
+  def _fp64_solve(self, b):
+    """Plain FP64 Cholesky solve of (K + lambda I) x = b (the fallback)."""
+    if self._L64 is None:
+      A = self.Kop[:self.n, :self.n].to(torch.float64).clone()
+      A.diagonal().add_(self.lam)
+      L, info = torch.linalg.cholesky_ex(A)
+      if int(info) != 0:
+        raise ValueError(
+            f'Cholesky factorisation failed (leading minor {int(info)} is '
+            'not positive definite); increase diag_reg.')
+      self._L64 = L
+    return torch.cholesky_solve(b.to(torch.float64), self._L64)

@@ -791,57 +791,71 @@ __device__ __forceinline__ void tri_range(int tri, int i0, int k, int& kb,
 // of a [k][RHS] right-hand side: lane l holds A[row l%16][4 (l/16) + kk]
 // and X[4 (l/16) + kk][rhs l%16] for MFMA kk (the reduction index is
 // permuted identically in both operands).
-template <typename TX>
+// TK: element type of the matrix operand (float, or double for an FP64
+// kernel matrix in the residual operator).
+template <typename TK, typename TX>
 struct Chunk {
-  float4 a;
+  TK a[4];
   TX x[4];
 };
 
-template <typename TX>
-__device__ __forceinline__ void load_chunk(Chunk<TX>& c, bool ok,
-                                           const float* __restrict__ arow,
+template <typename TK, typename TX>
+__device__ __forceinline__ void load_chunk(Chunk<TK, TX>& c, bool ok,
+                                           const TK* __restrict__ arow,
                                            const TX* __restrict__ xcol,
                                            int j0) {
   if (ok) {
-    c.a = *(const float4*)(arow + j0);
+    if constexpr (std::is_same<TK, float>::value) {
+      const float4 v = *(const float4*)(arow + j0);
+      c.a[0] = v.x;
+      c.a[1] = v.y;
+      c.a[2] = v.z;
+      c.a[3] = v.w;
+    } else {
+      const double2 v0 = *(const double2*)(arow + j0);
+      const double2 v1 = *(const double2*)(arow + j0 + 2);
+      c.a[0] = v0.x;
+      c.a[1] = v0.y;
+      c.a[2] = v1.x;
+      c.a[3] = v1.y;
+    }
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk) c.x[kk] = xcol[(long long)(j0 + kk) * RHS];
   } else {
-    c.a = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) c.x[kk] = (TX)0;
+    for (int kk = 0; kk < 4; ++kk) {
+      c.a[kk] = (TK)0;
+      c.x[kk] = (TX)0;
+    }
   }
 }
 
+template <typename TK>
 __device__ __forceinline__ void mfma_chunk(doublex4& acc,
-                                           const Chunk<double>& c) {
-  acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)c.a.x, c.x[0], acc, 0,
-                                             0, 0);
-  acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)c.a.y, c.x[1], acc, 0,
-                                             0, 0);
-  acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)c.a.z, c.x[2], acc, 0,
-                                             0, 0);
-  acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)c.a.w, c.x[3], acc, 0,
-                                             0, 0);
+                                           const Chunk<TK, double>& c) {
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk)
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)c.a[kk], c.x[kk], acc,
+                                               0, 0, 0);
 }
 
 __device__ __forceinline__ void mfma_chunk(floatx4& acc,
-                                           const Chunk<float>& c) {
-  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(c.a.x, c.x[0], acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(c.a.y, c.x[1], acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(c.a.z, c.x[2], acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(c.a.w, c.x[3], acc, 0, 0, 0);
+                                           const Chunk<float, float>& c) {
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk)
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(c.a[kk], c.x[kk], acc, 0, 0,
+                                               0);
 }
 
 // acc += A[16 rows, chunks c0, c0 + step, ...] X over the chunk indices
 // c0 + step * i < c1 (chunks of 16 columns), software-pipelined: the loads
 // of the next group of 4 chunks are in flight while the current group's
 // MFMAs run.
-template <typename TX, typename ACC>
+template <typename TK, typename TX, typename ACC>
 __device__ __forceinline__ void row_stripe_product(
-    ACC& acc, const float* __restrict__ arow, const TX* __restrict__ xcol,
+    ACC& acc, const TK* __restrict__ arow, const TX* __restrict__ xcol,
     int c0, int c1, int step) {
-  Chunk<TX> cur[4], nxt[4];
+  Chunk<TK, TX> cur[4], nxt[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int c = c0 + step * u;
@@ -863,14 +877,15 @@ __device__ __forceinline__ void row_stripe_product(
 
 // (K + lam I) P in FP64 for 32 rows per workgroup of 8 waves: wave w
 // computes rows [i0 + 16 (w & 1), +16) over the 16-column chunks
-// c = w >> 1 (mod 4) on v_mfma_f64_16x16x4f64 (K read in fp32, exact in
-// fp64); the four partial tiles of a row stripe are added through LDS in a
-// fixed order.
+// c = w >> 1 (mod 4) on v_mfma_f64_16x16x4f64 (K read in fp32 -- exact in
+// fp64 -- or in fp64); the four partial tiles of a row stripe are added
+// through LDS in a fixed order.
 // mode 0: Q = (K + lam I) P, part[wg][c] = sum_rows P Q.
 // mode 1: D = B - (K + lam I) P; R = D, R32 = D (when R is given),
 //         part[wg][c] = sum_rows D^2.
+template <typename TK>
 __global__ __launch_bounds__(512) void k_symv_rows(
-    const float* __restrict__ K, long long ldk, int n, double lam, int mode,
+    const TK* __restrict__ K, long long ldk, int n, double lam, int mode,
     const double* __restrict__ P, double* __restrict__ Q,
     const double* __restrict__ B, double* __restrict__ R,
     float* __restrict__ R32, double* __restrict__ part) {
@@ -980,6 +995,16 @@ __device__ __forceinline__ void block_slot_sums(const double* __restrict__ s,
     out[t] = a;
   }
   __syncthreads();
+}
+
+// out[c] = sum over nslot partials [slot][RHS] of column c (one block of
+// 256 threads, fixed order), so the host reads RHS values per test.
+__global__ __launch_bounds__(256) void k_colsum(const double* __restrict__ s,
+                                                int nslot,
+                                                double* __restrict__ out) {
+  __shared__ double scratch[256], res[RHS];
+  block_slot_sums(s, nslot, scratch, res);
+  if (threadIdx.x < RHS) out[threadIdx.x] = res[threadIdx.x];
 }
 
 // P_new = Z + beta P_old with the Polak-Ribiere (flexible CG) coefficient
@@ -1112,7 +1137,7 @@ __global__ __launch_bounds__(256) void k_pad32(int n, const float* __restrict__ 
 
 // ------------------------------------------------------------ host side
 struct Ws {
-  double *b, *x, *r, *p0, *p1, *q, *pq, *rz0, *rz1, *rr, *bb, *rzx;
+  double *b, *x, *r, *p0, *p1, *q, *pq, *rz0, *rz1, *rr, *bb, *rzx, *sums;
   float *r32, *z32, *y1, *u2, *y2, *v1;
   size_t bytes;
 };
@@ -1140,6 +1165,7 @@ static Ws carve(char* base, int n) {
   w.rr = (double*)take(slots);
   w.bb = (double*)take(slots);
   w.rzx = (double*)take(slots);
+  w.sums = (double*)take(RHS * sizeof(double));
   w.r32 = (float*)take(v32);
   w.z32 = (float*)take(v32);
   w.y1 = (float*)take(v32);
@@ -1190,7 +1216,8 @@ static void precond(const float* W, const float* WT, long long ld, int n,
        st);                                            // z1 = W11^T v1
 }
 
-static int pcg_run(const float* K, int n, long long ldk, double lam,
+template <typename TK>
+static int pcg_run(const TK* K, int n, long long ldk, double lam,
                    const float* W, const float* WT, long long ldw, int n1,
                    const double* Bin, int nrhs, double* Xout, double tol,
                    int max_iter, char* work, int* iters, double* resid,
@@ -1199,19 +1226,16 @@ static int pcg_run(const float* K, int n, long long ldk, double lam,
   const int nvec = (n + VROWS - 1) / VROWS;    // vector-kernel workgroups
   const int nsym = n / 32;                     // k_symv_rows workgroups
   const int nrz = n / 16;                      // rz slots
-  std::vector<double> host((size_t)(n / 16 + nvec) * RHS);
+  // Column sums of per-workgroup partials, reduced on the device; the host
+  // reads RHS doubles per convergence test.
   auto col_sums = [&](const double* dev, int nslot, double* out) -> int {
-    hipError_t e = hipMemcpyAsync(host.data(), dev,
-                                  (size_t)nslot * RHS * sizeof(double),
+    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, st, dev, nslot,
+                       w.sums);
+    hipError_t e = hipMemcpyAsync(out, w.sums, RHS * sizeof(double),
                                   hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess)
       return set_error((int)e, "nta_pcg64: %s", hipGetErrorString(e));
-    for (int c = 0; c < RHS; ++c) {
-      double s = 0.0;
-      for (int k = 0; k < nslot; ++k) s += host[(size_t)k * RHS + c];
-      out[c] = s;
-    }
     return 0;
   };
   hipLaunchKernelGGL(k_pcg_io, dim3(nvec), dim3(256), 0, st, n, nrhs, Bin,
@@ -1228,8 +1252,8 @@ static int pcg_run(const float* K, int n, long long ldk, double lam,
   };
   // r <- b - A x (FP64), replacing the recurrence residual when R is given.
   auto true_residual = [&](double* R, float* R32) -> int {
-    hipLaunchKernelGGL(k_symv_rows, dim3(nsym), dim3(512), 0, st, K, ldk, n,
-                       lam, 1, w.x, nullptr, w.b, R, R32, w.rr);
+    hipLaunchKernelGGL(k_symv_rows<TK>, dim3(nsym), dim3(512), 0, st, K,
+                       ldk, n, lam, 1, w.x, nullptr, w.b, R, R32, w.rr);
     return col_sums(w.rr, nsym, rn);
   };
   precond(W, WT, ldw, n, n1, w.r32, w.z32, w.r, w.rz0, w, st);
@@ -1244,8 +1268,9 @@ static int pcg_run(const float* K, int n, long long ldk, double lam,
     hipLaunchKernelGGL(k_pcg_p, dim3(nvec), dim3(256), 0, st, n, w.z32,
                        p_old, p_new, rz_new, rzs[it & 1], nrz, w.rzx, nvec,
                        it == 1);
-    hipLaunchKernelGGL(k_symv_rows, dim3(nsym), dim3(512), 0, st, K, ldk, n,
-                       lam, 0, p_new, w.q, nullptr, nullptr, nullptr, w.pq);
+    hipLaunchKernelGGL(k_symv_rows<TK>, dim3(nsym), dim3(512), 0, st, K,
+                       ldk, n, lam, 0, p_new, w.q, nullptr, nullptr, nullptr,
+                       w.pq);
     hipLaunchKernelGGL(k_pcg_xr, dim3(nvec), dim3(256), 0, st, n, w.pq, nsym,
                        rz_new, nrz, p_new, w.q, w.z32, w.x, w.r, w.r32, w.rr,
                        w.rzx);
@@ -1359,6 +1384,24 @@ extern "C" int nta_pcg64(const float* K, int n, long long ldk, double lam,
       !X || !work)
     return set_error(NTA_EINVAL, "nta_pcg64: bad ldk=%lld max_iter=%d or "
                      "null pointer", ldk, max_iter);
-  return pcg_run(K, n, ldk, lam, W, WT, ldw, n1, B, nrhs, X, tol, max_iter,
-                 (char*)work, iters, resid, (hipStream_t)stream);
+  return pcg_run<float>(K, n, ldk, lam, W, WT, ldw, n1, B, nrhs, X, tol,
+                        max_iter, (char*)work, iters, resid,
+                        (hipStream_t)stream);
+}
+
+extern "C" int nta_pcg64_k64(const double* K, int n, long long ldk,
+                             double lam, const float* W, const float* WT,
+                             long long ldw, int n1, const double* B,
+                             int nrhs, double* X, double tol, int max_iter,
+                             void* work, long long work_bytes, int* iters,
+                             double* resid, void* stream) {
+  int rc = check_solve_args("nta_pcg64_k64", n, ldw, n1, nrhs, work_bytes);
+  if (rc) return rc;
+  if (ldk < n || ldk % 2 != 0 || max_iter < 1 || !K || !W || !WT || !B ||
+      !X || !work)
+    return set_error(NTA_EINVAL, "nta_pcg64_k64: bad ldk=%lld max_iter=%d "
+                     "or null pointer", ldk, max_iter);
+  return pcg_run<double>(K, n, ldk, lam, W, WT, ldw, n1, B, nrhs, X, tol,
+                         max_iter, (char*)work, iters, resid,
+                         (hipStream_t)stream);
 }

@@ -288,12 +288,17 @@ class KernelFunction(torch.autograd.Function):
     gx1, gx2, gwv, gbv = kernel_vjp(
         plan, x1c, None if ctx.sym else x2.to(torch.float32).contiguous(),
         g_nngp, g_ntk)
-    gW = 2.0 * ctx.W * sum(gwv)
-    gb = 2.0 * ctx.b * sum(gbv)
     dev = x1.device
-    return (gx1.to(x1.dtype), None if ctx.sym else gx2.to(x2.dtype),
-            torch.tensor(gW, dtype=torch.float64, device=dev),
-            torch.tensor(gb, dtype=torch.float64, device=dev), None)
+    need = ctx.needs_input_grad
+    # Gradients only for the inputs that require them: W_std / b_std passed
+    # as Python floats (fixed hyperparameters) get None.
+    gW = torch.tensor(2.0 * ctx.W * sum(gwv), dtype=torch.float64,
+                      device=dev) if need[2] else None
+    gb = torch.tensor(2.0 * ctx.b * sum(gbv), dtype=torch.float64,
+                      device=dev) if need[3] else None
+    gx1 = gx1.to(x1.dtype) if need[0] else None
+    gx2 = None if ctx.sym or not need[1] else gx2.to(x2.dtype)
+    return gx1, gx2, gW, gb, None
 
 
 def myrtle_builder(depths=(2, 1, 1), readout_W_std=1.0, readout_b_std=0.0):

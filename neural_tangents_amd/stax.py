@@ -92,6 +92,7 @@ class _DenseSpec:
   W_std: float
   b_std: Optional[float]
   parameterization: str = 'ntk'
+  s: tuple = (1, 1)
   name: str = 'Dense'
 
 
@@ -153,18 +154,34 @@ def _conv_layer_fns(spec: _ConvSpec):
 
 
 def _dense_layer_fns(spec: _DenseSpec):
+  """``Dense`` init / apply (``linear.py:846-895``): ntk parameterization
+  draws N(0, 1) weights and scales the output by ``W_std / sqrt(fan_in)``;
+  standard parameterization draws ``out_dim * s[1]`` outputs with weights
+  N(0, W_std^2 / (fan_in / s[0])), biases N(0, b_std^2), and divides the
+  product by ``sqrt(s[0])``."""
+  standard = spec.parameterization == 'standard'
+  s0, s1 = spec.s
+
   def init_fn(rng, input_shape):
     g = _as_generator(rng)
     c = input_shape[-1]
-    W = torch.randn((c, spec.out_dim), generator=g, dtype=torch.float64)
-    b = None if spec.b_std is None else torch.randn((spec.out_dim,),
-                                                    generator=g,
+    out = spec.out_dim * s1 if standard else spec.out_dim
+    W = torch.randn((c, out), generator=g, dtype=torch.float64)
+    b = None if spec.b_std is None else torch.randn((out,), generator=g,
                                                     dtype=torch.float64)
-    return tuple(input_shape[:-1]) + (spec.out_dim,), (W, b)
+    if standard:
+      W = W * (spec.W_std / math.sqrt(c / s0))
+      b = None if b is None else b * spec.b_std
+    return tuple(input_shape[:-1]) + (out,), (W, b)
 
   def apply_fn(params, inputs, **kwargs):
     W, b = params
     W = W.to(inputs.device, inputs.dtype)
+    if standard:
+      y = inputs @ W / math.sqrt(s0)
+      if b is not None:
+        y = y + b.to(y.device, y.dtype)
+      return y
     y = inputs @ W * (spec.W_std / math.sqrt(W.shape[0]))
     if b is not None:
       y = y + spec.b_std * b.to(y.device, y.dtype)
@@ -411,6 +428,9 @@ def Conv(out_chan: int, filter_shape: Sequence[int],
   padding = padding.upper()
   if padding not in ('SAME', 'VALID', 'CIRCULAR'):
     raise ValueError(f'Unknown padding {padding!r}.')
+  parameterization = parameterization.lower()
+  if parameterization not in ('ntk', 'standard'):
+    raise ValueError(f'Parameterization not supported: {parameterization}')
   return _layer(_ConvSpec(out_chan, filter_shape, strides, padding,
                           float(W_std), None if b_std is None else
                           float(b_std), parameterization))
@@ -499,10 +519,14 @@ def GlobalSumPool(batch_axis: int = 0, channel_axis: int = -1):
 def Dense(out_dim: int, W_std: float = 1., b_std: Optional[float] = None,
           batch_axis: int = 0, channel_axis: int = -1,
           parameterization: str = 'ntk', s=(1, 1)):
-  """``Dense`` (``linear.py:762-931``)."""
+  """``Dense`` (``linear.py:762-931``); ``parameterization`` is ``'ntk'``
+  or ``'standard'`` (``s``: the width scales of the standard form)."""
+  parameterization = parameterization.lower()
+  if parameterization not in ('ntk', 'standard'):
+    raise ValueError(f'Parameterization not supported: {parameterization}')
   return _layer(_DenseSpec(int(out_dim), float(W_std),
                            None if b_std is None else float(b_std),
-                           parameterization))
+                           parameterization, tuple(int(v) for v in s)))
 
 
 def Flatten(batch_axis: int = 0, batch_axis_out: int = 0):

@@ -16,6 +16,10 @@ baseline), never as the thing measured or shipped.
   standardised Gaussian images; one-hot targets).
 
 Each function cites the reference file:line it follows.  What pins each
-function (closed forms, brute force, the reference's own known answers,
-Monte-Carlo agreement with finite networks) lives in ``tests/test_oracle_*.py``.
+function lives in ``tests/test_oracle_pins.py`` (closed forms, brute force,
+the reference's own known answers, Gaussian quadrature, library routines,
+the gradient-flow ODE and matrix-exponential forms of the predictors) and
+``tests/test_oracle_mc.py`` (Monte-Carlo agreement of the NNGP / NTK
+recursions with empirical kernels of finite networks, the reference's own
+validation method, with planted mistakes shown to fail).
 """

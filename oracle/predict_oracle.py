@@ -73,3 +73,39 @@ def ensemble_mean_finite_t(k_train_train, y_train, k_test_train, t,
     return evecs @ (-expm1[:, None] * proj)
   fac = -expm1 / np.abs(evals)
   return np.asarray(k_test_train, np.float64) @ (evecs @ (fac[:, None] * proj))
+
+
+def ensemble_cov_finite_t(k_train_train_g, nngp_train_train, k_test_train_g,
+                          k_test_train_nngp, nngp_test_test, t, g,
+                          size, learning_rate=1.0, diag_reg=0.0,
+                          diag_reg_absolute_scale=False):
+  """Finite-time posterior covariance of ``gradient_descent_mse_ensemble``
+  (``predict.py:1038-1090``) for one time ``t``: ``g`` is the kernel that
+  drives training ('nngp' or 'ntk'), ``size`` is ``y_train.size`` (the
+  normaliser of ``expm1``, 1277-1294).  ``k_test_train_g is None`` gives the
+  training-set covariance (1043-1069), else the test set (1072-1090)."""
+  A = add_diagonal_regularizer(np.asarray(k_train_train_g, np.float64),
+                               diag_reg, diag_reg_absolute_scale)
+  evals, V = np.linalg.eigh(A)
+  tt = float(t) * learning_rate
+  lam = np.maximum(evals, 0.0)
+
+  def inv_expm1(s):
+    return np.expm1(-lam * s / size) / np.abs(evals)
+
+  nngp_dd = np.asarray(nngp_train_train, np.float64)
+  if k_test_train_g is None:
+    if g == 'nngp':
+      return (V * (lam * np.exp(-2 * lam * tt / size))) @ V.T
+    ex = (V * np.exp(-lam * tt / size)) @ V.T
+    return ex @ nngp_dd @ ex.T
+  ktd = np.asarray(k_test_train_g, np.float64)
+  ntt = np.asarray(nngp_test_test, np.float64)
+  if g == 'nngp':
+    return ntt - ktd @ ((V * -inv_expm1(2 * tt)) @ V.T) @ ktd.T
+  neg = -inv_expm1(tt)
+  term_1 = (V * neg) @ V.T @ ktd.T                     # [N, T]
+  term_2 = ktd @ ((V * neg) @ V.T) @ np.asarray(k_test_train_nngp,
+                                                np.float64).T
+  term_2 = term_2 + term_2.T
+  return term_1.T @ nngp_dd @ term_1 - term_2 + ntt

@@ -564,14 +564,23 @@ def _affine(mat, W_std, b_std):
   return mat
 
 
-def dense_layer(W_std=1.0, b_std=None):
-  """``Dense`` ``linear.py:762-931``, ``kernel_fn`` 902-926 (ntk param.)."""
+def dense_layer(W_std=1.0, b_std=None, parameterization='ntk',
+                input_width=None):
+  """``Dense`` ``linear.py:762-931``, ``kernel_fn`` 902-926.  The
+  ``'standard'`` parameterization (913-918) needs ``input_width`` =
+  ``shape1[channel_axis] / s[0]`` of the layer's input."""
   def kernel_fn(k: OKernel) -> OKernel:
+    ntk = k.ntk
+    if parameterization == 'standard':
+      if ntk is not None:
+        ntk = k.nngp.dtype.type(input_width) * k.nngp + k.nngp.dtype.type(
+            W_std ** 2) * ntk
+        if b_std is not None:
+          ntk = ntk + 1.0
     cov1 = _affine(k.cov1, W_std, b_std)
     nngp = _affine(k.nngp, W_std, b_std)
     cov2 = _affine(k.cov2, W_std, b_std)
-    ntk = k.ntk
-    if ntk is not None:
+    if parameterization == 'ntk' and ntk is not None:
       ntk = nngp + nngp.dtype.type(W_std ** 2) * ntk
     return k.replace(cov1=cov1, nngp=nngp, cov2=cov2, ntk=ntk,
                      is_gaussian=True)

@@ -107,3 +107,75 @@ def test_non_spd_raises():
     _solve.CholeskySolver(K, 0.0, True, 'mixed')
   with pytest.raises(ValueError):
     _solve.CholeskySolver(K, 0.0, True, 'fp64')
+
+
+def _spectrum_kernel(n, cond, seed):
+  """FP64 SPD kernel with eigenvalues log-spaced over [1/cond, 1] and a
+  random orthogonal basis: its entries are not fp32-representable."""
+  g = torch.Generator(device='cuda').manual_seed(seed)
+  Q, _ = torch.linalg.qr(torch.randn(n, n, generator=g, device='cuda',
+                                     dtype=torch.float64))
+  ev = torch.logspace(0, -np.log10(cond), n, dtype=torch.float64,
+                      device='cuda')
+  K = (Q * ev) @ Q.T
+  return 0.5 * (K + K.T)
+
+
+def test_mixed_solver_on_fp64_kernel_solves_the_fp64_system():
+  """A float64 kernel with cond ~ 1e6 whose entries are not
+  fp32-representable: the mixed solve reads the FP64 entries in its residual
+  operator (nta_pcg64_k64), so it matches the FP64 solve of the same system
+  (the solution of fp32(K) differs from it by ~cond * 2^-24)."""
+  from neural_tangents_amd import _solve
+  n = 1024
+  K = _spectrum_kernel(n, 1e6, seed=5)
+  assert float((K.float().double() - K).abs().max()) > 0
+  y = torch.randn(n, 10, dtype=torch.float64, device='cuda',
+                  generator=torch.Generator(device='cuda').manual_seed(2))
+  s = _solve.CholeskySolver(K, 1e-7, True, 'mixed')
+  x = s.solve(y)
+  assert s.impl.k64 and not s.impl.fell_back
+  want = _solve.CholeskySolver(K, 1e-7, True, 'fp64').solve(y)
+  err = rel_err(x.cpu().numpy(), want.cpu().numpy())
+  assert err <= 1e-8, err
+  # solving the fp32-rounded system instead is visibly different
+  x32 = _solve.CholeskySolver(K.float(), 1e-7, True, 'fp64').solve(y)
+  assert rel_err(x32.cpu().numpy(), want.cpu().numpy()) > 100 * max(err,
+                                                                    1e-12)
+
+
+def test_mixed_solver_falls_back_when_refinement_stops_short():
+  """max_iter = 1 cannot reach the tolerance: the solve warns and returns
+  the FP64 Cholesky solution instead of the unconverged iterate."""
+  from neural_tangents_amd import _solve
+  n = 512
+  K = _spectrum_kernel(n, 1e5, seed=6).float()
+  y = torch.randn(n, 3, dtype=torch.float64, device='cuda',
+                  generator=torch.Generator(device='cuda').manual_seed(4))
+  s = _solve.CholeskySolver(K, 1e-6, True, 'mixed', max_iter=1)
+  with pytest.warns(RuntimeWarning, match='FP64 Cholesky'):
+    x = s.solve(y)
+  assert s.impl.fell_back
+  assert s.last_residual > s.impl.accept_residual()
+  want = _solve.CholeskySolver(K, 1e-6, True, 'fp64').solve(y)
+  assert rel_err(x.cpu().numpy(), want.cpu().numpy()) <= 1e-12
+
+
+def test_mixed_solve_n8192_at_myrtle_conditioning():
+  """n = 8192 (the kernel-regression workload's train size) with the
+  condition number of its Myrtle-5 NTK at diag_reg = 1e-4 (~2.6e6): the
+  mixed solve matches the FP64 Cholesky solve at <= 1e-6 (alpha and a test
+  prediction K_td alpha)."""
+  from neural_tangents_amd import _solve
+  n = 8192
+  K = _spectrum_kernel(n, 2.6e6, seed=7).float()
+  y = torch.from_numpy(one_hot_targets(n, seed=8)).double().cuda()
+  s = _solve.CholeskySolver(K, 0.0, True, 'mixed')
+  a = s.solve(y)
+  want = _solve.CholeskySolver(K, 0.0, True, 'fp64').solve(y)
+  err = rel_err(a.cpu().numpy(), want.cpu().numpy())
+  print(f'n=8192 mixed vs fp64: rel err {err:.2e}, {s.iterations} iters, '
+        f'residual {s.last_residual:.2e}', flush=True)
+  assert err <= 1e-6
+  ktd = K[:1024].double()
+  assert rel_err((ktd @ a).cpu().numpy(), (ktd @ want).cpu().numpy()) <= 1e-6

@@ -469,3 +469,134 @@ def test_circular_full_vs_diagonal_spatial():
   diag = so.kernel_fn(layers, x1, x2, ('nngp', 'ntk'), diagonal_spatial=True)
   for g in ('nngp', 'ntk'):
     np.testing.assert_allclose(full[g], diag[g], rtol=1e-12)
+
+
+# ------------------------------------------ posterior covariance, finite t
+
+def _spd(rng, n, shift=0.5):
+  A = rng.standard_normal((n, n))
+  return A @ A.T / n + shift * np.eye(n)
+
+
+def _joint(rng, nd, nt):
+  """A joint (test + train) NNGP covariance and an NTK of the same shape."""
+  Kj = _spd(rng, nd + nt, 0.3)
+  Tj = _spd(rng, nd + nt, 0.6) + 0.5 * Kj
+  sl_t, sl_d = slice(0, nt), slice(nt, nt + nd)
+  return Kj, Tj, sl_t, sl_d
+
+
+def test_gp_inference_cov_nngp_is_schur_complement():
+  """NNGP posterior covariance = inverse of the test block of the inverse
+  joint covariance (the Schur complement), with the diagonal regulariser on
+  the train block; numpy block inverses, not a solve."""
+  rng = np.random.default_rng(11)
+  nd, nt = 12, 5
+  Kj, _, st, sd = _joint(rng, nd, nt)
+  reg = 1e-2
+  lam = reg * np.trace(Kj[sd, sd]) / nd
+  J = Kj.copy()
+  J[sd, sd] += lam * np.eye(nd)
+  want = np.linalg.inv(np.linalg.inv(J)[st, st])
+  got = po.gp_inference_cov(Kj[sd, sd], Kj[sd, sd], Kj[st, sd], Kj[st, sd],
+                            Kj[st, st], 'nngp', diag_reg=reg)
+  np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-11)
+
+
+def test_gp_inference_cov_ntk_is_linear_map_covariance():
+  """NTK posterior covariance = Cov[f_inf] for the infinite-time
+  gradient-descent output f_inf = f0_test - T_td T_dd^-1 (f0_train - y) with
+  f0 ~ N(0, joint NNGP): M J M^T for M = [I, -T_td T_dd^-1].  With the NTK
+  equal to the NNGP (and no regulariser) it reduces to the NNGP branch."""
+  rng = np.random.default_rng(12)
+  nd, nt = 10, 4
+  Kj, Tj, st, sd = _joint(rng, nd, nt)
+  reg = 3e-3
+  lam = reg * np.trace(Tj[sd, sd]) / nd
+  Treg = Tj[sd, sd] + lam * np.eye(nd)
+  M = np.concatenate([np.eye(nt), -np.linalg.solve(Treg, Tj[sd, st]).T],
+                     axis=1)
+  Jn = np.block([[Kj[st, st], Kj[st, sd]], [Kj[sd, st], Kj[sd, sd]]])
+  want = M @ Jn @ M.T
+  got = po.gp_inference_cov(Kj[sd, sd], Tj[sd, sd], Kj[st, sd], Tj[st, sd],
+                            Kj[st, st], 'ntk', diag_reg=reg)
+  np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-11)
+  same_n = po.gp_inference_cov(Kj[sd, sd], Kj[sd, sd], Kj[st, sd],
+                               Kj[st, sd], Kj[st, st], 'nngp')
+  same_t = po.gp_inference_cov(Kj[sd, sd], Kj[sd, sd], Kj[st, sd],
+                               Kj[st, sd], Kj[st, st], 'ntk')
+  np.testing.assert_allclose(same_t, same_n, rtol=1e-9, atol=1e-11)
+
+
+def test_finite_t_mean_solves_gradient_flow():
+  """Finite-t mean = solution of the gradient-flow ODE of the MSE loss,
+  df_train/dt = -(lr / y.size) A (f_train - y), df_test/dt = -(lr / y.size)
+  K_td (f_train - y), f(0) = 0 (scipy solve_ivp, no eigendecomposition), at
+  three times, with the regulariser and a learning rate."""
+  import scipy.integrate as si
+  rng = np.random.default_rng(13)
+  nd, nt, k = 9, 4, 3
+  Kd = _spd(rng, nd)
+  Ktd = rng.standard_normal((nt, nd)) * 0.5
+  y = rng.standard_normal((nd, k))
+  reg, lr = 1e-2, 0.7
+  A = Kd + reg * np.trace(Kd) / nd * np.eye(nd)
+  size = y.size
+
+  def rhs(_, z):
+    f = z[:nd * k].reshape(nd, k)
+    d = -(lr / size) * A @ (f - y)
+    dt = -(lr / size) * Ktd @ (f - y)
+    return np.concatenate([d.ravel(), dt.ravel()])
+
+  ts = [3.0, 40.0, 400.0]
+  sol = si.solve_ivp(rhs, (0, ts[-1]), np.zeros(nd * k + nt * k),
+                     t_eval=ts, rtol=1e-11, atol=1e-13, method='DOP853')
+  for i, t in enumerate(ts):
+    f_tr = sol.y[:nd * k, i].reshape(nd, k)
+    f_te = sol.y[nd * k:, i].reshape(nt, k)
+    got_te = po.ensemble_mean_finite_t(Kd, y, Ktd, t, learning_rate=lr,
+                                       diag_reg=reg)
+    got_tr = po.ensemble_mean_finite_t(Kd, y, None, t, learning_rate=lr,
+                                       diag_reg=reg)
+    np.testing.assert_allclose(got_te, f_te, rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose(got_tr, f_tr, rtol=1e-7, atol=1e-9)
+
+
+@pytest.mark.parametrize('g', ['nngp', 'ntk'])
+def test_finite_t_cov_is_linear_map_covariance(g):
+  """Finite-t covariance = Cov[f(t)] of the gradient-flow output, a linear
+  map of the prior sample f0 ~ N(0, joint NNGP) (train block regularised
+  when the NNGP drives training): f_train(t) = E f0_d + (I - E) y,
+  f_test(t) = f0_t - T_td T^-1 (I - E)(f0_d - y), E = expm(-T t lr / size)
+  (scipy.linalg.expm, no eigendecomposition)."""
+  import scipy.linalg as sl
+  rng = np.random.default_rng(14 if g == 'nngp' else 15)
+  nd, nt = 8, 3
+  Kj, Tj, st, sd = _joint(rng, nd, nt)
+  if g == 'nngp':
+    Tj = Kj
+  reg, lr, size = 2e-2, 1.3, nd * 2
+  lam = reg * np.trace(Tj[sd, sd]) / nd
+  T = Tj[sd, sd] + lam * np.eye(nd)
+  Kdd = Kj[sd, sd] + (lam * np.eye(nd) if g == 'nngp' else 0.0)
+  for t in (0.5, 6.0, 60.0):
+    E = sl.expm(-T * t * lr / size)
+    got_tr = po.ensemble_cov_finite_t(Tj[sd, sd], Kj[sd, sd], None, None,
+                                      None, t, g, size, lr, reg)
+    np.testing.assert_allclose(got_tr, E @ Kdd @ E.T, rtol=1e-8, atol=1e-11)
+    M = np.concatenate(
+        [np.eye(nt), -Tj[st, sd] @ np.linalg.solve(T, np.eye(nd) - E)],
+        axis=1)
+    J = np.block([[Kj[st, st], Kj[st, sd]], [Kj[sd, st], Kdd]])
+    got_te = po.ensemble_cov_finite_t(Tj[sd, sd], Kj[sd, sd], Tj[st, sd],
+                                      Kj[st, sd], Kj[st, st], t, g, size, lr,
+                                      reg)
+    np.testing.assert_allclose(got_te, M @ J @ M.T, rtol=1e-8, atol=1e-11)
+  # t -> inf: the infinite-time (gp_inference) covariance
+  big = po.ensemble_cov_finite_t(Tj[sd, sd], Kj[sd, sd], Tj[st, sd],
+                                 Kj[st, sd], Kj[st, st], 1e9, g, size, lr,
+                                 reg)
+  inf = po.gp_inference_cov(Kj[sd, sd], Tj[sd, sd], Kj[st, sd], Tj[st, sd],
+                            Kj[st, st], g, diag_reg=reg)
+  np.testing.assert_allclose(big, inf, rtol=1e-7, atol=1e-9)

@@ -176,3 +176,36 @@ def test_circular_padding_flags_and_no_fusion():
   assert p.stages[0].flags(True) & _lib.CONV_CIRCULAR
   assert not p.stages[0].flags(True) & _lib.CONV_VALID
   assert not _engine._fuses_first_two(p, (1, 32, 32, 3))
+
+
+def test_standard_dense_head_coefficients_match_oracle():
+  """Dense(parameterization='standard') in the readout fold equals the
+  oracle's dense layers applied to a scalar kernel (nngp = m, ntk = mt),
+  with the fan-in taken from the last Conv's out_chan and the s scales."""
+  import numpy as np
+  from oracle import stax_oracle as so
+  p = _plan(stax.Conv(6, (3, 3), padding='SAME'), stax.GlobalAvgPool(),
+            stax.Dense(5, 1.3, 0.4, parameterization='standard', s=(2, 3)),
+            stax.Dense(2, 0.7, 0.2),
+            stax.Dense(1, 0.9, None, parameterization='STANDARD'))
+  a, b, c, d, e = p.head_coef
+  m, mt = 0.37, 1.91
+  k = so.OKernel(nngp=np.array(m), ntk=np.array(mt), cov1=np.array(m),
+                 cov2=None, x1_is_x2=False, is_gaussian=False,
+                 is_reversed=False, diagonal_batch=True,
+                 diagonal_spatial=False, shape1=None, shape2=None)
+  k = so.dense_layer(1.3, 0.4, 'standard', input_width=6 / 2)(k)
+  k = so.dense_layer(0.7, 0.2)(k)
+  k = so.dense_layer(0.9, None, 'standard', input_width=2)(k)
+  assert abs(a * m + b - k.nngp) < 1e-12
+  assert abs(c * m + d * mt + e - k.ntk) < 1e-12
+
+
+def test_unknown_parameterization_raises_value_error():
+  with pytest.raises(ValueError):
+    stax.Dense(1, parameterization='mup')
+  with pytest.raises(ValueError):
+    stax.Conv(1, (3, 3), parameterization='xyz')
+  with pytest.raises(NotImplementedError):
+    _plan(stax.Conv(1, (3, 3), padding='SAME', parameterization='standard'),
+          stax.GlobalAvgPool())
