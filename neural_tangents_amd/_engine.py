@@ -84,20 +84,34 @@ class _Unsupported(NotImplementedError):
   pass
 
 
-def compile_plan(specs, in_shape) -> Plan:
-  """Groups layer specs into fused stages; validates shapes and options."""
+def compile_plan(specs, in_shape, from_state: bool = False,
+                 in_gaussian: bool = False) -> Plan:
+  """Groups layer specs into fused stages; validates shapes and options.
+
+  ``in_shape`` is NHWC ``[N, H, W, C]`` (a flat ``[N, D]`` input is passed as
+  ``[N, 1, 1, D]``).  ``from_state``: the network is applied to a given
+  kernel (``kernel_fn(Kernel)``, ``stax/requirements.py:955-975``) whose
+  full-spatial state has spatial size ``H x W``, instead of to inputs; then
+  no stage computes an input covariance and ``in_gaussian`` is the input
+  kernel's ``is_gaussian``.
+
+  ``Dense`` before a readout acts on the channel axis at every spatial
+  position, so on kernels it is a 1x1 Conv without the axis reversal
+  (``linear.py:902-926`` is the same affine map as ``linear.py:1387-1398``
+  with a 1x1 filter)."""
   from . import stax as S
   if len(in_shape) != 4:
     raise NotImplementedError(
-        'The HIP engine runs NHWC image inputs [N, H, W, C]; got input shape '
-        f'{tuple(in_shape)}.')
+        'The HIP engine runs NHWC image inputs [N, H, W, C] or flat inputs '
+        f'[N, D]; got input shape {tuple(in_shape)}.')
   _, H, W, C = in_shape
   stages: List[Stage] = []
   cur: Optional[Stage] = None
   head = None
   dense = []
-  is_gaussian = False
+  is_gaussian = bool(in_gaussian) if from_state else False
   n_convs = 0
+  n_affine = 0
   chans = C
   # SumPool = AvgPool x window^2 on kernels.  Every layer here is linear or
   # positively homogeneous of degree 1 in (nngp, ntk, cov) jointly (the
@@ -147,22 +161,40 @@ def compile_plan(specs, in_shape) -> Plan:
       b_var = 0.0 if spec.b_std is None else float(spec.b_std) ** 2
       cur = Stage(conv=(fh, fw, spec.padding == 'VALID', w_var, b_var),
                   circular=spec.padding == 'CIRCULAR',
-                  first=not stages and n_convs == 0,
+                  first=not from_state and not stages and n_affine == 0,
                   wscale=w_var / float(spec.W_std) ** 2
                   if spec.W_std != 0 else 1.0)
-      if not cur.first and not stages:
+      if not from_state and not cur.first and not stages:
         raise NotImplementedError('The network must start with a Conv.')
       n_convs += 1
+      n_affine += 1
       is_gaussian = True
       chans = spec.out_chan
+      continue
+    if isinstance(spec, S._DenseSpec):
+      if spec.parameterization != 'ntk':
+        raise NotImplementedError('Dense before a readout runs with the ntk '
+                                  'parameterization only.')
+      close()
+      w_var = float(spec.W_std) ** 2 * pend
+      pend = 1.0
+      b_var = 0.0 if spec.b_std is None else float(spec.b_std) ** 2
+      cur = Stage(conv=(1, 1, False, w_var, b_var),
+                  first=not from_state and not stages and n_affine == 0,
+                  wscale=w_var / float(spec.W_std) ** 2
+                  if spec.W_std != 0 else 1.0)
+      n_affine += 1
+      is_gaussian = True
+      chans = spec.out_dim
       continue
     if isinstance(spec, (S._ReluSpec, S._ActSpec)) and not is_gaussian:
       raise ValueError('The input to the activation function must be '
                        'Gaussian, i.e. a random affine transform is '
                        'required before the activation function.')
-    if not stages and cur is None:
+    if not from_state and not stages and cur is None:
       raise NotImplementedError(
-          f'The HIP engine needs a Conv as the first layer, got {spec.name}.')
+          'The HIP engine needs a Conv or Dense as the first layer, got '
+          f'{spec.name}.')
     if isinstance(spec, S._ReluSpec):
       if cur is None or cur.relu or cur.pool is not None:
         close()
@@ -217,18 +249,14 @@ def compile_plan(specs, in_shape) -> Plan:
       if isinstance(spec, S._FlattenSpec):
         is_gaussian = False
       continue
-    if isinstance(spec, S._DenseSpec):
-      raise NotImplementedError('Dense before a readout (on spatial '
-                                'kernels) is not supported by the HIP '
-                                'engine.')
     raise NotImplementedError(f'Layer {spec.name} is not supported by the '
                               'HIP engine.')
   close()
-  if not stages:
-    raise NotImplementedError('The network has no Conv layer.')
+  if not stages and not from_state:
+    raise NotImplementedError('The network has no Conv or Dense layer.')
 
   # The first stage carries only Conv (+ Relu); its pool becomes a stage.
-  if stages[0].pool is not None:
+  if stages and stages[0].first and stages[0].pool is not None:
     p = stages[0].pool
     stages[0].pool = None
     stages.insert(1, Stage(pool=p))
@@ -278,7 +306,8 @@ def compile_plan(specs, in_shape) -> Plan:
       a, b = wv * a, wv * b + bv
       ntk_m, ntk_t, ntk_1 = a + wv * ntk_m, wv * ntk_t, b + wv * ntk_1
   coef = (a, b, ntk_m, ntk_t, ntk_1)
-  diag = head == 'flatten' and all(s.pool is None for s in stages)
+  diag = (not from_state and head == 'flatten' and
+          all(s.pool is None for s in stages))
   return Plan(stages=stages, head=head, head_coef=coef, n_convs=n_convs,
               is_gaussian=is_gaussian, out_channels=chans,
               out_spatial=(h, w), diag=diag, out_scale=pend)
@@ -314,7 +343,8 @@ def _state_elems(h, w):
 
 
 def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
-               collect_q=None, head_out=None, ld=0, keep=None, qn=None):
+               collect_q=None, head_out=None, ld=0, keep=None, qn=None,
+               init=None):
   """Runs all stages for one tile.  Self tile when ``n2 == 0`` (then
   ``collect_q`` receives the normaliser of each ReLU stage, computed from the
   self state on the way).  Returns the final (nngp, ntk) state when the plan
@@ -322,12 +352,21 @@ def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
   (a list) receives the output state of every stage (the checkpoints the
   adjoint recomputes from).  ``qn`` = (first-ReLU normaliser tables of the
   tile's x1 and x2 images, then the second ReLU's), from ``nta_qnorm``,
-  selects the fused launch of the first two stages (cross tiles only)."""
-  dev = xa.device
+  selects the fused launch of the first two stages (cross tiles only).
+  ``init`` = (nngp, ntk) input state of the tile ([T, H, H, W, W], ntk may
+  be None for zero) for plans compiled ``from_state``; then ``xa`` and
+  ``xb`` are unused."""
   T = n1 if n2 == 0 else n1 * n2
-  _, H, W, C = xa.shape
   nngp = ntk = None
   fl_ntk = want_ntk and n2 != 0
+  if init is not None:
+    nngp, ntk = init
+    dev = nngp.device
+    if fl_ntk and ntk is None:
+      ntk = torch.zeros_like(nngp)
+  else:
+    dev = xa.device
+    _, H, W, C = xa.shape
   skip = 0
   if qn is not None and n2 != 0:
     s0, s1 = plan.stages[0], plan.stages[1]
@@ -408,11 +447,11 @@ def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
   a, b, c, d, e = plan.head_coef
   hflags = (_lib.FLATTEN if plan.head == 'flatten' else 0) | (
       _lib.NTK if fl_ntk else 0)
-  last = plan.stages[-1]
+  ho, wo = plan.out_spatial
   out_n, out_t = head_out
   nt = 2 if fl_ntk else 1
   _launch('nta_head', T * nt * 4 * (nngp[0].numel() + 1), T, _lib.ptr(nngp),
-          _lib.ptr(ntk), n1, n2, last.hout, last.wout, hflags, a, b, c, d, e,
+          _lib.ptr(ntk), n1, n2, ho, wo, hflags, a, b, c, d, e,
           _lib.ptr(out_n), _lib.ptr(out_t), ld, stream)
   return None
 
@@ -436,8 +475,9 @@ def _fuses_first_two(plan: Plan, in_shape) -> bool:
           and s1.pool == (2, 2))
 
 
-def _per_pair_bytes(plan: Plan, ntk: bool, fused: bool = False) -> int:
-  m = 0
+def _per_pair_bytes(plan: Plan, ntk: bool, fused: bool = False,
+                    in_hw=None) -> int:
+  m = 0 if in_hw is None else _state_elems(*in_hw)
   for si, s in enumerate(plan.stages):
     if fused and si < 2:
       # the first stage's state and the second conv's output are never
@@ -482,6 +522,115 @@ def _self_chain(plan: Plan, x, want_cov, tile_bytes, stream):
   if plan.head is None and want_cov:
     cov = torch.cat(full) if len(full) > 1 else full[0]
   return qs, cov
+
+
+def _self_chain_state(plan: Plan, cov, want_cov, tile_bytes, stream):
+  """``_self_chain`` for a plan compiled ``from_state``: the self chains
+  start from the per-image states ``cov`` [N, H, H, W, W]."""
+  N, H = cov.shape[0], cov.shape[1]
+  W = cov.shape[3]
+  per = _per_pair_bytes(plan, False, in_hw=(H, W))
+  chunk = max(1, min(N, tile_bytes // per))
+  q_parts = [[] for _ in plan.stages]
+  out = None
+  if plan.head is not None:
+    out = torch.empty((N,), dtype=torch.float32, device=cov.device)
+  full = []
+  for i0 in range(0, N, chunk):
+    n = min(chunk, N - i0)
+    init = (cov[i0:i0 + n].contiguous(), None)
+    collect = [None] * len(plan.stages)
+    if plan.head is not None:
+      _run_chain(plan, None, None, n, 0, None, None, False, stream,
+                 collect_q=collect, head_out=(out[i0:i0 + n], None), ld=0,
+                 init=init)
+    else:
+      st, _ = _run_chain(plan, None, None, n, 0, None, None, False, stream,
+                         collect_q=collect, init=init)
+      if want_cov:
+        full.append(st if st is not init[0] else st.clone())
+    for si, q in enumerate(collect):
+      if q is not None:
+        q_parts[si].append(q)
+  qs = [torch.cat(p) if len(p) > 1 else (p[0] if p else None)
+        for p in q_parts]
+  if plan.head is None and want_cov:
+    out = torch.cat(full) if len(full) > 1 else full[0]
+  return qs, out
+
+
+def run_state(plan: Plan, nngp, ntk, cov1, cov2, want_ntk, want_cov,
+              tile_bytes=DEFAULT_TILE_BYTES):
+  """Applies a plan compiled ``from_state`` to a kernel: ``nngp``, ``ntk``
+  [N1, N2, H, H, W, W] (natural axis order; ``ntk`` None for zero), per-image
+  states ``cov1`` [N1, H, H, W, W] and ``cov2`` [N2, ...] or None (x2 is x1).
+  Returns the dict of ``run``."""
+  dev = nngp.device
+  stream = _lib.stream_handle(dev)
+  N1, N2, H = nngp.shape[0], nngp.shape[1], nngp.shape[2]
+  W = nngp.shape[4]
+  sym = cov2 is None
+  q1s, c1 = _self_chain_state(plan, cov1, want_cov, tile_bytes, stream)
+  if sym:
+    q2s, c2 = q1s, None
+  else:
+    q2s, c2 = _self_chain_state(plan, cov2, want_cov, tile_bytes, stream)
+  per = _per_pair_bytes(plan, want_ntk, in_hw=(H, W)) + 4 * _state_elems(
+      H, W) * (2 if want_ntk else 1)
+  T = max(1, tile_bytes // per)
+  if sym and plan.head is not None and T < N1 * N2:
+    ti = tj = max(1, min(N1, math.isqrt(T)))
+  else:
+    tj = min(N2, T)
+    ti = max(1, min(N1, T // tj))
+  ho, wo = plan.out_spatial
+  if plan.head is not None:
+    out_n = torch.empty((N1, N2), dtype=torch.float32, device=dev)
+    out_t = torch.empty((N1, N2), dtype=torch.float32, device=dev) \
+        if want_ntk else None
+  else:
+    shp = (N1, N2, ho, ho, wo, wo)
+    out_n = torch.empty(shp, dtype=torch.float32, device=dev)
+    out_t = torch.empty(shp, dtype=torch.float32, device=dev) \
+        if want_ntk else None
+  tri = sym and plan.head is not None and ti == tj
+  for i0 in range(0, N1, ti):
+    n1 = min(ti, N1 - i0)
+    for j0 in range(0, N2, tj):
+      n2 = min(tj, N2 - j0)
+      if tri and j0 + n2 <= i0:
+        continue
+      qa = [None if q is None else q[i0:i0 + n1] for q in q1s]
+      qb = [None if q is None else q[j0:j0 + n2] for q in q2s]
+      sl = (slice(i0, i0 + n1), slice(j0, j0 + n2))
+      init_n = nngp[sl].reshape((n1 * n2,) + tuple(nngp.shape[2:]))
+      init_t = None
+      if want_ntk and ntk is not None:
+        init_t = ntk[sl].reshape((n1 * n2,) + tuple(ntk.shape[2:]))
+      init = (init_n.contiguous(), None if init_t is None else
+              init_t.contiguous())
+      if plan.head is not None:
+        _run_chain(plan, None, None, n1, n2, qa, qb, want_ntk, stream,
+                   head_out=(out_n[i0:, j0:],
+                             out_t[i0:, j0:] if want_ntk else None),
+                   ld=N2, init=init)
+      else:
+        st_n, st_t = _run_chain(plan, None, None, n1, n2, qa, qb, want_ntk,
+                                stream, init=init)
+        sh = st_n.shape[1:]
+        out_n[sl] = st_n.view((n1, n2) + sh)
+        if want_ntk:
+          out_t[sl] = st_t.view((n1, n2) + sh)
+  if tri:
+    _symmetrize_from_upper(out_n)
+    if want_ntk:
+      _symmetrize_from_upper(out_t)
+  out = {'cov1': c1, 'cov2': c2, 'nngp': out_n, 'ntk': out_t}
+  if plan.out_scale != 1.0:
+    for k in ('nngp', 'ntk', 'cov1', 'cov2'):
+      if out[k] is not None:
+        out[k] = out[k] * plan.out_scale
+  return out
 
 
 def _diag_tables(plan: Plan):

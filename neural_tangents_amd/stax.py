@@ -328,18 +328,31 @@ class AnalyticKernelFn:
     self._plans = {}
     self.__name__ = 'kernel_fn'
 
-  def plan(self, in_shape):
-    key = tuple(in_shape[1:])
+  def plan(self, in_shape, from_state=False, in_gaussian=False):
+    if len(in_shape) == 2:
+      self._check_flat()
+      in_shape = (in_shape[0], 1, 1, in_shape[1])
+    key = tuple(in_shape[1:]) + (bool(from_state), bool(in_gaussian))
     if key not in self._plans:
-      self._plans[key] = _engine.compile_plan(self.specs, (1,) + key)
+      self._plans[key] = _engine.compile_plan(
+          self.specs, (1,) + tuple(in_shape[1:]), from_state=from_state,
+          in_gaussian=in_gaussian)
     return self._plans[key]
+
+  def _check_flat(self):
+    for spec in self.specs:
+      if isinstance(spec, (_ConvSpec, _AvgPoolSpec)):
+        raise ValueError(f'{spec.name} needs spatial inputs [N, H, W, C]; '
+                         'got flat inputs [N, D].')
 
   def __call__(self, x1_or_kernel, x2=None, get=None, *, pattern=None,
                mask_constant=None, diagonal_batch=None,
                diagonal_spatial=None, tile_bytes=None, **kwargs):
     if isinstance(x1_or_kernel, Kernel):
-      raise NotImplementedError('Applying kernel_fn to a Kernel is not '
-                                'supported by the HIP engine.')
+      if x2 is not None:
+        raise ValueError('`x2` must be None when `kernel_fn` is applied to '
+                         'a `Kernel`.')
+      return self._apply_to_kernel(x1_or_kernel, get, tile_bytes)
     if pattern is not None or mask_constant is not None:
       raise NotImplementedError('pattern / mask_constant are not supported '
                                 'by the HIP engine.')
@@ -355,7 +368,15 @@ class AnalyticKernelFn:
     if x2t is not None and tuple(x2t.shape[1:]) != tuple(x1.shape[1:]):
       raise ValueError(f'x1 and x2 must agree apart from the batch axis, got '
                        f'{tuple(x1.shape)} and {tuple(x2t.shape)}.')
-    plan = self.plan(x1.shape)
+    flat = x1.ndim == 2
+    shape1_in = tuple(x1.shape)
+    shape2_in = tuple((x1 if x2t is None else x2t).shape)
+    if flat:
+      # [N, D] -> [N, 1, 1, D]: Dense layers run as 1x1 stages.
+      x1 = x1.reshape(x1.shape[0], 1, 1, x1.shape[1])
+      if x2t is not None:
+        x2t = x2t.reshape(x2t.shape[0], 1, 1, x2t.shape[1])
+    plan = self.plan(shape1_in)
     if diagonal_spatial and not plan.diag:
       # The diagonal-spatial path runs when the network allows it (a Flatten
       # readout and no pooling); other networks need the full spatial
@@ -370,10 +391,13 @@ class AnalyticKernelFn:
     is_rev = plan.head is None and plan.n_convs % 2 == 1
 
     def order(t, bnd):
-      if t is None or not is_rev:
+      if t is None:
         return t
-      # natural [.., h, h', w, w'] -> reference [.., w, w', h, h']
-      return t.permute(tuple(range(bnd)) + (bnd + 2, bnd + 3, bnd, bnd + 1))
+      if flat and plan.head is None:
+        return t.reshape(t.shape[:bnd])
+      if not is_rev:
+        return t
+      return _swap_spatial_pairs(t, bnd)
 
     if x2t is None:
       x1_is_x2 = True
@@ -383,14 +407,18 @@ class AnalyticKernelFn:
       # a 0-d device tensor, like the reference's jnp bool (utils.py:255-282);
       # evaluated without a host synchronisation.
       x1_is_x2 = torch.all(torch.abs(x1 - x2t) < 1e-12)
-    shape1 = self._out_shape(tuple(x1.shape))
-    shape2 = self._out_shape(tuple((x1 if x2t is None else x2t).shape))
+    shape1 = self._out_shape(shape1_in)
+    shape2 = self._out_shape(shape2_in)
     k = Kernel(nngp=order(res['nngp'], 2), ntk=order(res['ntk'], 2),
                cov1=order(res['cov1'], 1), cov2=order(res['cov2'], 1),
                x1_is_x2=x1_is_x2, is_gaussian=plan.is_gaussian,
                is_reversed=is_rev, is_input=False, diagonal_batch=True,
                diagonal_spatial=False, shape1=shape1, shape2=shape2,
                batch_axis=0, channel_axis=len(shape1) - 1)
+    return self._format(k, get, return_kernel, get_t)
+
+  @staticmethod
+  def _format(k, get, return_kernel, get_t):
     if return_kernel:
       return k
     if isinstance(get, str):
@@ -398,11 +426,114 @@ class AnalyticKernelFn:
     return _namedtuple('AnalyticKernel', get_t)(
         *[getattr(k, g) for g in get_t])
 
+  def _apply_to_kernel(self, k: Kernel, get, tile_bytes):
+    """``kernel_fn(kernel)``: the network applied to a given ``Kernel``
+    (``kernel_fn_kernel``, ``stax/requirements.py:955-975``), e.g. the output
+    of another ``kernel_fn`` with spatial axes.  The state is taken in the
+    reference's layout (``[N1, N2, X, X', Y, Y']`` or ``[N1, N2]``, reversed
+    when ``is_reversed``) and the result is laid out the same way."""
+    return_kernel, get_t = _canonicalize_get(get)
+    for g in get_t:
+      if g not in Kernel.__dataclass_fields__:
+        raise ValueError(f'Unknown `get` entry {g!r}.')
+    if k.diagonal_spatial:
+      raise NotImplementedError('Kernels with diagonal_spatial=True are not '
+                                'supported as kernel_fn inputs.')
+    if not k.diagonal_batch:
+      raise NotImplementedError('Kernels with diagonal_batch=False are not '
+                                'supported as kernel_fn inputs.')
+
+    def dev(t):
+      if t is None:
+        return None
+      if isinstance(t, np.ndarray):
+        t = torch.from_numpy(np.ascontiguousarray(t))
+      if not torch.cuda.is_available():
+        raise RuntimeError('kernel_fn runs on the GPU through libnta.so and '
+                           'no GPU is visible.')
+      d = t.device if t.is_cuda else torch.device('cuda', 0)
+      return t.to(device=d, dtype=torch.float32)
+
+    nngp = dev(k.nngp)
+    if nngp.ndim == 2:
+      flat = True
+      N1, N2 = nngp.shape
+      H = W = 1
+    elif nngp.ndim == 6:
+      flat = False
+      N1, N2 = nngp.shape[:2]
+    else:
+      raise NotImplementedError('kernel_fn(Kernel) takes kernels with two '
+                                f'spatial axes or none; got nngp of shape '
+                                f'{tuple(nngp.shape)}.')
+    if flat:
+      self._check_flat()
+
+    def natural(t, bnd):
+      if t is None:
+        return None
+      if flat:
+        return t.reshape(tuple(t.shape[:bnd]) + (1, 1, 1, 1)).contiguous()
+      t = _swap_spatial_pairs(t, bnd) if k.is_reversed else t
+      return t.contiguous()
+
+    nngp = natural(nngp, 2)
+    if not flat:
+      H, W = nngp.shape[2], nngp.shape[4]
+    ntk_in = dev(k.ntk) if isinstance(k.ntk, torch.Tensor) or isinstance(
+        k.ntk, np.ndarray) else k.ntk
+    want_ntk = 'ntk' in get_t and ntk_in is not None
+    if isinstance(ntk_in, (int, float)) or (
+        isinstance(ntk_in, torch.Tensor) and ntk_in.ndim == 0):
+      if float(ntk_in) != 0.0:
+        raise NotImplementedError('a scalar ntk other than 0 is not '
+                                  'supported as kernel_fn input.')
+      ntk_state = None                         # zero
+    else:
+      ntk_state = natural(ntk_in, 2) if want_ntk else None
+    cov1 = natural(dev(k.cov1), 1)
+    cov2 = natural(dev(k.cov2), 1)
+    C = k.shape1[-1] if k.shape1 else 1
+    plan = self.plan((1, H, W, C), from_state=True,
+                     in_gaussian=bool(k.is_gaussian))
+    want_cov = return_kernel or 'cov1' in get_t or 'cov2' in get_t
+    res = _engine.run_state(
+        plan, nngp, ntk_state, cov1, cov2, want_ntk, want_cov,
+        tile_bytes=tile_bytes or _engine.DEFAULT_TILE_BYTES)
+    is_rev = plan.head is None and (bool(k.is_reversed) !=
+                                    (plan.n_convs % 2 == 1))
+
+    def order(t, bnd):
+      if t is None:
+        return t
+      if flat and plan.head is None:
+        return t.reshape(t.shape[:bnd])
+      return _swap_spatial_pairs(t, bnd) if is_rev else t
+
+    out = Kernel(nngp=order(res['nngp'], 2),
+                 ntk=order(res['ntk'], 2) if want_ntk else None,
+                 cov1=order(res['cov1'], 1), cov2=order(res['cov2'], 1),
+                 x1_is_x2=k.x1_is_x2, is_gaussian=plan.is_gaussian,
+                 is_reversed=is_rev, is_input=False, diagonal_batch=True,
+                 diagonal_spatial=False,
+                 shape1=self._out_shape(tuple(k.shape1)),
+                 shape2=self._out_shape(tuple(k.shape2)),
+                 batch_axis=0, channel_axis=len(self._out_shape(
+                     tuple(k.shape1))) - 1)
+    return self._format(out, get, return_kernel, get_t)
+
   def _out_shape(self, in_shape):
     if self._init_fn is None:
       return in_shape
     out_shape, _ = self._init_fn(0, in_shape)
     return tuple(out_shape)
+
+
+def _swap_spatial_pairs(t, bnd):
+  """[.., X, X', Y, Y'] <-> [.., Y, Y', X, X'] after ``bnd`` batch axes: the
+  reference's axis reversal of a 2-D zipped kernel (``utils.py:465-480``),
+  its own inverse."""
+  return t.permute(tuple(range(bnd)) + (bnd + 2, bnd + 3, bnd, bnd + 1))
 
 
 def _layer(spec):

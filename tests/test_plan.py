@@ -209,3 +209,64 @@ def test_unknown_parameterization_raises_value_error():
   with pytest.raises(NotImplementedError):
     _plan(stax.Conv(1, (3, 3), padding='SAME', parameterization='standard'),
           stax.GlobalAvgPool())
+
+
+def test_from_state_plans():
+  """Plans for kernel_fn(Kernel): no input-covariance stage, layers may
+  start with Relu / pools, and a readout alone is a plan with no stages."""
+  kfn = stax.serial(stax.Relu(), stax.AvgPool((2, 2), (2, 2)),
+                    stax.GlobalAvgPool(), stax.Dense(1, 2.0))[2]
+  p = kfn.plan((1, 8, 8, 1), from_state=True, in_gaussian=True)
+  assert not any(s.first for s in p.stages)
+  assert p.stages[0].relu and p.stages[0].pool == (2, 2)
+  assert p.head == 'gap' and p.out_spatial == (4, 4)
+  with pytest.raises(ValueError):
+    kfn.plan((1, 8, 8, 1), from_state=True, in_gaussian=False)
+  head_only = stax.serial(stax.Flatten(), stax.Dense(10))[2]
+  p = head_only.plan((1, 3, 3, 1), from_state=True)
+  assert p.stages == [] and p.head == 'flatten' and not p.diag
+  with pytest.raises(NotImplementedError):
+    _plan(stax.Flatten(), stax.Dense(10))
+
+
+def test_dense_before_readout_is_a_1x1_stage():
+  """Dense on spatial (or flat) kernels is the affine map of a 1x1 Conv
+  without the axis reversal (linear.py:902-926, 1387-1398)."""
+  p = _plan(stax.Dense(64, 1.5, 0.1), stax.Relu(), stax.Dense(8, 2.0),
+            shape=(1, 1, 1, 20))
+  assert [(s.first, s.conv, s.relu) for s in p.stages] == [
+      (True, (1, 1, False, 2.25, 0.010000000000000002), True),
+      (False, (1, 1, False, 4.0, 0.0), False)]
+  assert p.n_convs == 0 and p.head is None
+  with pytest.raises(NotImplementedError):
+    _plan(stax.Dense(8, parameterization='standard'), stax.Relu())
+  kfn = stax.serial(stax.Conv(1, (3, 3)), stax.Relu())[2]
+  with pytest.raises(ValueError):
+    kfn.plan((4, 20))
+
+
+def test_kernel_reverse_transpose_follow_reference_axes():
+  """Kernel.reverse / transpose (utils/kernel.py:169-222) against the
+  oracle's reverse_zipped (utils.py:465-480) and numpy transposes."""
+  import numpy as np
+  import torch
+  from neural_tangents_amd.kernel import Kernel
+  from oracle import stax_oracle as so
+  rng = np.random.default_rng(0)
+  nngp = rng.standard_normal((2, 3, 4, 4, 5, 5))
+  cov1 = rng.standard_normal((2, 4, 4, 5, 5))
+  k = Kernel(nngp=torch.from_numpy(nngp), ntk=torch.zeros(()),
+             cov1=torch.from_numpy(cov1), cov2=None, x1_is_x2=False,
+             is_gaussian=True, is_reversed=False, is_input=False,
+             diagonal_batch=True, diagonal_spatial=False,
+             shape1=(2, 4, 5, 7), shape2=(3, 4, 5, 7), batch_axis=0,
+             channel_axis=3)
+  r = k.reverse()
+  assert r.is_reversed and r.ntk.ndim == 0
+  np.testing.assert_array_equal(r.nngp.numpy(), so.reverse_zipped(nngp, 2))
+  np.testing.assert_array_equal(r.cov1.numpy(), so.reverse_zipped(cov1, 1))
+  t = k.transpose((1, 0))
+  np.testing.assert_array_equal(t.nngp.numpy(),
+                                nngp.transpose(0, 1, 4, 5, 2, 3))
+  np.testing.assert_array_equal(t.cov1.numpy(), cov1.transpose(0, 3, 4, 1, 2))
+  assert torch.equal(k.transpose().nngp, k.nngp)
