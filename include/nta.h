@@ -289,6 +289,22 @@ int nta_pcg64_k64(const double* K, int n, long long ldk, double lam,
                   int max_iter, void* work, long long work_bytes, int* iters,
                   double* resid, void* stream);
 
+/*
+ * nta_pcg64 without a workspace argument: the iteration's vectors live in a
+ * static device array of the library (one per GPU, held until the process
+ * ends; about 19 MiB, enough for n <= 16384 and nrhs <= 16), so the call
+ * allocates nothing.  n > 16384 returns NTA_EINVAL.  Calls on one GPU share
+ * that array: calls that overlap on one GPU (other streams or threads)
+ * race.  Arguments and results as nta_pcg64.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).
+ */
+int nta_pcg64_noalloc(const float* K, int n, long long ldk, double lam,
+                      const float* W, const float* WT, long long ldw, int n1,
+                      const double* B, int nrhs, double* X, double tol,
+                      int max_iter, int* iters, double* resid, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,6 +48,8 @@ SIGNATURES = {
                   _LL, _P, _P, _P],
     'nta_pcg64_k64': [_P, _I, _LL, _D, _P, _P, _LL, _I, _P, _I, _P, _D, _I,
                       _P, _LL, _P, _P, _P],
+    'nta_pcg64_noalloc': [_P, _I, _LL, _D, _P, _P, _LL, _I, _P, _I, _P, _D,
+                          _I, _P, _P, _P],
     'nta_diag_chain': [_P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P,
                        _I, _D, _D, _D, _D, _D, _P, _P, _LL, _P],
     'nta_first_pool_stage': [_P, _P, _I, _I, _I, _I, _I, _I, _F, _F, _P, _P,

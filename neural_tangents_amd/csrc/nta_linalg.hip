@@ -30,6 +30,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -1404,4 +1405,79 @@ extern "C" int nta_pcg64_k64(const double* K, int n, long long ldk,
   return pcg_run<double>(K, n, ldk, lam, W, WT, ldw, n1, B, nrhs, X, tol,
                          max_iter, (char*)work, iters, resid,
                          (hipStream_t)stream);
+}
+
+// ---------------------------------------------- allocation-free refinement
+// nta_pcg64 with its workspace in a file-scope device array ("scratch"), for
+// callers that cannot allocate.  One copy per GPU; its device address is
+// taken with hipGetSymbolAddress once per device (the array's name is only a
+// host-side stand-in) and kept in g_scratch.  Calls on one GPU share it, so
+// overlapping calls on one GPU race; every call initialises the parts it
+// reads (k_pcg_io, the first preconditioner application), so nothing of a
+// previous call is seen.
+namespace nta {
+
+constexpr int kScratchMaxN = 16384;
+constexpr size_t kScratchBytes =
+    6 * (size_t)kScratchMaxN * RHS * sizeof(double) +         // v64
+    6 * (size_t)(kScratchMaxN / 16) * RHS * sizeof(double) +  // slots
+    256 +                                                     // sums
+    6 * (size_t)kScratchMaxN * RHS * sizeof(float);           // v32
+
+__device__ __align__(256) char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch[kMaxDevices];
+
+// Device address of this device's scratch (set up on the first call on the
+// current device), or nullptr after setting the error.
+static void* scratch_address() {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error((int)e, "nta_pcg64_noalloc: hipGetDevice: %s",
+              hipGetErrorString(e));
+    return nullptr;
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error(NTA_EINVAL, "nta_pcg64_noalloc: device %d out of range", dev);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(e != hipSuccess ? (int)e : NTA_EINVAL,
+                "nta_pcg64_noalloc: hipGetSymbolAddress(scratch): %s",
+                hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch[dev] = p;
+  }
+  return g_scratch[dev];
+}
+
+}  // namespace nta
+
+extern "C" int nta_pcg64_noalloc(const float* K, int n, long long ldk,
+                                 double lam, const float* W, const float* WT,
+                                 long long ldw, int n1, const double* B,
+                                 int nrhs, double* X, double tol,
+                                 int max_iter, int* iters, double* resid,
+                                 void* stream) {
+  if (n > kScratchMaxN)
+    return set_error(NTA_EINVAL, "nta_pcg64_noalloc: n=%d exceeds the "
+                     "static workspace's limit n <= %d (use nta_pcg64)", n,
+                     kScratchMaxN);
+  const long long need = nta_pcg64_work_bytes(n, nrhs);
+  if (need > (long long)kScratchBytes)
+    return set_error(NTA_EINVAL, "nta_pcg64_noalloc: n=%d nrhs=%d needs "
+                     "%lld bytes of workspace, the static one holds %lld", n,
+                     nrhs, need, (long long)kScratchBytes);
+  void* ws = scratch_address();
+  if (!ws) return NTA_EINVAL;
+  return nta_pcg64(K, n, ldk, lam, W, WT, ldw, n1, B, nrhs, X, tol, max_iter,
+                   ws, (long long)kScratchBytes, iters, resid, stream);
 }
