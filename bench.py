@@ -158,7 +158,7 @@ def tile_rows(kernel_fn, x1, x2):
   """Row-tile size of the engine's pair tiling for this call."""
   from neural_tangents_amd import _engine
   plan = kernel_fn.plan(x1.shape)
-  fused = _engine._fuses_first_two(plan, x1.shape)
+  fused = _engine._fused_stages(plan, x1.shape)
   per = _engine._per_pair_bytes(plan, True, fused)
   T = max(1, _engine.DEFAULT_TILE_BYTES // per)
   tj = min(x2.shape[0], T)
@@ -201,13 +201,15 @@ def kernel_table(per, steps):
           for n, v in per.items()}
 
 
-def krr_result(ntr, nte, reps):
+def krr_result(ntr, nte, reps, tol=1e-7):
   """Kernel-regression solve (infinite-time gradient_descent_mse_ensemble):
   Myrtle-5 32x32 NTK of ntr synthetic images, one-hot targets (-0.1),
   diag_reg = 1e-4 (relative to the mean trace).  Times the plain FP64
   Cholesky solve (rocSOLVER through torch) and the mixed solve (FP32 MFMA
   Cholesky + FP64-residual PCG), both on the same kernel, and reports the
-  relative error of alpha and of the test predictions against FP64."""
+  relative error of alpha and of the test predictions against FP64.  The
+  refinement stops when its estimated relative error (or residual) is at
+  most ``tol``, a tenth of the 1e-6 bar by default."""
   import torch
   from neural_tangents_amd import _solve, stax
   from neural_tangents_amd.data import krr_workload
@@ -231,7 +233,7 @@ def krr_result(ntr, nte, reps):
     for _ in range(reps):
       torch.cuda.synchronize()
       t = time.perf_counter()
-      s = _solve.CholeskySolver(k_dd, 1e-4, False, mode)
+      s = _solve.CholeskySolver(k_dd, 1e-4, False, mode, tol=tol)
       alpha = s.solve(y)
       f = k_td.to(torch.float64) @ alpha
       torch.cuda.synchronize()
@@ -257,6 +259,9 @@ def krr_result(ntr, nte, reps):
                          '(rocSOLVER); tflops counts n^3/3 over the whole '
                          'solve + predict time',
       'pcg_iterations': iters, 'final_rel_residual': resid,
+      'pcg_tol': tol,
+      'pcg_tol_note': 'refinement stops at estimated relative error '
+                      '||M^-1 r|| / ||x|| (or relative residual) <= tol',
       'fell_back_to_fp64': bool(smx.impl.fell_back),
       'rel_err_alpha_vs_fp64': err_a, 'rel_err_ftest_vs_fp64': err_f,
       'kernel_seconds': t_kernel,
@@ -278,11 +283,17 @@ def m10_result(steps):
   value = n1 * n2 * steps / elapsed
   bpp = canonical_bytes_per_pair(depths, side)
   log(f'myrtle10: {value:.0f} pairs/s')
+  table = kernel_table(per, steps)
+  dom = max(per, key=lambda n: per[n]['ms'])
   return {'workload': 'myrtle10_32x32_128x512', 'pairs_per_s': value,
           'ms_per_step': elapsed / steps * 1e3, 'steps': steps,
           'path_gbs_canonical': bpp * value / 1e9,
           'path_pct_hbm_peak': 100.0 * bpp * value / 1e9 / HBM_PEAK_GBS,
-          'kernels': kernel_table(per, steps)}
+          'dominant_kernel': dom,
+          'dominant_gbs_canonical': table[dom]['gbs'],
+          'dominant_frac_hbm_peak': table[dom]['gbs'] / HBM_PEAK_GBS,
+          'note': SCHEDULE_NOTES.get(dom, ''),
+          'kernels': table}
 
 
 def grad_result():
@@ -325,6 +336,11 @@ def grad_result():
 
 # How the algorithmic bytes of a launch relate to what it moves.
 SCHEDULE_NOTES = {
+    'nta_first3_pool_stage':
+        'first three stages fused ([Conv-Relu] x 3 - AvgPool, Myrtle-10): '
+        'achieved counts the per-layer schedule\'s bytes for those three '
+        'kernels (two 32^4 states written and read back, the pooled state '
+        'written); the kernel never materialises the first two stages',
     'nta_first_pool_stage':
         'first two stages fused (Conv-Relu-Conv-Relu-AvgPool): achieved '
         'counts the per-layer schedule\'s bytes for those two kernels (the '
@@ -362,6 +378,7 @@ def main():
   ap.add_argument('--krr-train', type=int, default=8192)
   ap.add_argument('--krr-test', type=int, default=1024)
   ap.add_argument('--krr-reps', type=int, default=3)
+  ap.add_argument('--krr-tol', type=float, default=1e-7)
   args = ap.parse_args()
   if args.gpus != 1:
     print('bench.py runs on one GPU only: this engine has no multi-GPU path '
@@ -369,7 +386,8 @@ def main():
           'design notes).', file=sys.stderr)
     sys.exit(2)
   if args.krr:
-    out = krr_result(args.krr_train, args.krr_test, args.krr_reps)
+    out = krr_result(args.krr_train, args.krr_test, args.krr_reps,
+                     args.krr_tol)
     out.update({'n_gpus': 1, 'higher_is_better': True,
                 'dtype': 'fp32+fp64',
                 'data': 'synthetic (blurred Gaussian images, seed 0)'})
@@ -448,7 +466,8 @@ def main():
     out['myrtle10'] = m10_result(3)
     torch.cuda.empty_cache()
   if not args.no_krr and args.workload == 'myrtle5_32x32_256x256':
-    out['krr'] = krr_result(args.krr_train, args.krr_test, args.krr_reps)
+    out['krr'] = krr_result(args.krr_train, args.krr_test, args.krr_reps,
+                            args.krr_tol)
     torch.cuda.empty_cache()
   if not args.no_grad and args.workload == 'myrtle5_32x32_256x256':
     out['grad'] = grad_result()

@@ -368,7 +368,24 @@ def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
     dev = xa.device
     _, H, W, C = xa.shape
   skip = 0
-  if qn is not None and n2 != 0:
+  if qn is not None and n2 != 0 and len(qn) == 6:
+    s0, s1, s2 = plan.stages[0], plan.stages[1], plan.stages[2]
+    out = torch.empty((T, s2.hout, s2.hout, s2.wout, s2.wout),
+                      dtype=torch.float32, device=dev)
+    out_t = torch.empty_like(out) if fl_ntk else None
+    # Algorithmic bytes: those of the three per-layer kernels it replaces
+    # (two 32^4 states written and read back, the pooled state written).
+    nt = 2 if fl_ntk else 1
+    alg = T * nt * 4 * (4 * _state_elems(s0.hc, s0.wc) + out[0].numel())
+    _launch('nta_first3_pool_stage', alg, T, _lib.ptr(xa), _lib.ptr(xb), n1,
+            n2, H, W, C, _lib.NTK if fl_ntk else 0, s0.conv[3], s0.conv[4],
+            _lib.ptr(qn[0]), _lib.ptr(qn[1]), s1.conv[3], s1.conv[4],
+            _lib.ptr(qn[2]), _lib.ptr(qn[3]), s2.conv[3], s2.conv[4],
+            _lib.ptr(qn[4]), _lib.ptr(qn[5]), _lib.ptr(out), _lib.ptr(out_t),
+            stream)
+    nngp, ntk = out, out_t
+    skip = 3
+  elif qn is not None and n2 != 0:
     s0, s1 = plan.stages[0], plan.stages[1]
     out = torch.empty((T, s1.hout, s1.hout, s1.wout, s1.wout),
                       dtype=torch.float32, device=dev)
@@ -475,14 +492,42 @@ def _fuses_first_two(plan: Plan, in_shape) -> bool:
           and s1.pool == (2, 2))
 
 
-def _per_pair_bytes(plan: Plan, ntk: bool, fused: bool = False,
+def _fuses_first_three(plan: Plan, in_shape) -> bool:
+  """Conv(3x3 SAME) -> Relu, three times, -> AvgPool(2x2) at the start of
+  the network (Myrtle-10) on 32x32 images with <= 4 channels: the first
+  three stages run as one launch (nta_first3_pool_stage)."""
+  if len(plan.stages) < 3:
+    return False
+  _, H, W, C = in_shape
+  s0, s1, s2 = plan.stages[0], plan.stages[1], plan.stages[2]
+
+  def conv_relu(s):
+    return (s.conv is not None and s.conv[:3] == (3, 3, False)
+            and not s.circular and s.relu and s.act == (1.0, 0.0)
+            and s.act_kind == 0)
+  return (H == 32 and W == 32 and 1 <= C <= 4 and s0.first
+          and conv_relu(s0) and s0.pool is None and conv_relu(s1)
+          and s1.pool is None and conv_relu(s2) and s2.pool == (2, 2))
+
+
+def _fused_stages(plan: Plan, in_shape) -> int:
+  """Number of leading stages one fused launch runs (0, 2 or 3)."""
+  if _fuses_first_three(plan, in_shape):
+    return 3
+  if _fuses_first_two(plan, in_shape):
+    return 2
+  return 0
+
+
+def _per_pair_bytes(plan: Plan, ntk: bool, fused=False,
                     in_hw=None) -> int:
   m = 0 if in_hw is None else _state_elems(*in_hw)
+  nf = 2 if fused is True else int(fused)
   for si, s in enumerate(plan.stages):
-    if fused and si < 2:
-      # the first stage's state and the second conv's output are never
-      # materialised
-      m = max(m, _state_elems(s.hout, s.wout)) if si == 1 else m
+    if si < nf:
+      # the fused stages' states are never materialised, except the last
+      # one's (pooled) output
+      m = max(m, _state_elems(s.hout, s.wout)) if si == nf - 1 else m
       continue
     m = max(m, _state_elems(s.hc, s.wc), _state_elems(s.hout, s.wout))
   # ping-pong input/output state, nngp (+ ntk)
@@ -710,12 +755,12 @@ def run(plan: Plan, x1, x2, want_nngp, want_ntk, want_cov,
     xb_all = x1
   N1, N2 = x1.shape[0], xb_all.shape[0]
   dev = x1.device
-  fused = _fuses_first_two(plan, x1.shape)
+  fused = _fused_stages(plan, x1.shape)
   qn1 = qn2 = None
   if fused:
-    qn1 = [_qnorm(q1s[0], stream), _qnorm(q1s[1], stream)]
-    qn2 = qn1 if x2 is None else [_qnorm(q2s[0], stream),
-                                  _qnorm(q2s[1], stream)]
+    qn1 = [_qnorm(q1s[k], stream) for k in range(fused)]
+    qn2 = qn1 if x2 is None else [_qnorm(q2s[k], stream)
+                                  for k in range(fused)]
   per = _per_pair_bytes(plan, want_ntk, fused)
   T = max(1, tile_bytes // per)
   if x2 is None and symmetric and plan.head is not None and T < N1 * N2:
@@ -747,8 +792,8 @@ def run(plan: Plan, x1, x2, want_nngp, want_ntk, want_cov,
       xb = xb_all[j0:j0 + n2]
       qn = None
       if fused:
-        qn = (qn1[0][i0:i0 + n1], qn2[0][j0:j0 + n2],
-              qn1[1][i0:i0 + n1], qn2[1][j0:j0 + n2])
+        qn = tuple(t for k in range(fused)
+                   for t in (qn1[k][i0:i0 + n1], qn2[k][j0:j0 + n2]))
       if plan.head is not None:
         on = nngp[i0:, j0:]
         ot = ntk[i0:, j0:] if want_ntk else None

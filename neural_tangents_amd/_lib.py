@@ -54,15 +54,19 @@ SIGNATURES = {
                        _I, _D, _D, _D, _D, _D, _P, _P, _LL, _P],
     'nta_first_pool_stage': [_P, _P, _I, _I, _I, _I, _I, _I, _F, _F, _P, _P,
                              _F, _F, _P, _P, _P, _P, _P],
+    'nta_first3_pool_stage': [_P, _P, _I, _I, _I, _I, _I, _I, _F, _F, _P,
+                              _P, _F, _F, _P, _P, _F, _F, _P, _P, _P, _P,
+                              _P],
     'nta_qnorm': [_P, _LL, _P, _P],
     'nta_relu_bwd': [_P, _P, _I, _I, _I, _I, _I, _F, _F, _F, _F, _P, _P, _P,
-                     _P, _P, _P, _P, _P, _P, _P, _P],
+                     _P, _P, _P, _P, _P, _P, _P],
 }
 
 # Entry points that return a value instead of a status.
 VALUE_FNS = {
     'nta_chol_split': (_I, [_I]),
     'nta_pcg64_work_bytes': (_LL, [_I, _I]),
+    'nta_relu_bwd_parts': (_LL, [_I, _I, _I]),
 }
 
 

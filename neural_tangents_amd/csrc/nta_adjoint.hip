@@ -13,7 +13,8 @@
 // Given the pooled cotangents gN, gT this writes, per element,
 //   gaN = w_var (gc + gt)   gaT = w_var gt        (cotangents of aN, aT)
 //   gqa, gqb                                     (cotangents of qa, qb)
-//   gwv = (gc + gt) aN + gt aT,  gbv = gc + gt     (of w_var, b_var)
+//   gwv = (gc + gt) aN + gt aT,  gbv = gc + gt     (of w_var, b_var; summed
+//                                                  per workgroup into gwb)
 // with gt = gTD ds and gc = gJ ds + gTD t d(ds)/dc.  The derivative rules at
 // s = 0 follow the reference's custom JVPs (_sqrt_jvp and _arctan2_jvp,
 // stax/elementwise.py:1283-1310): the tangent of s is zero there, so the
@@ -37,8 +38,10 @@ __global__ void k_relu_bwd(const float* __restrict__ aN,
                            const float* __restrict__ gT,
                            float* __restrict__ gaN, float* __restrict__ gaT,
                            float* __restrict__ gqa, float* __restrict__ gqb,
-                           float* __restrict__ gwv, float* __restrict__ gbv) {
+                           double* __restrict__ gwb) {
+  __shared__ double red[2][4];
   const bool self = n2 == 0;
+  double sw = 0.0, sb = 0.0;
   const uint64_t T = self ? (uint64_t)n1 : (uint64_t)n1 * n2;
   const uint64_t per = (uint64_t)S * S * S * S;
   const int So = S / pool;
@@ -90,37 +93,66 @@ __global__ void k_relu_bwd(const float* __restrict__ aN,
     if (gaT) gaT[idx] = aT ? w_var * gt : 0.0f;
     gqa[idx] = gj * dj_qa + gtd * t * dd_qa;
     gqb[idx] = gj * dj_qb + gtd * t * dd_qb;
-    gwv[idx] = gct * an + gt * at;
-    gbv[idx] = gct;
+    sw += (double)(gct * an + gt * at);
+    sb += (double)gct;
   }
+  // Per-workgroup partial sums of the d/dw_var and d/db_var terms (fixed
+  // order: wave shuffles, then the four waves).
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    sw += __shfl_xor(sw, o);
+    sb += __shfl_xor(sb, o);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wave] = sw;
+    red[1][wave] = sb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    gwb[2 * blockIdx.x] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    gwb[2 * blockIdx.x + 1] =
+        (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+  }
+}
+
+// Workgroups of a k_relu_bwd launch (= partial-sum slots of gwb).
+static long long relu_bwd_blocks(int n1, int n2, int S) {
+  const uint64_t T = n2 == 0 ? (uint64_t)n1 : (uint64_t)n1 * n2;
+  const uint64_t total = T * S * S * S * S;
+  uint64_t g = (total + 255) / 256;
+  if (g > (1u << 16)) g = 1u << 16;
+  return (long long)g;
 }
 
 }  // namespace nta
 
 using namespace nta;
 
+extern "C" long long nta_relu_bwd_parts(int n1, int n2, int S) {
+  if (n1 < 0 || n2 < 0 || S <= 0) return -1;
+  if (n1 == 0) return 0;
+  return relu_bwd_blocks(n1, n2, S);
+}
+
 extern "C" int nta_relu_bwd(const float* aN, const float* aT, int n1, int n2,
                             int S, int pool, int self_off, float w_var,
                             float b_var, float act_k1, float act_k0,
                             const float* q1, const float* q2, const float* gN,
                             const float* gT, float* gaN, float* gaT,
-                            float* gqa, float* gqb, float* gwv, float* gbv,
+                            float* gqa, float* gqb, double* gwb,
                             void* stream) {
   if (n1 < 0 || n2 < 0 || S <= 0 || (pool != 1 && pool != 2) ||
       S % pool != 0)
     return set_error(NTA_EINVAL, "nta_relu_bwd: bad n1=%d n2=%d S=%d "
                      "pool=%d", n1, n2, S, pool);
   if (n1 == 0) return NTA_OK;
-  if (!aN || !q1 || (n2 > 0 && !q2) || !gaN || !gqa || !gqb || !gwv ||
-      !gbv)
+  if (!aN || !q1 || (n2 > 0 && !q2) || !gaN || !gqa || !gqb || !gwb)
     return set_error(NTA_EINVAL, "nta_relu_bwd: null pointer");
-  const uint64_t T = n2 == 0 ? (uint64_t)n1 : (uint64_t)n1 * n2;
-  const uint64_t total = T * S * S * S * S;
-  uint64_t g = (total + 255) / 256;
-  if (g > (1u << 20)) g = 1u << 20;
+  const long long g = relu_bwd_blocks(n1, n2, S);
   hipLaunchKernelGGL(k_relu_bwd, dim3((unsigned)g), dim3(256), 0,
                      (hipStream_t)stream, aN, aT, n1, n2, S, pool, self_off,
                      w_var, b_var, act_k1, act_k0, q1, q2, gN, gT, gaN, gaT,
-                     gqa, gqb, gwv, gbv);
+                     gqa, gqb, gwb);
   return check_launch("nta_relu_bwd");
 }

@@ -24,6 +24,8 @@ int launch_stage_fast(const float* in_nngp, const float* in_ntk, int n1,
                       float b_var, float k1, float k0, const float* q1,
                       const float* q2, float* out_nngp, float* out_ntk,
                       hipStream_t stream);
+int launch_stencil_fast(const float* in, long long T, int S, float w_var,
+                        float b_var, float* out, hipStream_t st);
 int launch_first_fast(const float* x1, const float* x2, int n1, int n2, int S,
                       int C, int flags, float w_var, float b_var, float k1,
                       float k0, const float* q1, const float* q2,

@@ -483,6 +483,101 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   }
 }
 
+// Stencil alone (no activation, no pool): out = w9 (3x3 shared-offset
+// stencil sum of in) + b over T states [S]^4 -- the Conv of a stage without
+// its ReLU, e.g. the stencil means the adjoint recomputes and its
+// self-adjoint transpose.  A workgroup sweeps diagonal d = h - h' of JPW
+// states as the no-pool path of k_stage_fast does.
+template <int S>
+__global__ __launch_bounds__(256) void k_stencil_fast(
+    const float* __restrict__ in, long long T, int ngroups, int nblk,
+    float w9, float b_var, float* __restrict__ out) {
+  constexpr int H2 = S / 2;
+  constexpr int TPP = H2 * H2;
+  constexpr int JPW = 256 / TPP;
+  constexpr int LS = S + 2;
+  constexpr int PL = LS * LS;
+  constexpr long long PER = (long long)S * S * S * S;
+  constexpr int ndiag = 2 * S - 1;
+  __shared__ __align__(16) float lds[JPW * PL];
+  const int L = xcd_remap(blockIdx.x, nblk);
+  if (L >= ndiag * ngroups) return;       // whole block exits together
+  const int g = L / ndiag;
+  const int di = L - g * ndiag;
+  const int job = threadIdx.x / TPP;
+  const int t = threadIdx.x - job * TPP;
+  const int W = t / H2;
+  const int Wp = t - W * H2;
+  const long long p = (long long)g * JPW + job;
+  const bool valid = p < T;
+  for (int k = threadIdx.x; k < JPW * PL; k += 256) lds[k] = 0.0f;
+  __syncthreads();
+  float* L0 = lds + job * PL;
+  const long long p0 = (long long)g * JPW;
+  const long long left = T - p0;
+  const unsigned nbytes = (unsigned)((left < JPW ? left : JPW) * PER * 4);
+  const __amdgpu_buffer_rsrc_t rn = make_rsrc(in + p0 * PER, nbytes);
+  const unsigned tofs = (unsigned)(job * PER + (2 * W) * S + 2 * Wp) * 4u;
+  const int lofs = (2 * W + 1) * LS + 2 * Wp + 1;
+  const int d = di - (S - 1);
+  const int hs = d > 0 ? d : 0;
+  const int he = d < 0 ? S - 1 + d : S - 1;
+  float cA[4], cB[4];
+  {
+    const int hh[1] = {hs};
+    const int hp[1] = {hs - d};
+    float ws[1][4];
+    wpass<S, 1>(rn, hh, hp, L0, tofs, lofs, ws);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      cA[e] = 0.0f;               // plane (hs-1, hs-1-d) is always outside
+      cB[e] = ws[0][e];
+    }
+  }
+  for (int h = hs; h <= he; ++h) {
+    const int hq = h - d;
+    const int hh[1] = {h + 1};
+    const int hp[1] = {h + 1 - d};
+    float ws[1][4], v[4];
+    wpass<S, 1>(rn, hh, hp, L0, tofs, lofs, ws);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      v[e] = fmaf(w9, cA[e] + cB[e] + ws[0][e], b_var);
+      cA[e] = cB[e];
+      cB[e] = ws[0][e];
+    }
+    if (valid) {
+      const long long o =
+          p * PER + ((long long)(h * S + hq) * S + 2 * W) * S + 2 * Wp;
+      *reinterpret_cast<float2*>(out + o) = make_float2(v[0], v[1]);
+      *reinterpret_cast<float2*>(out + o + S) = make_float2(v[2], v[3]);
+    }
+  }
+}
+
+int launch_stencil_fast(const float* in, long long T, int S, float w_var,
+                        float b_var, float* out, hipStream_t st) {
+  const float w9 = w_var / 9.0f;
+  auto go = [&](auto sc) -> int {
+    constexpr int SS = decltype(sc)::value;
+    constexpr int JPW = 256 / ((SS / 2) * (SS / 2));
+    const long long ngroups = (T + JPW - 1) / JPW;
+    const long long nb = ngroups * (2 * SS - 1);
+    if (nb > 0x7ffffff0LL)
+      return set_error(NTA_EINVAL, "nta_stage: tile too large (%lld)", T);
+    const int nblk = (int)((nb + 7) / 8 * 8);
+    hipLaunchKernelGGL(k_stencil_fast<SS>, dim3(nblk), dim3(256), 0, st, in,
+                       T, (int)ngroups, nblk, w9, b_var, out);
+    return check_launch("nta_stage");
+  };
+  switch (S) {
+    case 32: return go(std::integral_constant<int, 32>());
+    case 16: return go(std::integral_constant<int, 16>());
+    case 8: return go(std::integral_constant<int, 8>());
+  }
+  return kNotHandled;
+}
+
 // ------------------------------------------------------------ first stage
 
 // Loads image n of x ([N][S][S][C], C <= 4) into a zero-padded,
@@ -527,7 +622,9 @@ __device__ __forceinline__ void first32_epilogue(
   }
 }
 
-template <bool NTK>
+// RELU = false: the stencil of the input Gram alone (scale G + b_var into
+// out_n; q1, q2 and out_t unused), e.g. for the adjoint's recomputation.
+template <bool NTK, bool RELU = true>
 __global__ __launch_bounds__(256) void k_first32(
     const float* __restrict__ x1, const float* __restrict__ x2, int n1,
     int n2, int C, int nblk, float scale, float b_var,
@@ -553,7 +650,7 @@ __global__ __launch_bounds__(256) void k_first32(
   const float* qbp = self ? q1 : q2;
   load_padded<S>(x1, i, C, true, xp1);
   load_padded<S>(xb, j, C, true, xp2);
-  if (threadIdx.x < S) {
+  if (RELU && threadIdx.x < S) {
     float2 v = qnorm(q1[(i * S + h) * S + threadIdx.x]);
     lq1[threadIdx.x] = make_float2(v.x * kInv2Pi, v.y);
   }
@@ -578,13 +675,15 @@ __global__ __launch_bounds__(256) void k_first32(
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int m = 8 * (r >> 2) + 4 * half + (r & 3);
-    sq1r[r] = lq1[m].x;
-    rq1r[r] = lq1[m].y;
+    sq1r[r] = RELU ? lq1[m].x : 0.0f;
+    rq1r[r] = RELU ? lq1[m].y : 0.0f;
   }
-  __syncthreads();                      // xp1 reads done: reuse as lq2
-  for (int k = threadIdx.x; k < S * S; k += 256)
-    lq2[k] = qnorm(qbp[j * (S * S) + k]);
-  __syncthreads();
+  if constexpr (RELU) {
+    __syncthreads();                    // xp1 reads done: reuse as lq2
+    for (int k = threadIdx.x; k < S * S; k += 256)
+      lq2[k] = qnorm(qbp[j * (S * S) + k]);
+    __syncthreads();
+  }
 
   const float* pb0 = xp2 + half * PLN + col;
   auto gram = [&](int hq) {
@@ -606,12 +705,20 @@ __global__ __launch_bounds__(256) void k_first32(
   for (int hq = wave; hq < S; hq += 4) {
     floatx16 nxt;
     if (hq + 4 < S) nxt = gram(hq + 4);
-    const float2 qb = lq2[hq * S + col];
     float* on = out_n + p * PER + (long long)(h * S + hq) * (S * S);
-    float* ot = NTK ? out_t + p * PER + (long long)(h * S + hq) * (S * S)
-                    : nullptr;
-    first32_epilogue<NTK>(cur, sq1r, rq1r, qb, scale, b_var, on, ot, half,
-                          col);
+    if constexpr (RELU) {
+      const float2 qb = lq2[hq * S + col];
+      float* ot = NTK ? out_t + p * PER + (long long)(h * S + hq) * (S * S)
+                      : nullptr;
+      first32_epilogue<NTK>(cur, sq1r, rq1r, qb, scale, b_var, on, ot, half,
+                            col);
+    } else {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int m = 8 * (r >> 2) + 4 * half + (r & 3);
+        on[m * S + col] = fmaf(scale, cur[r], b_var);
+      }
+    }
     cur = nxt;
   }
 }
@@ -1215,6 +1322,543 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
   }
 }
 
+// ------------------------------------------- first three stages, fused
+
+// Conv(3x3 SAME) -> ReLU -> Conv(3x3 SAME) -> ReLU -> Conv(3x3 SAME) ->
+// ReLU -> AvgPool(2x2) on 32x32 images (the first three layers of
+// Myrtle-10), with neither the first nor the second stage's 32^4 state
+// ever written.  The sweep is k_first_pool32's, one stage deeper: a step of
+// pooled row H on pooled diagonal D needs third-stage pre-activations on
+// the input diagonals 2D - 1, 2D, 2D + 1 (rows 2H, 2H + 1), hence six new
+// second-stage planes per step, at the same offsets (2H + ar, 2Hp + br) as
+// the first-stage planes of k_first_pool32, and six new first-stage planes
+// one row further on (2H + 1 + ar, 2Hp + 1 + br).  Each conv's (h, h') 3-tap
+// sum is a register window along the diagonal (stage 1: two w-summed planes
+// carried per input diagonal; stage 2: as the first stage of
+// k_first_pool32), each (w, w') 3-tap sum an LDS exchange.  The second
+// stage's output is computed per thread on its 2 x 2 (w, w') block, kept
+// 2 pi times larger like the first stage's (the factor moves into the next
+// conv's weight), and zero on planes outside the grid (the third conv's
+// zero padding).
+template <bool NTK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_first3_pool32(const float* __restrict__ x1,
+                     const float* __restrict__ x2, int n1, int n2, int C,
+                     int nblk, float scale1, float b1,
+                     const float2* __restrict__ qa1,
+                     const float2* __restrict__ qa2, float wv9b, float b2,
+                     const float2* __restrict__ qb1,
+                     const float2* __restrict__ qb2, float wv9c, float b3,
+                     const float2* __restrict__ qc1,
+                     const float2* __restrict__ qc2, float* __restrict__ out_n,
+                     float* __restrict__ out_t) {
+  constexpr int S = 32, H2 = 16, SO = 16;
+  constexpr long long PER_OUT = (long long)SO * SO * SO * SO;
+  constexpr int NT = NTK ? 2 : 1;
+  __shared__ __align__(16) _Float16 xh[2 * 2 * kXS * kXS * 4];
+  __shared__ __align__(16) float lp[6 * kPL2];        // exchange planes
+  __shared__ unsigned smax[2];
+  const bool self = n2 == 0;
+  const long long T = self ? (long long)n1 : (long long)n1 * n2;
+  const int L = xcd_remap(blockIdx.x, nblk);
+  if ((long long)L >= T * kF12Slots) return;   // whole block exits together
+  const long long p = L / kF12Slots;
+  const int slot = L - (int)(p * kF12Slots);
+  const long long i = self ? p : p / n2;
+  const long long j = self ? p : p - i * n2;
+  const int t = threadIdx.x;
+  // Block role (stages 2 and 3): the thread's 2x2 (w, w') block at
+  // (2W, 2Wp).
+  const int W = t / H2;
+  const int Wp = t - W * H2;
+  const int lofs = (2 * W + 1) * kLS2 + 2 * Wp + 1;
+  // MFMA role (stage 1): quadrant of the wave, rows w0 .. w0 + 3, column wq.
+  const int lane = t & 63, wave = t >> 6;
+  const int m16 = lane & 15, c4 = lane >> 4;
+  const int w0 = 16 * (wave >> 1) + 4 * c4;
+  const int wq = 16 * (wave & 1) + m16;
+  const int wr = (w0 + 1) * kLS2 + wq + 1;
+
+  static_assert((6 * kPL2) % 4 == 0, "exchange planes in float4 units");
+  for (int k = t; k < 6 * kPL2 / 4; k += 256)
+    reinterpret_cast<float4*>(lp)[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (t < 2) smax[t] = 0u;
+  const float* xi = x1 + i * (S * S * C);
+  const float* xj = (self ? x1 : x2) + j * (S * S * C);
+  __syncthreads();
+  {
+    float mi = 0.0f, mj = 0.0f;
+    for (int k = t; k < S * S * C; k += 256) {
+      mi = fmaxf(mi, fabsf(xi[k]));
+      mj = fmaxf(mj, fabsf(xj[k]));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      mi = fmaxf(mi, __shfl_xor(mi, o));
+      mj = fmaxf(mj, __shfl_xor(mj, o));
+    }
+    if (lane == 0) {
+      atomicMax(&smax[0], __float_as_uint(mi));
+      atomicMax(&smax[1], __float_as_uint(mj));
+    }
+  }
+  __syncthreads();
+  int ei, ej;
+  frexpf(__uint_as_float(smax[0]), &ei);
+  frexpf(__uint_as_float(smax[1]), &ej);
+  const float scale_g = ldexpf(scale1, ei + ej);
+  for (int k = t; k < 2 * kXS * kXS; k += 256) {
+    const int img = k >= kXS * kXS;
+    const int pos = k - img * (kXS * kXS);
+    const int hh = pos / kXS - 1;
+    const int ww = pos - (pos / kXS) * kXS - 1;
+    const float sc = ldexpf(1.0f, img ? -ej : -ei);
+    const float* src = (img ? xj : xi) + (hh * S + ww) * C;
+    const bool in = (unsigned)hh < (unsigned)S && (unsigned)ww < (unsigned)S;
+    half4v hv4, lv4;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const float v = (in && c < C) ? src[c] * sc : 0.0f;
+      const _Float16 hv = (_Float16)v;
+      hv4[c] = hv;
+      lv4[c] = (_Float16)(v - (float)hv);
+    }
+    *reinterpret_cast<half4v*>(xh + ((img * 2) * (kXS * kXS) + pos) * 4) =
+        hv4;
+    *reinterpret_cast<half4v*>(xh + ((img * 2 + 1) * (kXS * kXS) + pos) *
+                                        4) = lv4;
+  }
+  __syncthreads();
+  constexpr int kRowB = kXS * 4 * 2;
+  constexpr int kHlB = kXS * kXS * 4 * 2;
+  int oa[2], ob[2];
+#pragma unroll
+  for (int st = 0; st < 2; ++st) {
+    const int tx = st == 0 ? 0 : 2;
+    oa[st] = c4 < 3 ? c4 * kRowB + (16 * (wave >> 1) + m16 + tx) * 8
+                    : (st == 0 ? (kXS - 1) * 8 : 0);
+    ob[st] = c4 < 3 ? c4 * kRowB + (16 * (wave & 1) + m16 + tx) * 8
+                    : (st == 0 ? (kXS - 1) * 8 : 0);
+  }
+  const char* xhb = reinterpret_cast<const char*>(xh);
+  // Operands of image row `row` (clamped into the image: rows past it
+  // belong to planes that are dropped).
+  auto opnd8 = [&](int img, int hl, int row) {
+    row = row < S ? row : S - 1;
+    const char* b = xhb + (img * 2 + hl) * kHlB + row * kRowB +
+                    (img ? ob[0] : oa[0]);
+    const half4v lo = *reinterpret_cast<const half4v*>(b);
+    const half4v hi = *reinterpret_cast<const half4v*>(b + 8);
+    return half8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+  };
+  auto opnd4 = [&](int img, int hl, int row) {
+    row = row < S ? row : S - 1;
+    const char* b = xhb + (img * 2 + hl) * kHlB + row * kRowB +
+                    (img ? ob[1] : oa[1]);
+    const half4v lo = *reinterpret_cast<const half4v*>(b);
+    const _Float16 z = 0;
+    if (img == 1) return half8v{lo[0], lo[1], lo[2], lo[3], z, z, z, z};
+    const half4v hi = *reinterpret_cast<const half4v*>(b + 8);
+    return half8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+  };
+  const float2* qi = qa1 + i * (S * S);
+  const float2* qj = (self ? qa1 : qa2) + j * (S * S);
+  const float2* ri = qb1 + i * (S * S);
+  const float2* rj = (self ? qb1 : qb2) + j * (S * S);
+  const float2* ui = qc1 + i * (S * S);
+  const float2* uj = (self ? qc1 : qc2) + j * (S * S);
+  auto ld_r = [&](const float2* tab, int row, int col) {
+    row = row < 0 ? 0 : (row < S ? row : S - 1);
+    return *reinterpret_cast<const float4*>(tab + row * S + col);
+  };
+
+  // One first-stage plane (hh, hp), this wave's quadrant (MFMA layout),
+  // 2 pi times the first stage's state; zero outside the grid.
+  auto s1 = [&](int hh, int hp, float (&vn)[4], float (&vt)[4]) {
+    const bool ok = (unsigned)hh < (unsigned)S && (unsigned)hp < (unsigned)S;
+    if (!ok) hh = hp = 0;
+    floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (ok) {
+      const half8v ah = opnd8(0, 0, hh), al = opnd8(0, 1, hh);
+      const half8v bh = opnd8(1, 0, hp), bl = opnd8(1, 1, hp);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc, 0, 0, 0);
+      const half8v ch = opnd4(0, 0, hh), cl = opnd4(0, 1, hh);
+      const half8v dh = opnd4(1, 0, hp), dl = opnd4(1, 1, hp);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ch, dh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ch, dl, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(cl, dh, acc, 0, 0, 0);
+    }
+    const float4 q01 = *reinterpret_cast<const float4*>(qi + hh * S + w0);
+    const float4 q23 = *reinterpret_cast<const float4*>(qi + hh * S + w0 + 2);
+    const float2 qb = qj[hp * S + wq];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float4 Q = e < 2 ? q01 : q23;
+      const float sa = (e & 1) ? Q.z : Q.x, ra = (e & 1) ? Q.w : Q.y;
+      const float cv = fmaf(scale_g, acc[e], b1);
+      const float rho = cv * (ra * qb.y);
+      const float a = fminf(fabsf(rho), 1.0f);
+      const float th = acos_pos5(a, 1.0f - a);
+      const float pm = rho < 0.0f ? th : kPi - th;
+      const float sn = fsqrt(fmaf(-a, a, 1.0f));
+      vn[e] = ok ? fmaf(pm, rho, sn) * (sa * qb.x) : 0.0f;
+      vt[e] = ok ? cv * pm : 0.0f;
+    }
+  };
+
+  // Second-stage ReLU of one plane (row hh of image i, hp of image j) from
+  // its pre-activation sums (three w-summed first-stage planes, nngp and
+  // ntk) on the thread's 2x2 block; tables A (image i, row hh: (sq, rq) of
+  // columns 2W, 2W + 1) and B (image j, row hp).  2 pi times the state,
+  // zero outside the grid.
+  const float wv2 = wv9b * kInv2Pi, wv3 = wv9c * kInv2Pi;
+  auto relu2 = [&](bool ok, float4 A, float4 B, const float (&pn)[4],
+                   const float (&pt)[4], float (&vn)[4], float (&vt)[4]) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int c = e >> 1, d = e & 1;
+      const float sa = c ? A.z : A.x, ra = c ? A.w : A.y;
+      const float sb = d ? B.z : B.x, rb = d ? B.w : B.y;
+      const float cv = fmaf(wv2, pn[e], b2);
+      const float rho = cv * (ra * rb);
+      const float a = fminf(fabsf(rho), 1.0f);
+      const float th = acos_pos5(a, 1.0f - a);
+      const float pm = rho < 0.0f ? th : kPi - th;
+      const float sn = fsqrt(fmaf(-a, a, 1.0f));
+      vn[e] = ok ? fmaf(pm, rho, sn) * (sa * sb) : 0.0f;
+      if constexpr (NTK) {
+        const float tv = fmaf(wv2, pt[e], cv);
+        vt[e] = ok ? tv * pm : 0.0f;
+      } else {
+        vt[e] = 0.0f;
+      }
+    }
+  };
+  // (w, w')-diagonal 3-tap sums of K planes held in the block layout.
+  auto b_exchange = [&](auto kc, const float (*v)[4], float (*ws)[4]) {
+    constexpr int K = decltype(kc)::value;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      float* d = lp + k * kPL2 + lofs;
+      d[0] = v[k][0];
+      d[1] = v[k][1];
+      d[kLS2] = v[k][2];
+      d[kLS2 + 1] = v[k][3];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) read_diag3<32, kLS2>(lp + k * kPL2, lofs, ws[k]);
+  };
+  using K4 = std::integral_constant<int, 4>;
+  using K6 = std::integral_constant<int, 6>;
+
+  // Plane offsets: second-stage plane k of step H is (2H + ar[k],
+  // 2Hp + br[k]), first-stage plane k is one row further, (2H + 1 + ar[k],
+  // 2Hp + 1 + br[k]).  Diagonals: k = 0, 1 on 2D, 2, 3 on 2D + 1, 4, 5 on
+  // 2D - 1.
+  constexpr int ar[6] = {1, 2, 1, 2, 0, 1};
+  constexpr int br[6] = {1, 2, 0, 1, 1, 2};
+
+  const int nsweep = f12_nsweep(slot);
+  for (int sw = 0; sw < nsweep; ++sw) {
+  const int D = f12_diag(slot, sw);
+  const int Hs = D > 0 ? D : 0;
+  const int He = D < 0 ? H2 - 1 + D : H2 - 1;
+
+  // Loop-carried state of the sweep (updated in place; the normaliser
+  // tables are loaded where they are used, which keeps the carried state
+  // within the register budget of two waves per SIMD).
+  struct Carry {
+    float c1[NT][6][4];      // w-summed first-stage planes (see above)
+    float c2[NT][4][4];      // w-summed second-stage planes
+    floatx4 acc[6];          // patch Grams of the step's six planes
+  };
+  Carry A;
+  {
+    // Sweep start (H = Hs, Hp = Hs - D): first-stage planes on diagonal 2D
+    // rows 2Hs-2 .. 2Hs+1 (I0..I3), on 2D+1 rows 2Hs-1 .. 2Hs+1 (J0..J2),
+    // on 2D-1 rows 2Hs-2 .. 2Hs (K0..K2); second-stage planes (2Hs-1,
+    // 2Hps-1), (2Hs, 2Hps), (2Hs, 2Hps-1), (2Hs-1, 2Hps) from them.
+    const int h0 = 2 * Hs, g0 = 2 * (Hs - D);
+    const int hh[10] = {h0 - 2, h0 - 1, h0, h0 + 1,  h0 - 1, h0, h0 + 1,
+                        h0 - 2, h0 - 1, h0};
+    const int hp[10] = {g0 - 2, g0 - 1, g0, g0 + 1,  g0 - 2, g0 - 1, g0,
+                        g0 - 1, g0, g0 + 1};
+    float w1[NT][10][4];
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      float vn[5][4], vt[5][4];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) s1(hh[5 * half + k], hp[5 * half + k],
+                                     vn[k], vt[k]);
+      float ws[5][4];
+      s1_exchange<5>(lp, vn, wr, lofs, ws);
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) w1[0][5 * half + k][e] = ws[k][e];
+      if constexpr (NTK) {
+        s1_exchange<5>(lp, vt, wr, lofs, ws);
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) w1[NT - 1][5 * half + k][e] = ws[k][e];
+      }
+    }
+    // c1: I2, I3, J1, J2, K1, K2
+    constexpr int cidx[6] = {2, 3, 5, 6, 8, 9};
+#pragma unroll
+    for (int tt = 0; tt < NT; ++tt)
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) A.c1[tt][k][e] = w1[tt][cidx[k]][e];
+    // second-stage planes: rows (2Hs-1, 2Hps-1) = I0+I1+I2, (2Hs, 2Hps) =
+    // I1+I2+I3, (2Hs, 2Hps-1) = J0+J1+J2, (2Hs-1, 2Hps) = K0+K1+K2
+    constexpr int sa0[4] = {0, 1, 4, 7};
+    const int rh[4] = {h0 - 1, h0, h0, h0 - 1};
+    const int rp[4] = {g0 - 1, g0, g0 - 1, g0};
+    float pn[4][4], pt[4][4], vn[4][4], vt[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int b0 = sa0[k];
+        pn[k][e] = w1[0][b0][e] + w1[0][b0 + 1][e] + w1[0][b0 + 2][e];
+        pt[k][e] = w1[NT - 1][b0][e] + w1[NT - 1][b0 + 1][e] +
+                   w1[NT - 1][b0 + 2][e];
+      }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const bool ok = (unsigned)rh[k] < (unsigned)S &&
+                      (unsigned)rp[k] < (unsigned)S;
+      relu2(ok, ld_r(ri, rh[k], 2 * W), ld_r(rj, rp[k], 2 * Wp), pn[k],
+            pt[k], vn[k], vt[k]);
+    }
+    float ws[4][4];
+    b_exchange(K4(), vn, ws);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) A.c2[0][k][e] = ws[k][e];
+    if constexpr (NTK) {
+      b_exchange(K4(), vt, ws);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) A.c2[NT - 1][k][e] = ws[k][e];
+    }
+  }
+
+  auto ld_a = [&](int row, float4& lo, float4& hi) {
+    row = row < S ? row : S - 1;
+    lo = *reinterpret_cast<const float4*>(qi + row * S + w0);
+    hi = *reinterpret_cast<const float4*>(qi + row * S + w0 + 2);
+  };
+  auto ld_b = [&](int row) { return qj[(row < S ? row : S - 1) * S + wq]; };
+  // Patch Grams of the six first-stage planes of step Hn: image rows
+  // 2Hn + 1 + r (r = 0..2) of i and 2Hpn + 1 + r of j.
+  auto gram6 = [&](int Hn, floatx4 (&acc)[6]) {
+    const int Hpn = Hn - D;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acc[k] = {0.0f, 0.0f, 0.0f, 0.0f};
+    {
+      half8v ah[3], al[3], bh[3], bl[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        ah[r] = opnd8(0, 0, 2 * Hn + 1 + r);
+        bh[r] = opnd8(1, 0, 2 * Hpn + 1 + r);
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[ar[k]], bh[br[k]],
+                                                        acc[k], 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) bl[r] = opnd8(1, 1, 2 * Hpn + 1 + r);
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[ar[k]], bl[br[k]],
+                                                        acc[k], 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) al[r] = opnd8(0, 1, 2 * Hn + 1 + r);
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[ar[k]], bh[br[k]],
+                                                        acc[k], 0, 0, 0);
+    }
+    {
+      half8v ah[3], al[3], bh[3], bl[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        ah[r] = opnd4(0, 0, 2 * Hn + 1 + r);
+        bh[r] = opnd4(1, 0, 2 * Hpn + 1 + r);
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[ar[k]], bh[br[k]],
+                                                        acc[k], 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) bl[r] = opnd4(1, 1, 2 * Hpn + 1 + r);
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[ar[k]], bl[br[k]],
+                                                        acc[k], 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) al[r] = opnd4(0, 1, 2 * Hn + 1 + r);
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[ar[k]], bh[br[k]],
+                                                        acc[k], 0, 0, 0);
+    }
+  };
+  gram6(Hs, A.acc);
+
+  // First-stage ReLU of the step's planes (MASK: some plane is outside the
+  // grid).
+  auto relu1 = [&](const Carry& in, int H, auto mask, float (&vn)[6][4],
+                   float (&vt)[6][4]) {
+    constexpr bool MASK = decltype(mask)::value;
+    const int Hp = H - D;
+    float4 qal[3], qah[3];
+    float2 qbv[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      ld_a(2 * H + 1 + r, qal[r], qah[r]);
+      qbv[r] = ld_b(2 * Hp + 1 + r);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const bool ok = 2 * H + 1 + ar[k] < S && 2 * Hp + 1 + br[k] < S;
+      const float2 qb = qbv[br[k]];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float4 Q = e < 2 ? qal[ar[k]] : qah[ar[k]];
+        const float sa = (e & 1) ? Q.z : Q.x, ra = (e & 1) ? Q.w : Q.y;
+        const float cv = fmaf(scale_g, in.acc[k][e], b1);
+        const float rho = cv * (ra * qb.y);
+        const float a = fminf(fabsf(rho), 1.0f);
+        const float th = acos_pos5(a, 1.0f - a);
+        const float pm = rho < 0.0f ? th : kPi - th;
+        const float sn = fsqrt(fmaf(-a, a, 1.0f));
+        const float n = fmaf(pm, rho, sn) * (sa * qb.x);
+        const float tv = cv * pm;
+        vn[k][e] = MASK ? (ok ? n : 0.0f) : n;
+        vt[k][e] = MASK ? (ok ? tv : 0.0f) : tv;
+      }
+    }
+  };
+
+  auto step = [&](int H, Carry& cy) {
+    const int Hp = H - D;
+    float vn[6][4], vt[6][4];
+    if (2 * H + 3 < S && 2 * Hp + 3 < S)
+      relu1(cy, H, std::false_type(), vn, vt);
+    else
+      relu1(cy, H, std::true_type(), vn, vt);
+
+    // Stage 1 -> 2: w-sums of the new planes, h-window sums with the
+    // carried ones, then the second ReLU on the six second-stage planes.
+    float p2[NT][6][4];
+    auto adv1 = [&](const float (&v)[6][4], int tt) {
+      float ws[6][4];
+      s1_exchange<6>(lp, v, wr, lofs, ws);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        p2[tt][0][e] = cy.c1[tt][0][e] + cy.c1[tt][1][e] + ws[0][e];
+        p2[tt][1][e] = cy.c1[tt][1][e] + ws[0][e] + ws[1][e];
+        p2[tt][2][e] = cy.c1[tt][2][e] + cy.c1[tt][3][e] + ws[2][e];
+        p2[tt][3][e] = cy.c1[tt][3][e] + ws[2][e] + ws[3][e];
+        p2[tt][4][e] = cy.c1[tt][4][e] + cy.c1[tt][5][e] + ws[4][e];
+        p2[tt][5][e] = cy.c1[tt][5][e] + ws[4][e] + ws[5][e];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) cy.c1[tt][k][e] = ws[k][e];
+      }
+    };
+    adv1(vn, 0);
+    if constexpr (NTK) adv1(vt, NT - 1);
+    float v2n[6][4], v2t[6][4];
+    {
+      float4 r2i[3], r2j[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        r2i[r] = ld_r(ri, 2 * H + r, 2 * W);
+        r2j[r] = ld_r(rj, 2 * Hp + r, 2 * Wp);
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        const bool ok = 2 * H + ar[k] < S && 2 * Hp + br[k] < S;
+        relu2(ok, r2i[ar[k]], r2j[br[k]], p2[0][k], p2[NT - 1][k], v2n[k],
+              v2t[k]);
+      }
+    }
+    // Stage 2 -> 3: w-sums, h-window sums: P00, P11, P10, P01.
+    float pre[NT][4][4];
+    auto adv2 = [&](const float (*v)[4], int tt) {
+      float ws[6][4];
+      b_exchange(K6(), v, ws);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float z0 = cy.c2[tt][1][e] + ws[0][e];
+        pre[tt][0][e] = cy.c2[tt][0][e] + z0;
+        pre[tt][1][e] = z0 + ws[1][e];
+        pre[tt][2][e] = cy.c2[tt][2][e] + ws[2][e] + ws[3][e];
+        pre[tt][3][e] = cy.c2[tt][3][e] + ws[4][e] + ws[5][e];
+        cy.c2[tt][0][e] = ws[0][e];
+        cy.c2[tt][1][e] = ws[1][e];
+        cy.c2[tt][2][e] = ws[3][e];
+        cy.c2[tt][3][e] = ws[5][e];
+      }
+    };
+    adv2(v2n, 0);
+    if constexpr (NTK) adv2(v2t, NT - 1);
+    // The next step's matrix work overlaps the third stage (past the sweep
+    // end it computes planes that are never used).
+    gram6(H + 1, cy.acc);
+    // Third stage + pool, as the second stage of k_first_pool32.
+    float4 r3i[2], r3j[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      r3i[r] = ld_r(ui, 2 * H + r, 2 * W);
+      r3j[r] = ld_r(uj, 2 * Hp + r, 2 * Wp);
+    }
+    float acc_n = 0.0f, acc_t = 0.0f;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const float4 Ai = r3i[x >> 1];
+      const float sa = (x & 1) ? Ai.z : Ai.x, ra = (x & 1) ? Ai.w : Ai.y;
+      float inner = 0.0f;
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        const float4 Bj = r3j[y >> 1];
+        const float sb = (y & 1) ? Bj.z : Bj.x, rb = (y & 1) ? Bj.w : Bj.y;
+        const int pl = (x >> 1) == (y >> 1) ? (x >> 1) : 2 + (y >> 1);
+        const int e = 2 * (x & 1) + (y & 1);
+        const float cv = fmaf(wv3, pre[0][pl][e], b3);
+        const float tv = NTK ? fmaf(wv3, pre[NT - 1][pl][e], cv) : 0.0f;
+        const float rho = cv * (ra * rb);
+        const float a = fminf(fabsf(rho), 1.0f);
+        const float th = acos_pos5(a, 1.0f - a);
+        const float pm = rho < 0.0f ? th : kPi - th;
+        const float sn = fsqrt(fmaf(-a, a, 1.0f));
+        inner = fmaf(fmaf(pm, rho, sn), sb, inner);
+        if constexpr (NTK) acc_t = fmaf(pm, tv, acc_t);
+      }
+      acc_n = fmaf(inner, sa, acc_n);
+    }
+    const long long o =
+        p * PER_OUT + ((long long)(H * SO + Hp) * SO + W) * SO + Wp;
+    constexpr float kScale = kInv2Pi / 16.0f;
+    st_state(out_n + o, acc_n * kScale);
+    if constexpr (NTK) st_state(out_t + o, acc_t * kScale);
+  };
+
+  for (int H = Hs; H <= He; ++H) step(H, A);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_qnorm(const float* __restrict__ q,
                                                long long n,
                                                float2* __restrict__ out) {
@@ -1320,13 +1964,21 @@ int launch_first_fast(const float* x1, const float* x2, int n1, int n2, int S,
       (flags & (NTA_ACT_ERF | NTA_ACT_GELU | NTA_CONV_CIRCULAR)))
     return kNotHandled;
   if (C > 4 || (S != 32 && S != 16)) return kNotHandled;
+  const bool relu = flags & NTA_RELU;
+  const bool want_ntk = flags & NTA_NTK;
+  if (!relu && (want_ntk || S != 32)) return kNotHandled;
   const long long T = n2 == 0 ? (long long)n1 : (long long)n1 * n2;
   if (T * S > 0x7ffffff0LL)
     return set_error(NTA_EINVAL, "nta_first_stage: tile too large (%lld "
                      "pairs)", T);
   const int nblk = pad8(T * S);
   const float scale = w_var / (9.0f * (float)C);
-  const bool want_ntk = flags & NTA_NTK;
+  if (!relu) {
+    hipLaunchKernelGGL((k_first32<false, false>), dim3(nblk), dim3(256), 0,
+                       stream, x1, x2, n1, n2, C, nblk, scale, b_var, q1, q2,
+                       nngp, ntk);
+    return check_launch("nta_first_stage");
+  }
   if (S == 32) {
     if (want_ntk)
       hipLaunchKernelGGL(k_first32<true>, dim3(nblk), dim3(256), 0, stream,
@@ -1392,6 +2044,50 @@ extern "C" int nta_first_pool_stage(
                        (hipStream_t)stream, x1, x2, n1, n2, C, nblk, scale1,
                        b_var1, a1, a2, wv9, b_var2, c1, c2, out_nngp, out_ntk);
   return check_launch("nta_first_pool_stage");
+}
+
+// Conv(3x3 SAME) -> ReLU, three times, -> AvgPool(2x2 / 2) on 32x32 images
+// with C <= 4 channels, as one kernel (k_first3_pool32).
+extern "C" int nta_first3_pool_stage(
+    const float* x1, const float* x2, int n1, int n2, int H, int W, int C,
+    int flags, float w_var1, float b_var1, const float* qn1a,
+    const float* qn2a, float w_var2, float b_var2, const float* qn1b,
+    const float* qn2b, float w_var3, float b_var3, const float* qn1c,
+    const float* qn2c, float* out_nngp, float* out_ntk, void* stream) {
+  if (n1 < 0 || n2 < 0)
+    return set_error(NTA_EINVAL, "nta_first3_pool_stage: bad n1=%d n2=%d",
+                     n1, n2);
+  if (H != 32 || W != 32 || C < 1 || C > 4)
+    return set_error(NTA_EUNSUPPORTED, "nta_first3_pool_stage: needs 32x32 "
+                     "images with 1..4 channels (got %dx%dx%d)", H, W, C);
+  if (n1 == 0) return NTA_OK;
+  const bool ntk = flags & NTA_NTK;
+  if (!x1 || (n2 > 0 && !x2) || !qn1a || !qn1b || !qn1c ||
+      (n2 > 0 && (!qn2a || !qn2b || !qn2c)) || !out_nngp ||
+      (ntk && !out_ntk))
+    return set_error(NTA_EINVAL, "nta_first3_pool_stage: null pointer for a "
+                     "requested operand");
+  const long long T = n2 == 0 ? (long long)n1 : (long long)n1 * n2;
+  if (T * kF12Slots > 0x7ffffff0LL)
+    return set_error(NTA_EINVAL, "nta_first3_pool_stage: tile too large "
+                     "(%lld pairs)", T);
+  const int nblk = (int)((T * kF12Slots + 7) / 8 * 8);
+  const float scale1 = w_var1 / (9.0f * (float)C);
+  const float wv9b = w_var2 / 9.0f, wv9c = w_var3 / 9.0f;
+  auto f2 = [](const float* q) { return reinterpret_cast<const float2*>(q); };
+  if (ntk)
+    hipLaunchKernelGGL(k_first3_pool32<true>, dim3(nblk), dim3(256), 0,
+                       (hipStream_t)stream, x1, x2, n1, n2, C, nblk, scale1,
+                       b_var1, f2(qn1a), f2(qn2a), wv9b, b_var2, f2(qn1b),
+                       f2(qn2b), wv9c, b_var3, f2(qn1c), f2(qn2c), out_nngp,
+                       out_ntk);
+  else
+    hipLaunchKernelGGL(k_first3_pool32<false>, dim3(nblk), dim3(256), 0,
+                       (hipStream_t)stream, x1, x2, n1, n2, C, nblk, scale1,
+                       b_var1, f2(qn1a), f2(qn2a), wv9b, b_var2, f2(qn1b),
+                       f2(qn2b), wv9c, b_var3, f2(qn1c), f2(qn2c), out_nngp,
+                       out_ntk);
+  return check_launch("nta_first3_pool_stage");
 }
 
 // (sqrt(q), 1 / sqrt(q)) per element, (0, 0) where q <= 0.

@@ -373,8 +373,7 @@ extern "C" int nta_first_stage(const float* x1, const float* x2, int n1,
       ((flags & NTA_RELU) && (!q1 || (n2 > 0 && !q2))))
     return set_error(NTA_EINVAL, "nta_first_stage: null pointer for a "
                      "requested operand");
-  if ((flags & NTA_RELU) && !(flags & NTA_CONV_VALID) && fh == 3 &&
-      fw == 3 && H == W) {
+  if (!(flags & NTA_CONV_VALID) && fh == 3 && fw == 3 && H == W) {
     int rc2 = launch_first_fast(x1, x2, n1, n2, H, C, flags, w_var, b_var,
                                 act_k1, act_k0, q1, q2, nngp, ntk,
                                 (hipStream_t)stream);
@@ -434,6 +433,14 @@ extern "C" int nta_stage(const float* in_nngp, const float* in_ntk, int n1,
     return set_error(NTA_EINVAL, "nta_stage: pool window %dx%d larger than "
                      "%dx%d", ph, pw, Hc, Wc);
   int Ho = Hc / ph, Wo = Wc / pw;
+  if ((flags & NTA_CONV) && !(flags & (NTA_RELU | NTA_NTK | NTA_CONV_VALID |
+                                        NTA_CONV_CIRCULAR)) &&
+      fh == 3 && fw == 3 && H == W && ph == 1 && pw == 1) {
+    const long long T = n2 == 0 ? (long long)n1 : (long long)n1 * n2;
+    int rc2 = launch_stencil_fast(in_nngp, T, H, w_var, b_var, out_nngp,
+                                  (hipStream_t)stream);
+    if (rc2 != kNotHandled) return rc2;
+  }
   if ((flags & NTA_CONV) && (flags & NTA_RELU) && !(flags & NTA_CONV_VALID) &&
       fh == 3 && fw == 3 && H == W && ph == pw) {
     int rc2 = launch_stage_fast(in_nngp, in_ntk, n1, n2, H, ph, flags, w_var,

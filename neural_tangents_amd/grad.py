@@ -70,23 +70,28 @@ NO_SELF = 2 ** 31 - 1
 
 def _relu_bwd(aN, aT, n1, n2, S, pool, w_var, b_var, q1, q2, gN, gT,
               stream, self_off=NO_SELF, act=(1.0, 0.0)):
+  """Adjoint of one stage's ReLU (+ pool).  Returns (gaN, gaT, gqa, gqb,
+  gw, gb) with gw, gb the d/dw_var, d/db_var contributions as 0-d FP64
+  device tensors (summed from the kernel's per-workgroup partials, with no
+  host synchronisation)."""
   T = aN.shape[0]
   gaN = torch.empty_like(aN)
   gaT = torch.empty_like(aN) if aT is not None else None
   gqa = torch.empty_like(aN)
   gqb = torch.empty_like(aN)
-  gwv = torch.empty_like(aN)
-  gbv = torch.empty_like(aN)
+  parts = _lib.value('nta_relu_bwd_parts', n1, n2, S)
+  gwb = torch.empty((max(parts, 1), 2), dtype=torch.float64,
+                    device=aN.device)
   _lib.call('nta_relu_bwd', _lib.ptr(aN), _lib.ptr(aT), n1, n2, S, pool,
             self_off, w_var, b_var, act[0], act[1], _lib.ptr(q1),
             _lib.ptr(q2), _lib.ptr(gN),
             _lib.ptr(gT), _lib.ptr(gaN), _lib.ptr(gaT), _lib.ptr(gqa),
-            _lib.ptr(gqb), _lib.ptr(gwv), _lib.ptr(gbv), stream)
+            _lib.ptr(gqb), _lib.ptr(gwb), stream)
   v = lambda t: t.reshape(T, S, S, S, S)   # (h, h', w, w')
   gqa_img = v(gqa).sum(dim=(2, 4))          # over (h', w') -> [T, S, S]
   gqb_img = v(gqb).sum(dim=(1, 3))          # over (h, w)
-  return (gaN, gaT, gqa_img, gqb_img, float(gwv.double().sum()),
-          float(gbv.double().sum()))
+  sums = gwb[:parts].sum(dim=0)
+  return gaN, gaT, gqa_img, gqb_img, sums[0], sums[1]
 
 
 def _check_plan(plan):
@@ -131,8 +136,9 @@ def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=64):
                                             stream)[0]
   gq1 = [torch.zeros_like(q) for q in q1s]
   gq2 = gq1 if sym else [torch.zeros_like(q) for q in q2s]
-  gwv = [0.0] * L
-  gbv = [0.0] * L
+  # d/dw_var, d/db_var per stage, accumulated on the device (FP64)
+  gwv = [torch.zeros((), dtype=torch.float64, device=dev) for _ in range(L)]
+  gbv = [torch.zeros((), dtype=torch.float64, device=dev) for _ in range(L)]
   gx1 = torch.zeros_like(x1)
   gx2 = gx1 if sym else torch.zeros_like(x2)
   first = plan.stages[0]
@@ -201,7 +207,9 @@ def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=64):
   # Self chains: push the normaliser cotangents through each image's chain.
   for x, gq, gx in ((x1, gq1, gx1),) + (() if sym else ((x2, gq2, gx2),)):
     _self_chain_vjp(plan, x, gq, gx, gwv, gbv, stream, C, fh, fw)
-  return gx1, (None if sym else gx2), gwv, gbv
+  sums = torch.stack(gwv + gbv).cpu()       # one synchronisation
+  return (gx1, (None if sym else gx2), [float(v) for v in sums[:L]],
+          [float(v) for v in sums[L:]])
 
 
 def _first_gram(xa, xb, n1, n2, S, C, fh, fw, stream):
@@ -250,8 +258,8 @@ def _self_chain_vjp(plan, x, gq, gx, gwv, gbv, stream, C, fh, fw,
       diag_a = an[:, idx[:, None], idx[:, None], idx[None, :], idx[None, :]]
       v[:, idx[:, None], idx[:, None], idx[None, :], idx[None, :]] += \
           w_var * gq_tot
-      gwv[si] += dw + float((gq_tot.double() * diag_a.double()).sum())
-      gbv[si] += db + float(gq_tot.double().sum())
+      gwv[si] += dw + (gq_tot * diag_a).sum(dtype=torch.float64)
+      gbv[si] += db + gq_tot.sum(dtype=torch.float64)
       if si > 0:
         gR = _stencil(gaN, n, S, fh_s, fw_s, valid, stream)
       else:

@@ -30,7 +30,7 @@ def _net(b1, b2, readout):
 def _both(monkeypatch, kfn, x1, x2, get):
   fused = kfn(x1, x2, get)
   with monkeypatch.context() as m:
-    m.setattr(_engine, '_fuses_first_two', lambda plan, shape: False)
+    m.setattr(_engine, '_fused_stages', lambda plan, shape: 0)
     plain = kfn(x1, x2, get)
   return fused, plain
 

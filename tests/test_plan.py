@@ -119,9 +119,18 @@ def test_first_two_stage_fusion_selection():
     p = _engine.compile_plan(stax.myrtle(depth)[2].specs, (1, 32, 32, 3))
     assert _engine._fuses_first_two(p, (1, 32, 32, 3))
     assert not _engine._fuses_first_two(p, (1, 32, 32, 5))
-  # Myrtle-10 has a third Conv before the first pool
+  # Myrtle-10 has a third Conv before the first pool: the three-stage
+  # launch takes it
   p = _engine.compile_plan(stax.myrtle(10)[2].specs, (1, 32, 32, 3))
   assert not _engine._fuses_first_two(p, (1, 32, 32, 3))
+  assert _engine._fuses_first_three(p, (1, 32, 32, 3))
+  assert _engine._fused_stages(p, (1, 32, 32, 3)) == 3
+  assert _engine._fused_stages(p, (1, 32, 32, 5)) == 0
+  assert _engine._per_pair_bytes(p, True, 3) == 4 * 16 ** 4 * 2 * 2
+  for depth in (5, 7):
+    p = _engine.compile_plan(stax.myrtle(depth)[2].specs, (1, 32, 32, 3))
+    assert not _engine._fuses_first_three(p, (1, 32, 32, 3))
+    assert _engine._fused_stages(p, (1, 32, 32, 3)) == 2
   p = _engine.compile_plan(stax.myrtle(5)[2].specs, (1, 16, 16, 3))
   assert not _engine._fuses_first_two(p, (1, 16, 16, 3))
   conv = lambda: stax.Conv(1, (3, 3), padding='SAME')
