@@ -235,7 +235,7 @@ def krr_result(ntr, nte, reps, tol=1e-7):
       t = time.perf_counter()
       s = _solve.CholeskySolver(k_dd, 1e-4, False, mode, tol=tol)
       alpha = s.solve(y)
-      f = k_td.to(torch.float64) @ alpha
+      f = _solve.kernel_times(k_td, alpha)
       torch.cuda.synchronize()
       times.append(time.perf_counter() - t)
     res[mode] = (sorted(times)[len(times) // 2], alpha, f,
@@ -257,7 +257,8 @@ def krr_result(ntr, nte, reps, tol=1e-7):
       'plain_fp64_tflops': chol_flop / t64 / 1e12,
       'plain_fp64_note': 'torch.linalg.cholesky_ex + cholesky_solve in FP64 '
                          '(rocSOLVER); tflops counts n^3/3 over the whole '
-                         'solve + predict time',
+                         'solve + predict time; both forms predict with '
+                         'nta_gemm_f32_f64 (fp32 k_td, FP64 alpha)',
       'pcg_iterations': iters, 'final_rel_residual': resid,
       'pcg_tol': tol,
       'pcg_tol_note': 'refinement stops at estimated relative error '

@@ -140,6 +140,28 @@ int nta_first_pool_stage(const float* x1, const float* x2, int n1, int n2,
                          void* stream);
 
 /*
+ * The first three stages of a Myrtle-10-type network as one launch on a
+ * tile of pairs: [Conv(3x3, SAME) -> Relu] x 3 -> AvgPool(2x2, stride 2) on
+ * 32x32 images with 1..4 channels; neither the first nor the second
+ * stage's [32][32][32][32] state is written.  Same result as
+ * nta_first_stage followed by two nta_stage calls (within FP32 rounding).
+ * Replaces: the reference functions of those entry points, for
+ * stax.serial(Conv, Relu, Conv, Relu, Conv, Relu, AvgPool)
+ * (stax/combinators.py:40-68).
+ * Arguments as nta_first_pool_stage, with qn1c, qn2c the (sqrt(q),
+ * 1/sqrt(q)) tables of the third ReLU and w_var3, b_var3 its conv's.
+ * Outputs: [T][16][16][16][16].  Returns NTA_EUNSUPPORTED for other image
+ * shapes.
+ */
+int nta_first3_pool_stage(const float* x1, const float* x2, int n1, int n2,
+                          int H, int W, int C, int flags, float w_var1,
+                          float b_var1, const float* qn1a, const float* qn2a,
+                          float w_var2, float b_var2, const float* qn1b,
+                          const float* qn2b, float w_var3, float b_var3,
+                          const float* qn1c, const float* qn2c,
+                          float* out_nngp, float* out_ntk, void* stream);
+
+/*
  * out[2k] = sqrt(q[k]), out[2k + 1] = 1 / sqrt(q[k]) (both 0 where
  * q[k] <= 0), for k < n: the per-position factors of the ReLU map
  * (elementwise.py:444-455 divides the covariance by sqrt(q1 q2)).
@@ -169,18 +191,24 @@ int nta_head(const float* nngp, const float* ntk, int n1, int n2, int H,
  * ntk inputs [T][S][S][S][S] (aT = NULL: first stage, ntk input 0); gN, gT:
  * pooled cotangents [T][S/p]^4.  Writes the cotangents of aN (gaN, already
  * times w_var), aT, of the normalisers per element (gqa, gqb; to be summed
- * over the partner axes) and the per-element terms of d/dw_var (gwv) and
- * d/db_var (gbv).  Pairs (a, b) with a + self_off == b (and every pair of a
- * self tile, n2 == 0) are pairs of identical images: their spatial diagonal
- * is treated as rho = 1 exactly.  Pass self_off = INT_MAX for none.
- * act_k1, act_k0: the ABRelu constants, as in nta_first_stage.
+ * over the partner axes) and, into gwb (device doubles, 2 per workgroup:
+ * nta_relu_bwd_parts(n1, n2, S) pairs), the workgroups' partial sums of the
+ * d/dw_var and d/db_var terms (the caller adds the slots: a fixed-order,
+ * deterministic reduction).  Pairs (a, b) with a + self_off == b (and every
+ * pair of a self tile, n2 == 0) are pairs of identical images: their spatial
+ * diagonal is treated as rho = 1 exactly.  Pass self_off = INT_MAX for
+ * none.  act_k1, act_k0: the ABRelu constants, as in nta_first_stage.
  */
 int nta_relu_bwd(const float* aN, const float* aT, int n1, int n2, int S,
                  int pool, int self_off, float w_var, float b_var,
                  float act_k1, float act_k0, const float* q1,
                  const float* q2, const float* gN, const float* gT,
-                 float* gaN, float* gaT, float* gqa, float* gqb, float* gwv,
-                 float* gbv, void* stream);
+                 float* gaN, float* gaT, float* gqa, float* gqb, double* gwb,
+                 void* stream);
+
+/* Partial-sum slots (pairs of doubles) of an nta_relu_bwd call on a tile of
+ * n1 x n2 pairs (n2 == 0: n1 self pairs) of side S; -1 for bad sizes. */
+long long nta_relu_bwd_parts(int n1, int n2, int S);
 
 /*
  * Diagonal-spatial chain: a network of nstage [Conv fh x fw, stride 1]
@@ -227,8 +255,12 @@ int nta_chol_split(int n);
 
 /*
  * FP32 Cholesky of A (lower triangle read; A is overwritten with
- * intermediates) by recursive blocking with fp32 MFMA products, with the
- * factor inverted on the way.  W (n x n) must be zero on entry.  On return,
+ * intermediates): right-looking, blocked on 128 columns (leaves factored
+ * and inverted in LDS) and 512-column panels, with fp32 MFMA products; the
+ * trailing updates overlap the next panel on a second stream of the
+ * library's (events order it against `stream`, which the call returns on).
+ * Each half of the factor is then inverted by recursive doubling.  W
+ * (n x n) must be zero on entry.  On return,
  * with L L^T = A and n1 = nta_chol_split(n):
  *   W[0:n1, 0:n1] = L11^-1, W[n1:n, n1:n] = L22^-1 (lower, zeros above),
  *   W[n1:n, 0:n1] = L21,
@@ -259,14 +291,18 @@ int nta_chol_precond(const float* W, const float* WT, int n, long long ldw,
  * Solves (K + lam I) X = B by conjugate gradients preconditioned with the
  * FP32 factor in W, WT (nta_chol_f32 of K + shift I), FP64 vectors, FP64
  * products on fp64 MFMA (K read in fp32, exact) and the FP64 shift lam.  B, X
- * [n][nrhs] fp64, nrhs <= 16.  Iterates until every column's relative
- * residual ||r|| / ||b|| is <= tol, the residual recomputed every 10
- * iterations stops improving for 50 iterations, or max_iter is reached;
- * *iters receives the iteration count and *resid the
- * largest relative residual of the returned X, recomputed from scratch
- * (host pointers, may be NULL).  Synchronises the stream once per
- * iteration to test convergence (reading 16 doubles reduced on the
- * device).
+ * [n][nrhs] fp64, nrhs <= 16.  Iterates until, for every column, the
+ * relative residual ||r|| / ||b|| or the estimated relative error
+ * ||M^-1 r|| / ||x|| (M = L L^T the FP32 factor, so M^-1 r is the error of
+ * x to the factor's accuracy) is <= tol -- tested on the recurrence
+ * residual and confirmed on the residual recomputed as B - (K + lam I) X --
+ * or the recomputed residual stops improving over 5 checkpoints (every 8
+ * iterations), or max_iter is reached.  *iters receives the iteration count
+ * and *resid the largest relative residual of the returned X, recomputed
+ * from scratch (host pointers, may be NULL).  Iterations run in batches of
+ * 4 without host synchronisation (the kernels of an iteration after
+ * convergence return at once); the stream is synchronised once per batch
+ * and per recomputed residual.
  * Replaces: cho_solve (predict.py:1254) of the regularised train kernel.
  */
 int nta_pcg64(const float* K, int n, long long ldk, double lam,
@@ -290,9 +326,21 @@ int nta_pcg64_k64(const double* K, int n, long long ldk, double lam,
                   double* resid, void* stream);
 
 /*
+ * Y = K X in FP64 for an fp32 m x n matrix K (row-major, leading dimension
+ * ldk, 16-byte aligned) and X [n][16] fp64 (up to 16 columns, zero padded):
+ * K's entries are exact in FP64 and the products accumulate on fp64 MFMA.
+ * Y [m][16] fp64.  m, n and ldk must be multiples of 32, 16 and 4
+ * (NTA_EUNSUPPORTED otherwise: pad).
+ * Replaces: the test-set mean jnp.tensordot(k_td, k_inv_y, ...)
+ * (predict.py:696) with the FP32 test-train kernel and the FP64 solution.
+ */
+int nta_gemm_f32_f64(const float* K, long long ldk, int m, int n,
+                     const double* X, double* Y, void* stream);
+
+/*
  * nta_pcg64 without a workspace argument: the iteration's vectors live in a
  * static device array of the library (one per GPU, held until the process
- * ends; about 19 MiB, enough for n <= 16384 and nrhs <= 16), so the call
+ * ends; 19,792,896 bytes, enough for n <= 16384 and nrhs <= 16), so the call
  * allocates nothing.  n > 16384 returns NTA_EINVAL.  Calls on one GPU share
  * that array: calls that overlap on one GPU (other streams or threads)
  * race.  Arguments and results as nta_pcg64.

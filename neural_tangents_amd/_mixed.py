@@ -3,10 +3,10 @@ FP64 residuals, preconditioned conjugate gradients.
 
 For ``A = K + lambda I`` (``predict.py:1224-1258``):
 
-1. ``nta_chol_f32`` factors ``fp32(K) + shift I`` in FP32 by recursive
-   blocking on fp32 MFMA and inverts the factor on the way; the
-   preconditioner ``(L L^T)^-1`` is applied from the blocks it leaves
-   (``nta_chol_precond``).
+1. ``nta_chol_f32`` factors ``fp32(K) + shift I`` in FP32 (right-looking
+   blocked Cholesky on fp32 MFMA, 128-column leaves inverted in LDS) and
+   inverts each half of the factor; the preconditioner ``(L L^T)^-1`` is
+   applied from the blocks it leaves (``nta_chol_precond``).
 2. ``nta_pcg64`` runs conjugate gradients on ``A`` with FP64 vectors:
    ``A p`` reads the kernel as given -- fp32 (exact in FP64) or, for a
    float64 kernel, its FP64 entries (``nta_pcg64_k64``) -- accumulates in
@@ -72,7 +72,7 @@ class MixedCholesky:
     # non-positive pivot: the kernel is too ill-conditioned for single
     # precision at this lambda), the shift grows by 10x; the FP64-residual
     # iteration still converges to the solution of the unshifted system.
-    mean_diag = float(torch.diagonal(K32)[:n].double().mean())
+    mean_diag = None
     shift = self.lam
     info = torch.zeros((1,), dtype=torch.int32, device=self.device)
     for attempt in range(8):
@@ -85,6 +85,8 @@ class MixedCholesky:
                 _lib.ptr(WT), _lib.ptr(info), stream)
       if int(info.item()) == 0:
         break
+      if mean_diag is None:
+        mean_diag = float(torch.diagonal(K32)[:n].double().mean())
       shift = max(10.0 * shift, 1e-7 * mean_diag)
     else:
       raise ValueError(

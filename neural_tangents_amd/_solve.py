@@ -30,6 +30,43 @@ def regularizer_value(A: torch.Tensor, diag_reg: float,
   return float(diag_reg) * trace / A.shape[0]
 
 
+def kernel_times(K: torch.Tensor, X: torch.Tensor) -> torch.Tensor:
+  """``K @ X`` in FP64: the test-set mean ``tensordot(k_td, k_inv_y)``
+  (``predict.py:696``) of a test-train kernel ``K`` [M, N] and the FP64
+  solution ``X`` [N, C].
+
+  An fp32 kernel on the GPU goes through ``nta_gemm_f32_f64`` (its entries
+  are exact in FP64; products accumulate on fp64 MFMA), padded to the
+  kernel's 32-row / 16-column granularity when needed; a float64 kernel is
+  multiplied as given.
+  """
+  if K.ndim != 2 or X.ndim != 2 or K.shape[1] != X.shape[0]:
+    raise ValueError(f'shapes {tuple(K.shape)} and {tuple(X.shape)} do not '
+                     'multiply')
+  if K.dtype != torch.float32 or not K.is_cuda:
+    return K.to(X.device, torch.float64) @ X.to(torch.float64)
+  m, n = K.shape
+  X = X.to(K.device, torch.float64)
+  mp, npad = -(-m // 32) * 32, -(-n // 16) * 16
+  if (mp == m and npad == n and K.is_contiguous() and n % 4 == 0 and
+      K.data_ptr() % 16 == 0):
+    Kp = K
+  else:
+    Kp = torch.zeros((mp, npad), dtype=torch.float32, device=K.device)
+    Kp[:m, :n] = K
+  stream = _lib.stream_handle(K.device)
+  cols = []
+  for c0 in range(0, X.shape[1], 16):
+    xc = X[:, c0:c0 + 16]
+    Xp = torch.zeros((npad, 16), dtype=torch.float64, device=K.device)
+    Xp[:n, :xc.shape[1]] = xc
+    Y = torch.empty((mp, 16), dtype=torch.float64, device=K.device)
+    _lib.call('nta_gemm_f32_f64', _lib.ptr(Kp), npad, mp, npad,
+              _lib.ptr(Xp), _lib.ptr(Y), stream)
+    cols.append(Y[:m, :xc.shape[1]])
+  return torch.cat(cols, dim=1) if len(cols) > 1 else cols[0]
+
+
 class CholeskySolver:
   """Factor once, then ``solve(b)`` for ``b`` of shape ``[N, K]``."""
 

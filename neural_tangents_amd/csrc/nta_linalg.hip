@@ -1,24 +1,17 @@
 // Dense linear algebra of the kernel-regression solve
 // (predict.py:1235-1258: cho_factor / cho_solve of K + lambda I).
 //
-// Mixed precision.  K + shift I is factored in FP32 by a recursive blocked
-// Cholesky that also inverts the factor as it goes: for a diagonal block
-//   [A11 A21^T; A21 A22] = [L11 0; L21 L22] [L11 0; L21 L22]^T
-// with W = L^-1,
-//   W11 = node(A11)                       (recursion)
-//   S   = A21 W11^T            (= L21)
-//   A22 -= S S^T
-//   T   = S W11                (into the A21 slot)
-//   W22 = node(A22)                       (recursion)
-//   W21 = -W22 T
-// All products run on one FP32 MFMA GEMM kernel (v_mfma_f32_32x32x2f32,
-// 128x128 or 64x64 tiles) that skips the zero blocks of triangular
-// operands; 128x128 nodes run as one workgroup in LDS (k_leaf128), whose
-// 64x64 halves are factored and inverted by one wavefront.  At the top level the
-// last two products are skipped: W keeps W11 = L11^-1, W22 = L22^-1 on its
-// diagonal blocks and S = L21 below them, and the preconditioner
-// M^-1 = (L L^T)^-1 is applied as two block triangular solves built from
-// those three blocks.
+// Mixed precision.  K + shift I is factored in FP32 by a right-looking
+// blocked Cholesky on 128-column blocks (blocked factorisation below): a
+// one-workgroup leaf factors and inverts each diagonal block in LDS, so the
+// triangular solve of the block column under it is a product with the
+// leaf's inverse, and the trailing updates are products too.  All products
+// run on one FP32 MFMA GEMM kernel (v_mfma_f32_32x32x2f32, 64x64 tiles)
+// that skips the zero blocks of triangular operands.  The
+// factor's inverse is then assembled inside each half of the matrix by
+// recursive doubling, so W keeps W11 = L11^-1, W22 = L22^-1 on its diagonal
+// blocks and L21 below them, and the preconditioner M^-1 = (L L^T)^-1 is
+// applied as two block triangular solves built from those three blocks.
 //
 // Refinement: preconditioned conjugate gradients on (K + lambda I) with
 // FP64 vectors.  (K + lambda I) p reads K in fp32 (exact in fp64) and
@@ -30,6 +23,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -42,7 +36,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 // ---------------------------------------------------------------- GEMM
-// C = alpha A op(B) + beta C on one 128x128 tile per workgroup.
+// C = alpha A op(B) + beta C on one TILE x TILE tile per workgroup.
 //   A: M x K row-major (element (m, k) at A[m * lda + k]).
 //   B: bkm = 0: N x K row-major, op(B) = B^T;  bkm = 1: K x N row-major.
 //   tri = 1: B (bkm = 0) is lower triangular (B[n][k] = 0 for k > n);
@@ -59,7 +53,10 @@ struct GemmJob {
   int M, N, K;
   float alpha, beta;
   int bkm, tri, lower;
-  int tiles_n, ntiles;
+  int tiles_m, tiles_n;    // tile rows and columns of the output
+  int ntiles;              // output tiles of one batch member (or range)
+  int batch;               // members of identical shape at fixed strides
+  long long sA, sB, sC;    // element strides between members
 };
 struct GemmJobs {
   GemmJob j[2];
@@ -223,14 +220,24 @@ __device__ __forceinline__ void gemm_tile(const GemmJob& J, int ti, int tj,
       }
 }
 
+// Tile bt of a two-job launch (workgroups loop over the tiles when the grid
+// is capped below the tile count).
 template <int TILE>
-__global__ __launch_bounds__(256) void k_gemm(GemmJobs jobs) {
-  __shared__ __align__(16) float As[2 * TILE * GSR];
-  __shared__ __align__(16) float Bs[2 * TILE * GSR];
-  int b = blockIdx.x;
-  const bool second = b >= jobs.j[0].ntiles;
-  const GemmJob J = second ? jobs.j[1] : jobs.j[0];
-  if (second) b -= jobs.j[0].ntiles;
+__device__ __forceinline__ void gemm_jobs_tile(const GemmJobs& jobs, int bt,
+                                               float* __restrict__ As,
+                                               float* __restrict__ Bs) {
+  int b = bt;
+  const int n0 = jobs.j[0].ntiles * jobs.j[0].batch;
+  const bool second = b >= n0;
+  GemmJob J = second ? jobs.j[1] : jobs.j[0];
+  if (second) b -= n0;
+  const int bi = b / J.ntiles;
+  b -= bi * J.ntiles;
+  if (bi) {
+    J.A += bi * J.sA;
+    J.B += bi * J.sB;
+    J.C += bi * J.sC;
+  }
   int ti, tj;
   if (J.lower) {
     ti = (int)((sqrtf(8.0f * b + 1.0f) - 1.0f) * 0.5f);
@@ -241,7 +248,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmJobs jobs) {
     // Triangular operands give tiles of unequal k-length: dispatch the
     // longest first (tri 1: long for large tj; tri 3: for large ti), so the
     // in-order dispatch balances the CUs.
-    const int tm = J.ntiles / J.tiles_n;
+    const int tm = J.tiles_m;
     if (J.tri == 1) {
       tj = J.tiles_n - 1 - b / tm;
       ti = b % tm;
@@ -260,25 +267,29 @@ __global__ __launch_bounds__(256) void k_gemm(GemmJobs jobs) {
     gemm_tile<TILE, true>(J, ti, tj, As, Bs);
   else
     gemm_tile<TILE, false>(J, ti, tj, As, Bs);
+  __syncthreads();                     // LDS free for the next tile
 }
 
-// Jobs of up to 512 x 512 outputs use 64x64 tiles (more workgroups, no
-// MFMA work on empty quadrants); larger ones 128x128 unless that leaves
-// fewer than NTA_CHOL_T64_BELOW workgroups (one 128 tile per CU is
-// resident, so a short grid idles most of the 256 CUs).
-#ifndef NTA_CHOL_T64_BELOW
-#define NTA_CHOL_T64_BELOW 512
-#endif
-static int tile_for(int M, int N) {
-  if (M <= 512 && N <= 512) return 64;
-  const int t128 = ((M + 127) / 128) * ((N + 127) / 128);
-  return t128 < NTA_CHOL_T64_BELOW ? 64 : 128;
+template <int TILE>
+__global__ __launch_bounds__(256) void k_gemm(GemmJobs jobs) {
+  __shared__ __align__(16) float As[2 * TILE * GSR];
+  __shared__ __align__(16) float Bs[2 * TILE * GSR];
+  const int total = jobs.j[0].ntiles * jobs.j[0].batch +
+                    jobs.j[1].ntiles * jobs.j[1].batch;
+  for (int bt = blockIdx.x; bt < total; bt += gridDim.x)
+    gemm_jobs_tile<TILE>(jobs, bt, As, Bs);
 }
+
+// The tile size of every launch: 64x64 tiles keep two or more workgroups
+// on a CU (a 128x128 tile's accumulators and prefetch registers fill a CU
+// with one), which measured faster on every shape the factorisation uses.
+constexpr int kTile = 64;
 
 static GemmJob make_job(const float* A, long long lda, const float* B,
                         long long ldb, float* C, long long ldc, int M, int N,
                         int K, float alpha, float beta, int bkm, int tri,
-                        int lower, int tile) {
+                        int lower) {
+  constexpr int tile = kTile;
   GemmJob j;
   j.A = A;
   j.B = B;
@@ -295,13 +306,26 @@ static GemmJob make_job(const float* A, long long lda, const float* B,
   j.tri = tri;
   j.lower = lower;
   const int tm = (M + tile - 1) / tile;
+  j.tiles_m = tm;
   j.tiles_n = (N + tile - 1) / tile;
   j.ntiles = lower ? tm * (tm + 1) / 2 : tm * j.tiles_n;
+  j.batch = 1;
+  j.sA = j.sB = j.sC = 0;
   return j;
 }
 
-static void launch_gemm(const GemmJob& a, const GemmJob* b, int tile,
-                        hipStream_t st) {
+// The job repeated for `batch` members whose operands lie sA, sB, sC
+// elements apart.
+static GemmJob batched(GemmJob j, int batch, long long sA, long long sB,
+                       long long sC) {
+  j.batch = batch;
+  j.sA = sA;
+  j.sB = sB;
+  j.sC = sC;
+  return j;
+}
+
+static void launch_gemm(const GemmJob& a, const GemmJob* b, hipStream_t st) {
   GemmJobs jobs;
   jobs.j[0] = a;
   if (b)
@@ -310,11 +334,10 @@ static void launch_gemm(const GemmJob& a, const GemmJob* b, int tile,
     jobs.j[1] = a;
     jobs.j[1].ntiles = 0;
   }
-  const int grid = jobs.j[0].ntiles + jobs.j[1].ntiles;
-  if (tile == 64)
-    hipLaunchKernelGGL(k_gemm<64>, dim3(grid), dim3(256), 0, st, jobs);
-  else
-    hipLaunchKernelGGL(k_gemm<128>, dim3(grid), dim3(256), 0, st, jobs);
+  int grid = jobs.j[0].ntiles * jobs.j[0].batch +
+             jobs.j[1].ntiles * jobs.j[1].batch;
+  if (grid <= 0) return;
+  hipLaunchKernelGGL(k_gemm<kTile>, dim3(grid), dim3(256), 0, st, jobs);
 }
 
 // ---------------------------------------------------------------- leaves
@@ -328,7 +351,7 @@ static void launch_gemm(const GemmJob& a, const GemmJob* b, int tile,
 //      substitution (one column of one block per lane), the blocks below
 //      them by two 16x16 MFMA products;
 //   5. W21 = -L22^-1 (L21 L11^-1) on MFMA.
-// k_leaf64 runs this on a 64 block; k_leaf128 runs it twice inside a 128
+// k_leaf64 runs this on a 64 block; leaf128_body runs it twice inside a 128
 // block, with the products that join the two halves on four wavefronts, so
 // a 128 block is one launch instead of five.
 constexpr int LP = 33;                      // LDS row stride of 32x32 tiles
@@ -625,18 +648,17 @@ __global__ __launch_bounds__(64) void k_leaf64(const float* __restrict__ A,
   block64_out<64>(W + (long long)k0 * ldw + k0, ldw, X, false, t);
 }
 
-// The 128x128 diagonal block at (k0, k0) as one recursion node:
+// The 128x128 diagonal block at (k0, k0), factored and inverted as two
+// 64x64 halves:
 // W11 = L11^-1, S = A21 W11^T, A22 -= S S^T, T = S W11, W22 = L22^-1,
-// W21 = -W22 T, all in LDS (four regions of 64x64).  Writes the lower
-// 32-blocks of W = L^-1 at (k0, k0) (its upper-right 32-blocks are left
-// untouched); A is only read.
-__global__ __launch_bounds__(256) void k_leaf128(const float* __restrict__ A,
-                                                 long long lda,
-                                                 float* __restrict__ W,
-                                                 long long ldw, int k0,
-                                                 int* __restrict__ info) {
-  __shared__ float X11[64 * XS], X21[64 * XS], X22[64 * XS], XT[64 * XS];
-  __shared__ __align__(16) LeafLds s;
+// W21 = -W22 T, in LDS (three 64x64 regions: W11 goes to memory as soon as
+// the products that read it are done, and its region then holds T).
+// Writes the lower 32-blocks of W = L^-1 at (k0, k0); A is only read.  Run
+// by a workgroup of 256 threads.
+__device__ __forceinline__ void leaf128_body(
+    const float* __restrict__ A, long long lda, float* __restrict__ W,
+    long long ldw, int k0, int* __restrict__ info, float* __restrict__ X11,
+    float* __restrict__ X21, float* __restrict__ X22, LeafLds& s) {
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   const float* A0 = A + (long long)k0 * lda + k0;
   block64_in<256>(X11, A0, lda, t);
@@ -656,13 +678,12 @@ __global__ __launch_bounds__(256) void k_leaf128(const float* __restrict__ A,
       X21[(32 * ib + mfma_row(e, lane)) * XS + 32 * jb + (lane & 31)] = d[e];
   }
   __syncthreads();
+  float* W0 = W + (long long)k0 * ldw + k0;
   {
-    // T = S W11; A22 -= S S^T on the three lower tiles
+    // T = S W11; A22 -= S S^T on the three lower tiles; W11 out; T into
+    // the W11 region once every read of it is done
     const floatx16 d = mm32<XS, XS, 64, false, false>(
         X21 + 32 * ib * XS, X11 + 32 * jb, lane);
-#pragma unroll
-    for (int e = 0; e < 16; ++e)
-      XT[(32 * ib + mfma_row(e, lane)) * XS + 32 * jb + (lane & 31)] = d[e];
     if (jb <= ib) {
       const floatx16 g = mm32<XS, XS, 64, false, true>(
           X21 + 32 * ib * XS, X21 + 32 * jb * XS, lane);
@@ -671,6 +692,11 @@ __global__ __launch_bounds__(256) void k_leaf128(const float* __restrict__ A,
         X22[(32 * ib + mfma_row(e, lane)) * XS + 32 * jb + (lane & 31)] -=
             g[e];
     }
+    block64_out<256>(W0, ldw, X11, false, t);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+      X11[(32 * ib + mfma_row(e, lane)) * XS + 32 * jb + (lane & 31)] = d[e];
   }
   __syncthreads();
   const int bad2 = chol_inv64(X22, t, s);
@@ -679,71 +705,247 @@ __global__ __launch_bounds__(256) void k_leaf128(const float* __restrict__ A,
   {
     // W21 = -W22 T into the S slot (S is no longer read)
     const floatx16 d = mm32<XS, XS, 64, false, false>(
-        X22 + 32 * ib * XS, XT + 32 * jb, lane);
+        X22 + 32 * ib * XS, X11 + 32 * jb, lane);
 #pragma unroll
     for (int e = 0; e < 16; ++e)
       X21[(32 * ib + mfma_row(e, lane)) * XS + 32 * jb + (lane & 31)] = -d[e];
   }
   __syncthreads();
-  float* W0 = W + (long long)k0 * ldw + k0;
-  block64_out<256>(W0, ldw, X11, false, t);
   block64_out<256>(W0 + 64 * ldw, ldw, X21, true, t);
   block64_out<256>(W0 + 64 * ldw + 64, ldw, X22, false, t);
 }
 
-// Split of a block of n rows into n1 + n2 (multiples of 64).  At the top
-// level the trailing block n2 is kept at or below 31 tiles of 128 when it
-// would be just above: its SYRK then has 496 lower tiles, one round on 256
-// CUs x 2 resident workgroups, instead of 528 (two rounds).
-static int split_of(int n, bool top) {
-  if (n <= 64) return n;
-  int n1 = 64 * ((n / 64 + 1) / 2);
-  if (top && n - n1 > 31 * 128 && n - n1 <= 32 * 128) n1 = n - 31 * 128;
-  return n1;
+constexpr int kLeafLds = 3 * 64 * XS * 4 + (int)sizeof(LeafLds);
+
+__global__ __launch_bounds__(256) void k_leaf128(const float* __restrict__ A,
+                                                 long long lda,
+                                                 float* __restrict__ W,
+                                                 long long ldw, int k0,
+                                                 int* __restrict__ info) {
+  __shared__ __align__(16) char smem[kLeafLds];
+  float* X = reinterpret_cast<float*>(smem);
+  leaf128_body(A, lda, W, ldw, k0, info, X, X + 64 * XS, X + 128 * XS,
+               *reinterpret_cast<LeafLds*>(X + 192 * XS));
 }
 
-static void chol_node(float* A, long long lda, float* W, long long ldw, int o,
-                      int n, bool top, int* info, hipStream_t st) {
-  if (n <= 64) {
-    hipLaunchKernelGGL(k_leaf64, dim3(1), dim3(64), 0, st, A, lda, W, ldw, o,
-                       info);
-    return;
+// ------------------------------------------------ blocked factorisation
+// Right-looking blocked Cholesky on 128-column blocks (the last one 64 when
+// n is an odd multiple of 64), two levels deep, on two streams.  Outer
+// panels of 512 columns; inside a panel, step k (block k at rows
+// [o, o + b), trailing rows r0 = o + b):
+//   leaf:   W_kk = L_kk^-1 from A_kk (one workgroup);
+//   panel:  L_ik = A_ik W_kk^T for the rows below (the inverse makes the
+//           triangular solve a GEMM), into W's block column k;
+//   inner:  A[r0:n, r0:panel end] -= L_{., k} L_{r0:panel end, k}^T.
+// After a panel (columns [O, E0)), its L updates the rest with K = 512
+// (four times the arithmetic intensity of per-128 updates, whose
+// read-modify-write of the trailing matrix would be memory-bound):
+//   head: the next panel's first 128 columns, on the caller's stream (the
+//         next leaf needs them), after the previous bulk update (which also
+//         writes them);
+//   mid:  the rest of the next panel's columns, on the side stream (the
+//         next panel's first inner update waits for it);
+//   bulk: everything right of the next panel (lower tiles), on the side
+//         stream, overlapping the next panel's factorisation.
+// The chain of leaves, panel products and inner updates is the critical
+// path; the n^3 / 3 flops of the bulk updates fill the GPU beside it.  The
+// updates use 64x64 tiles: a 128x128-tile workgroup fills a CU's registers
+// (one wave per SIMD), and on these K = 512 updates the smaller tiles, two
+// or more workgroups per CU, are faster (7168^2 lower x 512: 94 against 62
+// TFLOP/s on one MI355X).  The factor's inverse is then built by recursive
+// doubling inside each half (chol_invert_half).
+// Events of the side stream: the last panel product, the last mid update,
+// the last two bulk updates, the side stream's end.
+enum { kEvPanel, kEvMid, kEvBulk0, kEvBulk1, kEvSideEnd, kCholEvents };
+struct CholStreams {
+  hipStream_t side = nullptr;
+  hipEvent_t ev[kCholEvents] = {};
+};
+constexpr int kCholDevices = 64;
+static std::mutex g_chol_mu;
+static CholStreams g_chol[kCholDevices];
+
+// The side stream and events of the current device (created on first use,
+// default priority, not synchronising with the null stream), or nullptr
+// after setting the error.
+static CholStreams* chol_streams() {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kCholDevices) {
+    set_error(e != hipSuccess ? (int)e : NTA_EINVAL,
+              "nta_chol_f32: no usable device (%d)", dev);
+    return nullptr;
   }
-  if (n == 128 && !top) {
-    hipLaunchKernelGGL(k_leaf128, dim3(1), dim3(256), 0, st, A, lda, W, ldw,
-                       o, info);
-    return;
+  std::lock_guard<std::mutex> lock(g_chol_mu);
+  CholStreams& c = g_chol[dev];
+  if (!c.side) {
+    hipStream_t s = nullptr;
+    hipEvent_t ev[kCholEvents] = {};
+    e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    for (int k = 0; k < kCholEvents && e == hipSuccess; ++k)
+      e = hipEventCreateWithFlags(&ev[k], hipEventDisableTiming);
+    if (e != hipSuccess) {
+      set_error((int)e, "nta_chol_f32: stream/event setup: %s",
+                hipGetErrorString(e));
+      return nullptr;
+    }
+    for (int k = 0; k < kCholEvents; ++k) c.ev[k] = ev[k];
+    c.side = s;
   }
-  const int n1 = split_of(n, top), n2 = n - n1, r2 = o + n1;
-  chol_node(A, lda, W, ldw, o, n1, false, info, st);
-  float* A21 = A + (long long)r2 * lda + o;
-  float* A22 = A + (long long)r2 * lda + r2;
-  float* W11 = W + (long long)o * ldw + o;
-  float* W21 = W + (long long)r2 * ldw + o;
-  float* W22 = W + (long long)r2 * ldw + r2;
-  // One tile size for the whole node (the SYRK and T products share a
-  // launch).
-  const int tile = tile_for(n2, n1 > n2 ? n1 : n2);
-  // S = A21 W11^T  (W11 lower: B of the NT form is lower)
-  launch_gemm(make_job(A21, lda, W11, ldw, W21, ldw, n2, n1, n1, 1.0f, 0.0f,
-                       0, 1, 0, tile),
-              nullptr, tile, st);
-  // A22 -= S S^T (lower tiles), and T = S W11 into the A21 slot.
-  const GemmJob syrk = make_job(W21, ldw, W21, ldw, A22, lda, n2, n2, n1,
-                                -1.0f, 1.0f, 0, 0, 1, tile);
-  if (top) {
-    launch_gemm(syrk, nullptr, tile, st);
-  } else {
-    const GemmJob tt = make_job(W21, ldw, W11, ldw, A21, lda, n2, n1, n1,
-                                1.0f, 0.0f, 1, 2, 0, tile);
-    launch_gemm(syrk, &tt, tile, st);
+  return &c;
+}
+
+// Block offsets of an n x n factorisation (n a positive multiple of 64).
+static std::vector<int> chol_offsets(int n) {
+  std::vector<int> off;
+  for (int o = 0; o < n; o += 128) off.push_back(o);
+  off.push_back(n);
+  return off;                 // block k is [off[k], off[k + 1])
+}
+
+static int chol_split_of(int n) {
+  const std::vector<int> off = chol_offsets(n);
+  const int m = (int)off.size() - 1;
+  return m == 1 ? n : off[m / 2];
+}
+
+constexpr int kOuter = 512;
+
+static int chol_factor(float* A, long long lda, float* W, long long ldw,
+                       int n, int* info, hipStream_t st, CholStreams& cs) {
+  hipStream_t sd = cs.side;
+  hipEvent_t* ev = cs.ev;
+  hipError_t e = hipSuccess;
+  auto wait = [&](hipStream_t waiter, int id) {
+    if (e == hipSuccess) e = hipStreamWaitEvent(waiter, ev[id], 0);
+  };
+  auto record = [&](int id, hipStream_t recorder) {
+    if (e == hipSuccess) e = hipEventRecord(ev[id], recorder);
+  };
+  // A[r:n, c0:c1] -= L[r:n, kc:kc+K] L[c0:c1, kc:kc+K]^T (lower: only the
+  // tiles on and below the diagonal, for c0 == r, c1 == n)
+  auto update = [&](int r, int c0, int c1, int kc, int K, int lower,
+                    hipStream_t s) {
+    const float* Lr = W + (long long)r * ldw + kc;
+    const float* Lc = W + (long long)c0 * ldw + kc;
+    launch_gemm(make_job(Lr, ldw, Lc, ldw, A + (long long)r * lda + c0, lda,
+                         n - r, c1 - c0, K, -1.0f, 1.0f, 0, 0, lower),
+                nullptr, s);
+  };
+  const std::vector<int> off = chol_offsets(n);
+  const int m = (int)off.size() - 1;
+  int bulk_prev = -1;                 // event of the previous bulk update
+  int nbulk = 0;
+  bool mid_pending = false;           // the panel's first update waits
+  int k = 0;
+  while (k < m && e == hipSuccess) {
+    const int O = off[k];
+    const int ke = std::min(m, k + kOuter / 128);   // inner blocks [k, ke)
+    const int E0 = off[ke];                          // end of the panel
+    for (; k < ke; ++k) {
+      const int o = off[k], b = off[k + 1] - o, r0 = o + b, rem = n - r0;
+      if (b == 128)
+        hipLaunchKernelGGL(k_leaf128, dim3(1), dim3(256), 0, st, A, lda, W,
+                           ldw, o, info);
+      else
+        hipLaunchKernelGGL(k_leaf64, dim3(1), dim3(64), 0, st, A, lda, W,
+                           ldw, o, info);
+      if (rem == 0) break;
+      // panel: W[r0:n, o:o+b] = A[r0:n, o:o+b] W_kk^T (W_kk lower)
+      launch_gemm(make_job(A + (long long)r0 * lda + o, lda,
+                           W + (long long)o * ldw + o, ldw,
+                           W + (long long)r0 * ldw + o, ldw, rem, b, b, 1.0f,
+                           0.0f, 0, 1, 0),
+                  nullptr, st);
+      if (r0 < E0) {
+        // the columns [r0, E0) take the previous panel's mid update first
+        if (mid_pending) wait(st, kEvMid);
+        mid_pending = false;
+        update(r0, r0, E0, o, b, 0, st);
+      }
+    }
+    if (E0 >= n || e != hipSuccess) break;
+    const int E1 = off[std::min(m, ke + kOuter / 128)];   // next panel end
+    const int H1 = std::min(E1, E0 + 128);                // its first block
+    const int Wd = E0 - O;
+    // side stream: mid and bulk updates by this panel's L[., O:E0]
+    record(kEvPanel, st);
+    wait(sd, kEvPanel);
+    if (H1 < E1) {
+      update(H1, H1, E1, O, Wd, 0, sd);
+      record(kEvMid, sd);
+      mid_pending = true;
+    }
+    int bulk_now = -1;
+    if (E1 < n) {
+      update(E1, E1, n, O, Wd, 1, sd);
+      bulk_now = kEvBulk0 + (nbulk++ & 1);
+      record(bulk_now, sd);
+    }
+    // the next panel's first block columns, after the previous bulk update
+    if (bulk_prev >= 0) wait(st, bulk_prev);
+    bulk_prev = bulk_now;
+    update(E0, E0, H1, O, Wd, 0, st);
   }
-  chol_node(A, lda, W, ldw, r2, n2, false, info, st);
-  if (!top) {
-    // W21 = -W22 T
-    launch_gemm(make_job(W22, ldw, A21, lda, W21, ldw, n2, n1, n2, -1.0f,
-                         0.0f, 1, 3, 0, tile),
-                nullptr, tile, st);
+  // every side-stream update is complete before the stream goes on
+  record(kEvSideEnd, sd);
+  wait(st, kEvSideEnd);
+  if (e != hipSuccess)
+    return set_error((int)e, "nta_chol_f32: %s", hipGetErrorString(e));
+  return 0;
+}
+
+// Inverse of the factor inside blocks [kb, ke) (rows/columns off[kb] ..
+// off[ke]), from the block inverses on W's diagonal and the L blocks below
+// them, by recursive doubling: groups of s blocks are joined pairwise,
+//   [W_l 0; L_rl W_r] -> [W_l 0; -W_r (L_rl W_l) W_r],
+// the product T = L_rl W_l into scratch rows of `tmp` (leading dimension
+// ldt), then -W_r T over L_rl in W.  All joins of one size in a level run
+// as one batched launch (the one shorter join at the end, if any, as the
+// second job of that launch).
+static void chol_invert_half(float* W, long long ldw,
+                             const std::vector<int>& off, int kb, int ke,
+                             float* tmp, long long ldt, hipStream_t st,
+                             int smax) {
+  const int nb = ke - kb;
+  for (int s = 1; s < nb && s < smax; s *= 2) {
+    // joins: left blocks [kb + 2 g s, + s), right [kb + (2 g + 1) s, + s)
+    int full = 0;
+    while (kb + (2 * full + 2) * s <= ke &&
+           off[kb + (2 * full + 2) * s] - off[kb + (2 * full + 1) * s] ==
+               s * 128 &&
+           off[kb + (2 * full + 1) * s] - off[kb + 2 * full * s] == s * 128)
+      ++full;
+    const int g1 = full;                          // a shorter last join?
+    const bool extra = kb + (2 * g1 + 1) * s < ke;
+    const long long stW = (long long)(2 * s * 128) * (ldw + 1);
+    GemmJob tj[2], wj[2];
+    int nj = 0;
+    // T of join g in scratch rows [2 g s 128, + rs): the batch members
+    // and the shorter last join never overlap.
+    auto jobs_for = [&](int g, int cnt, int ls, int rs) {
+      const int lo = off[kb + 2 * g * s], ro = lo + ls;
+      float* Tg = tmp + (long long)(2 * g * s * 128) * ldt;
+      GemmJob t = make_job(W + (long long)ro * ldw + lo, ldw,
+                           W + (long long)lo * ldw + lo, ldw, Tg, ldt, rs, ls,
+                           ls, 1.0f, 0.0f, 1, 2, 0);
+      GemmJob w = make_job(W + (long long)ro * ldw + ro, ldw, Tg, ldt,
+                           W + (long long)ro * ldw + lo, ldw, rs, ls, rs,
+                           -1.0f, 0.0f, 1, 3, 0);
+      tj[nj] = batched(t, cnt, stW, stW, (long long)(2 * s * 128) * ldt);
+      wj[nj] = batched(w, cnt, stW, (long long)(2 * s * 128) * ldt, stW);
+      ++nj;
+    };
+    if (full > 0) jobs_for(0, full, s * 128, s * 128);
+    if (extra) {
+      const int lo = off[kb + 2 * g1 * s], ro = off[kb + (2 * g1 + 1) * s];
+      const int re = off[kb + (2 * g1 + 2) * s <= ke ? kb + (2 * g1 + 2) * s
+                                                     : ke];
+      jobs_for(g1, 1, ro - lo, re - ro);
+    }
+    if (nj == 0) break;
+    launch_gemm(tj[0], nj > 1 ? &tj[1] : nullptr, st);
+    launch_gemm(wj[0], nj > 1 ? &wj[1] : nullptr, st);
   }
 }
 
@@ -882,6 +1084,8 @@ __device__ __forceinline__ void row_stripe_product(
 // fp64 -- or in fp64); the four partial tiles of a row stripe are added
 // through LDS in a fixed order.
 // mode 0: Q = (K + lam I) P, part[wg][c] = sum_rows P Q.
+// mode 2: Q = K P for a rectangular K (rows = 32 gridDim.x, n columns):
+//         neither P's rows of Q's index nor part are touched.
 // mode 1: D = B - (K + lam I) P; R = D, R32 = D (when R is given),
 //         part[wg][c] = sum_rows D^2.
 template <typename TK>
@@ -889,7 +1093,9 @@ __global__ __launch_bounds__(512) void k_symv_rows(
     const TK* __restrict__ K, long long ldk, int n, double lam, int mode,
     const double* __restrict__ P, double* __restrict__ Q,
     const double* __restrict__ B, double* __restrict__ R,
-    float* __restrict__ R32, double* __restrict__ part) {
+    float* __restrict__ R32, double* __restrict__ part,
+    const int* __restrict__ done) {
+  if (done && *done) return;           // converged: the rest is a no-op
   __shared__ doublex4 red[4][2][64];
   __shared__ double dots[2][RHS];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -911,8 +1117,12 @@ __global__ __launch_bounds__(512) void k_symv_rows(
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const long long e = (long long)(i0 + 4 * v + g) * RHS + r;
-      const double pv = P[e];
       const double kp = ((acc[v] + a1[v]) + a2[v]) + a3[v];
+      if (mode == 2) {
+        Q[e] = kp;
+        continue;
+      }
+      const double pv = P[e];
       const double qv = fma(lam, pv, kp);
       if (mode == 0) {
         Q[e] = qv;
@@ -931,20 +1141,24 @@ __global__ __launch_bounds__(512) void k_symv_rows(
     if (g == 0) dots[s][r] = d;
   }
   __syncthreads();
-  if (t < RHS) part[(long long)blockIdx.x * RHS + t] = dots[0][t] + dots[1][t];
+  if (part && t < RHS)
+    part[(long long)blockIdx.x * RHS + t] = dots[0][t] + dots[1][t];
 }
 
 // Out = sa (A X) + sv V (fp32) for a row-major block A (m x k) on
 // v_mfma_f32_16x16x4f32: a workgroup of 8 waves owns 16 output rows, the
 // waves take the 16-column chunks round-robin and are added through LDS in
 // a fixed order.  tri: A is lower (1) or upper (2) triangular.  With r64
-// given, rzp[wg][c] = sum_rows r64 * Out (fp64).
+// given, rzp[wg][c] = sum_rows r64 * Out and zzp[wg][c] = sum_rows Out^2
+// (fp64).
 constexpr int GV_WAVES = 8;
 __global__ __launch_bounds__(64 * GV_WAVES) void k_gemv_rows(
     const float* __restrict__ A, long long lda, int m, int k, int tri,
     const float* __restrict__ X, float* __restrict__ Out, float sa,
     const float* __restrict__ V, float sv, const double* __restrict__ r64,
-    double* __restrict__ rzp) {
+    double* __restrict__ rzp, double* __restrict__ zzp,
+    const int* __restrict__ done) {
+  if (done && *done) return;
   __shared__ floatx4 red[GV_WAVES][64];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int r = lane & 15, g = lane >> 4;
@@ -958,7 +1172,7 @@ __global__ __launch_bounds__(64 * GV_WAVES) void k_gemv_rows(
   red[w][lane] = acc;
   __syncthreads();
   if (w != 0) return;
-  double d = 0.0;
+  double d = 0.0, zz = 0.0;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     float o = acc[v];
@@ -968,118 +1182,200 @@ __global__ __launch_bounds__(64 * GV_WAVES) void k_gemv_rows(
     o *= sa;
     if (V) o = fmaf(sv, V[e], o);
     Out[e] = o;
-    if (r64) d += r64[e] * (double)o;
+    if (r64) {
+      d += r64[e] * (double)o;
+      zz += (double)o * (double)o;
+    }
   }
   if (!r64) return;
   d += __shfl_xor(d, 16);
   d += __shfl_xor(d, 32);
-  if (g == 0) rzp[(long long)blockIdx.x * RHS + r] = d;
+  zz += __shfl_xor(zz, 16);
+  zz += __shfl_xor(zz, 32);
+  if (g == 0) {
+    rzp[(long long)blockIdx.x * RHS + r] = d;
+    if (zzp) zzp[(long long)blockIdx.x * RHS + r] = zz;
+  }
 }
 
+// Iteration state of the refinement, in device memory, written only by
+// the single-block kernels below (the vector kernels read it).
+struct PcgState {
+  double alpha[RHS];   // step of the current iteration
+  double beta[RHS];    // direction update of the next iteration
+  double rz[RHS];      // r . z of the current residual
+  double bn[RHS];      // ||b|| per column
+  double rel[RHS];     // ||r|| / ||b|| (recurrence or recomputed residual)
+  double est[RHS];     // ||M^-1 r|| / ||x||: estimated relative error of x
+  int done;            // 1: converged, every later kernel is a no-op
+  int iters;           // iteration that set done (or the last checked)
+  int restart;         // 1: the next direction is z (beta = 0)
+  int pad;
+};
+
 // Per-column sums of nslot partials [slot][RHS] by a 256-thread block:
-// thread t adds slots t/16, t/16 + 16, ... of column t%16, then a fixed
-// order over the 16 strands.  Result in out[RHS] (shared); ends with a
-// barrier.
+// thread t adds slots t/16, t/16 + 16, ... of column t%16 (the loop
+// unrolled by four so the loads are in flight together), then a fixed order
+// over the 16 strands.  Result in out[RHS] (shared); ends with a barrier.
 __device__ __forceinline__ void block_slot_sums(const double* __restrict__ s,
                                                 int nslot,
                                                 double* __restrict__ scratch,
                                                 double* __restrict__ out) {
   const int t = threadIdx.x;
   const int c = t % RHS, strand = t / RHS;
-  double v = 0.0;
-  for (int k = strand; k < nslot; k += 256 / RHS) v += s[k * RHS + c];
-  scratch[t] = v;
+  constexpr int ST = 256 / RHS;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  int k = strand;
+  for (; k + 3 * ST < nslot; k += 4 * ST) {
+    v0 += s[(k) * RHS + c];
+    v1 += s[(k + ST) * RHS + c];
+    v2 += s[(k + 2 * ST) * RHS + c];
+    v3 += s[(k + 3 * ST) * RHS + c];
+  }
+  for (; k < nslot; k += ST) v0 += s[k * RHS + c];
+  scratch[t] = (v0 + v1) + (v2 + v3);
   __syncthreads();
   if (t < RHS) {
     double a = 0.0;
-    for (int q = 0; q < 256 / RHS; ++q) a += scratch[q * RHS + t];
+    for (int q = 0; q < ST; ++q) a += scratch[q * RHS + t];
     out[t] = a;
   }
   __syncthreads();
 }
 
-// out[c] = sum over nslot partials [slot][RHS] of column c (one block of
-// 256 threads, fixed order), so the host reads RHS values per test.
-__global__ __launch_bounds__(256) void k_colsum(const double* __restrict__ s,
-                                                int nslot,
-                                                double* __restrict__ out) {
-  __shared__ double scratch[256], res[RHS];
-  block_slot_sums(s, nslot, scratch, res);
-  if (threadIdx.x < RHS) out[threadIdx.x] = res[threadIdx.x];
+// Start of the iteration: ||b|| from the bb partials, r . z of the first
+// preconditioned residual; the first direction is z.
+__global__ __launch_bounds__(256) void k_pcg_init(
+    const double* __restrict__ bb, int nbb, const double* __restrict__ rz,
+    int nrz, PcgState* __restrict__ st) {
+  __shared__ double scratch[256], sb[RHS], sr[RHS];
+  block_slot_sums(bb, nbb, scratch, sb);
+  block_slot_sums(rz, nrz, scratch, sr);
+  const int t = threadIdx.x;
+  if (t < RHS) {
+    st->bn[t] = sqrt(sb[t]);
+    st->rz[t] = sr[t];
+    st->alpha[t] = 0.0;
+    st->beta[t] = 0.0;
+    st->rel[t] = 1.0;
+    st->est[t] = INFINITY;
+  }
+  if (t == 0) {
+    st->done = 0;
+    st->iters = 0;
+    st->restart = 1;
+  }
 }
 
-// P_new = Z + beta P_old with the Polak-Ribiere (flexible CG) coefficient
-// beta = (r+ . z+  -  r+ . z) / (r . z) per column, which keeps the
-// iteration robust to a preconditioner that is only approximately
-// symmetric (fp32 products); 0 on the first step or when r . z = 0.
+// P_new = Z + beta P_old (beta = 0 after a restart).
 __global__ __launch_bounds__(256) void k_pcg_p(
     int n, const float* __restrict__ Z, const double* __restrict__ P_old,
-    double* __restrict__ P_new, const double* __restrict__ rz_new,
-    const double* __restrict__ rz_old, int nslot,
-    const double* __restrict__ rz_cross, int ncross, int first) {
-  __shared__ double beta[RHS], num[RHS], den[RHS], crs[RHS], scratch[256];
+    double* __restrict__ P_new, const PcgState* __restrict__ st) {
+  if (st->done) return;
   const int t = threadIdx.x;
-  if (!first) {
-    block_slot_sums(rz_new, nslot, scratch, num);
-    block_slot_sums(rz_old, nslot, scratch, den);
-    block_slot_sums(rz_cross, ncross, scratch, crs);
-  }
-  if (t < RHS)
-    beta[t] = (first || den[t] == 0.0) ? 0.0 : (num[t] - crs[t]) / den[t];
-  __syncthreads();
+  const bool first = st->restart != 0;
+  const double beta = first ? 0.0 : st->beta[t % RHS];
   const long long e0 = (long long)blockIdx.x * VROWS * RHS;
   for (int e = t; e < VROWS * RHS; e += 256) {
     const long long g = e0 + e;
     if (g >= (long long)n * RHS) break;
-    P_new[g] = first ? (double)Z[g] : fma(beta[e % RHS], P_old[g],
-                                          (double)Z[g]);
+    P_new[g] = first ? (double)Z[g] : fma(beta, P_old[g], (double)Z[g]);
   }
 }
 
-// alpha = rz / pq per column; X += alpha P; R -= alpha Q; R32 = R;
-// rr[wg][c] = sum_rows R^2, rzx[wg][c] = sum_rows R . Z (Z: the current
-// preconditioned residual, for the Polak-Ribiere beta).
+// alpha = (r . z) / (p . A p) per column from the product's partials.
+__global__ __launch_bounds__(256) void k_pcg_alpha(
+    const double* __restrict__ pq, int npq, PcgState* __restrict__ st) {
+  if (st->done) return;
+  __shared__ double scratch[256], den[RHS];
+  block_slot_sums(pq, npq, scratch, den);
+  const int t = threadIdx.x;
+  if (t < RHS) st->alpha[t] = den[t] != 0.0 ? st->rz[t] / den[t] : 0.0;
+}
+
+// X += alpha P; R -= alpha Q; R32 = R; partials per workgroup of
+// rr = sum_rows R^2, rzx = sum_rows R . Z (Z: the previous preconditioned
+// residual, for the Polak-Ribiere beta) and xx = sum_rows X^2.
 __global__ __launch_bounds__(256) void k_pcg_xr(
-    int n, const double* __restrict__ pq, int npq,
-    const double* __restrict__ rz, int nrz, const double* __restrict__ P,
+    int n, const PcgState* __restrict__ st, const double* __restrict__ P,
     const double* __restrict__ Q, const float* __restrict__ Z,
     double* __restrict__ X, double* __restrict__ R, float* __restrict__ R32,
-    double* __restrict__ rr, double* __restrict__ rzx) {
-  __shared__ double alpha[RHS], num[RHS], den[RHS];
+    double* __restrict__ rr, double* __restrict__ rzx,
+    double* __restrict__ xx) {
+  if (st->done) return;
   __shared__ double red[256];
   const int t = threadIdx.x;
-  block_slot_sums(pq, npq, red, den);
-  block_slot_sums(rz, nrz, red, num);
-  if (t < RHS) alpha[t] = den[t] != 0.0 ? num[t] / den[t] : 0.0;
-  __syncthreads();
   const int c = t % RHS;
+  const double alpha = st->alpha[c];
   const long long e0 = (long long)blockIdx.x * VROWS * RHS;
-  double s = 0.0, x = 0.0;
+  double s = 0.0, x = 0.0, q = 0.0;
   for (int e = t; e < VROWS * RHS; e += 256) {
     const long long g = e0 + e;
     if (g >= (long long)n * RHS) break;
-    X[g] = fma(alpha[c], P[g], X[g]);
-    const double rv = fma(-alpha[c], Q[g], R[g]);
+    const double xv = fma(alpha, P[g], X[g]);
+    X[g] = xv;
+    const double rv = fma(-alpha, Q[g], R[g]);
     R[g] = rv;
     R32[g] = (float)rv;
     s += rv * rv;
     x += rv * (double)Z[g];
+    q += xv * xv;
+  }
+  double* outs[3] = {rr, rzx, xx};
+  const double vals[3] = {s, x, q};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    __syncthreads();
+    red[t] = vals[k];
+    __syncthreads();
+    if (t < RHS) {
+      double v = 0.0;
+      for (int m = t; m < 256; m += RHS) v += red[m];
+      outs[k][(long long)blockIdx.x * RHS + t] = v;
+    }
+  }
+}
+
+// Convergence test and the next direction's beta, after the
+// preconditioner has produced z+ = M^-1 r+ (rz partials r+ . z+, zz
+// partials z+ . z+).  rr: partials of r+ . r+ (nrr slots); rzx: of
+// r+ . z (nullptr after a recomputed residual, which restarts the
+// direction); xx: of x . x.  Every column c < nrhs with b != 0 must have
+// rel <= tol or est <= tol.
+__global__ __launch_bounds__(256) void k_pcg_check(
+    const double* __restrict__ rr, int nrr, const double* __restrict__ rzx,
+    int nrzx, const double* __restrict__ xx, int nxx,
+    const double* __restrict__ rz, const double* __restrict__ zz, int nrz,
+    int nrhs, double tol, int it, PcgState* __restrict__ st) {
+  if (rzx && st->done) return;
+  __shared__ double scratch[256], srr[RHS], srx[RHS], sxx[RHS], srz[RHS],
+      szz[RHS];
+  __shared__ int ok[RHS];
+  block_slot_sums(rr, nrr, scratch, srr);
+  if (rzx) block_slot_sums(rzx, nrzx, scratch, srx);
+  block_slot_sums(xx, nxx, scratch, sxx);
+  block_slot_sums(rz, nrz, scratch, srz);
+  block_slot_sums(zz, nrz, scratch, szz);
+  const int t = threadIdx.x;
+  if (t < RHS) {
+    const double bn = st->bn[t];
+    const double rel = bn > 0.0 ? sqrt(srr[t]) / bn : 0.0;
+    const double est = sxx[t] > 0.0 ? sqrt(szz[t] / sxx[t])
+                                    : (bn > 0.0 ? INFINITY : 0.0);
+    st->rel[t] = rel;
+    st->est[t] = est;
+    ok[t] = (t >= nrhs || bn == 0.0 || rel <= tol || est <= tol) ? 1 : 0;
+    const double rz_old = st->rz[t];
+    st->beta[t] = (rzx && rz_old != 0.0) ? (srz[t] - srx[t]) / rz_old : 0.0;
+    st->rz[t] = srz[t];
   }
   __syncthreads();
-  red[t] = s;
-  __syncthreads();
-  if (t < RHS) {
-    double v = 0.0;
-    for (int k = t; k < 256; k += RHS) v += red[k];
-    rr[(long long)blockIdx.x * RHS + t] = v;
-  }
-  __syncthreads();
-  red[t] = x;
-  __syncthreads();
-  if (t < RHS) {
-    double v = 0.0;
-    for (int k = t; k < 256; k += RHS) v += red[k];
-    rzx[(long long)blockIdx.x * RHS + t] = v;
+  if (t == 0) {
+    int all = 1;
+    for (int c = 0; c < RHS; ++c) all &= ok[c];
+    st->done = all;
+    st->iters = it;
+    st->restart = rzx ? 0 : 1;
   }
 }
 
@@ -1138,7 +1434,8 @@ __global__ __launch_bounds__(256) void k_pad32(int n, const float* __restrict__ 
 
 // ------------------------------------------------------------ host side
 struct Ws {
-  double *b, *x, *r, *p0, *p1, *q, *pq, *rz0, *rz1, *rr, *bb, *rzx, *sums;
+  double *b, *x, *r, *p0, *p1, *q, *pq, *rz, *zz, *rr, *bb, *rzx, *xx;
+  PcgState* st;
   float *r32, *z32, *y1, *u2, *y2, *v1;
   size_t bytes;
 };
@@ -1161,12 +1458,13 @@ static Ws carve(char* base, int n) {
   w.p1 = (double*)take(v64);
   w.q = (double*)take(v64);
   w.pq = (double*)take(slots);
-  w.rz0 = (double*)take(slots);
-  w.rz1 = (double*)take(slots);
+  w.rz = (double*)take(slots);
+  w.zz = (double*)take(slots);
   w.rr = (double*)take(slots);
   w.bb = (double*)take(slots);
   w.rzx = (double*)take(slots);
-  w.sums = (double*)take(RHS * sizeof(double));
+  w.xx = (double*)take(slots);
+  w.st = (PcgState*)take(sizeof(PcgState));
   w.r32 = (float*)take(v32);
   w.z32 = (float*)take(v32);
   w.y1 = (float*)take(v32);
@@ -1179,43 +1477,61 @@ static Ws carve(char* base, int n) {
 
 static void gemv(const float* A, long long lda, int m, int k, int tri,
                  const float* X, float* out, float sa, const float* V,
-                 float sv, const double* r64, double* rzp, hipStream_t st) {
+                 float sv, const double* r64, double* rzp, double* zzp,
+                 const int* done, hipStream_t st) {
   hipLaunchKernelGGL(k_gemv_rows, dim3(m / 16), dim3(64 * GV_WAVES), 0, st,
-                     A, lda, m, k, tri, X, out, sa, V, sv, r64, rzp);
+                     A, lda, m, k, tri, X, out, sa, V, sv, r64, rzp, zzp,
+                     done);
 }
 
-// z = (L L^T)^-1 r (fp32) from W (W11, L21, W22) and WT = W^T; rz partials
-// (one slot per 16 rows, n / 16 slots) when r64 is given.
+// z = (L L^T)^-1 r (fp32) from W (W11, L21, W22) and WT = W^T; with r64
+// given, rz and zz partials (r . z and z . z, one slot per 16 rows, n / 16
+// slots).  done: the refinement's convergence flag (nullptr: always run).
 static void precond(const float* W, const float* WT, long long ld, int n,
                     int n1, const float* r32, float* z32, const double* r64,
-                    double* rzp, const Ws& w, hipStream_t st) {
+                    double* rzp, double* zzp, const int* done, const Ws& w,
+                    hipStream_t st) {
   const int n2 = n - n1;
   const long long o2 = (long long)n1 * RHS;
   if (n2 == 0) {
     gemv(W, ld, n, n, 1, r32, w.y1, 1.0f, nullptr, 0.0f, nullptr, nullptr,
-         st);
-    gemv(WT, ld, n, n, 2, w.y1, z32, 1.0f, nullptr, 0.0f, r64, rzp, st);
+         nullptr, done, st);
+    gemv(WT, ld, n, n, 2, w.y1, z32, 1.0f, nullptr, 0.0f, r64, rzp, zzp,
+         done, st);
     return;
   }
   const float* S = W + (long long)n1 * ld;            // L21 (n2 x n1)
   const float* W22 = S + n1;
   const float* ST = WT + n1;                           // L21^T (n1 x n2)
   const float* W22T = WT + (long long)n1 * ld + n1;
+  const long long s2 = (long long)(n1 / 16) * RHS;     // slots of rows n1..
   gemv(W, ld, n1, n1, 1, r32, w.y1, 1.0f, nullptr, 0.0f, nullptr, nullptr,
-       st);                                            // y1 = W11 r1
+       nullptr, done, st);                             // y1 = W11 r1
   gemv(S, ld, n2, n1, 0, w.y1, w.u2, -1.0f, r32 + o2, 1.0f, nullptr,
-       nullptr, st);                                   // u2 = r2 - S y1
+       nullptr, nullptr, done, st);                    // u2 = r2 - S y1
   gemv(W22, ld, n2, n2, 1, w.u2, w.y2, 1.0f, nullptr, 0.0f, nullptr,
-       nullptr, st);                                   // y2 = W22 u2
+       nullptr, nullptr, done, st);                    // y2 = W22 u2
   gemv(W22T, ld, n2, n2, 2, w.y2, z32 + o2, 1.0f, nullptr, 0.0f,
-       r64 ? r64 + o2 : nullptr, rzp ? rzp + (long long)(n1 / 16) * RHS
-                                     : nullptr,
-       st);                                            // z2 = W22^T y2
+       r64 ? r64 + o2 : nullptr, rzp ? rzp + s2 : nullptr,
+       zzp ? zzp + s2 : nullptr, done, st);            // z2 = W22^T y2
   gemv(ST, ld, n1, n2, 0, z32 + o2, w.v1, -1.0f, w.y1, 1.0f, nullptr,
-       nullptr, st);                                   // v1 = y1 - S^T z2
-  gemv(WT, ld, n1, n1, 2, w.v1, z32, 1.0f, nullptr, 0.0f, r64, rzp,
-       st);                                            // z1 = W11^T v1
+       nullptr, nullptr, done, st);                    // v1 = y1 - S^T z2
+  gemv(WT, ld, n1, n1, 2, w.v1, z32, 1.0f, nullptr, 0.0f, r64, rzp, zzp,
+       done, st);                                      // z1 = W11^T v1
 }
+
+// The refinement loop.  Iterations are launched in batches without host
+// synchronisation: every kernel of an iteration returns at once when the
+// device-side state says the solve has converged, so a batch that passes
+// convergence costs a few empty launches.  After each batch the host reads
+// the state (one synchronisation); on convergence, and every second batch,
+// the residual is recomputed as b - A x (the recurrence drifts on
+// ill-conditioned systems) and tested again, and the direction restarts.
+// Stop: every column's relative residual ||r|| / ||b|| or estimated
+// relative error ||M^-1 r|| / ||x|| <= tol (M^-1 r is the error x* - x to
+// the accuracy of the FP32 factor), or no progress of the recomputed
+// residual's test quantity over 5 checkpoints.
+constexpr int kPcgBatch = 4;
 
 template <typename TK>
 static int pcg_run(const TK* K, int n, long long ldk, double lam,
@@ -1226,84 +1542,102 @@ static int pcg_run(const TK* K, int n, long long ldk, double lam,
   const Ws w = carve(work, n);
   const int nvec = (n + VROWS - 1) / VROWS;    // vector-kernel workgroups
   const int nsym = n / 32;                     // k_symv_rows workgroups
-  const int nrz = n / 16;                      // rz slots
-  // Column sums of per-workgroup partials, reduced on the device; the host
-  // reads RHS doubles per convergence test.
-  auto col_sums = [&](const double* dev, int nslot, double* out) -> int {
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, st, dev, nslot,
-                       w.sums);
-    hipError_t e = hipMemcpyAsync(out, w.sums, RHS * sizeof(double),
+  const int nrz = n / 16;                      // rz / zz slots
+  const int* done = &w.st->done;
+  PcgState hs;
+  auto read_state = [&]() -> int {
+    hipError_t e = hipMemcpyAsync(&hs, w.st, sizeof(PcgState),
                                   hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess)
       return set_error((int)e, "nta_pcg64: %s", hipGetErrorString(e));
     return 0;
   };
+  auto worst = [&](const double* v) {
+    double m = 0.0;
+    for (int c = 0; c < nrhs; ++c)
+      if (hs.bn[c] > 0.0) m = fmax(m, v[c]);
+    return m;
+  };
+  // r <- b - A x (FP64), z <- M^-1 r, then the test on the recomputed
+  // residual (restarts the direction).
+  auto recompute = [&](int it) {
+    hipLaunchKernelGGL(k_symv_rows<TK>, dim3(nsym), dim3(512), 0, st, K,
+                       ldk, n, lam, 1, w.x, nullptr, w.b, w.r, w.r32, w.rr,
+                       nullptr);
+    precond(W, WT, ldw, n, n1, w.r32, w.z32, w.r, w.rz, w.zz, nullptr, w,
+            st);
+    hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(256), 0, st, w.rr, nsym,
+                       nullptr, 0, w.xx, nvec, w.rz, w.zz, nrz, nrhs, tol,
+                       it, w.st);
+  };
   hipLaunchKernelGGL(k_pcg_io, dim3(nvec), dim3(256), 0, st, n, nrhs, Bin,
                      w.b, w.r, w.r32, w.x, w.p0, w.bb, nullptr);
-  double bn[RHS], rn[RHS];
-  int rc = col_sums(w.bb, nvec, bn);
-  if (rc) return rc;
-  for (int c = 0; c < RHS; ++c) bn[c] = sqrt(bn[c]);
-  auto rel_of = [&](const double* ss) {
-    double v = 0.0;
-    for (int c = 0; c < nrhs; ++c)
-      if (bn[c] > 0.0) v = fmax(v, sqrt(ss[c]) / bn[c]);
-    return v;
-  };
-  // r <- b - A x (FP64), replacing the recurrence residual when R is given.
-  auto true_residual = [&](double* R, float* R32) -> int {
-    hipLaunchKernelGGL(k_symv_rows<TK>, dim3(nsym), dim3(512), 0, st, K,
-                       ldk, n, lam, 1, w.x, nullptr, w.b, R, R32, w.rr);
-    return col_sums(w.rr, nsym, rn);
-  };
-  precond(W, WT, ldw, n, n1, w.r32, w.z32, w.r, w.rz0, w, st);
+  precond(W, WT, ldw, n, n1, w.r32, w.z32, w.r, w.rz, w.zz, nullptr, w, st);
+  hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(256), 0, st, w.bb, nvec,
+                     w.rz, nrz, w.st);
   double* pbuf[2] = {w.p0, w.p1};
-  double* rzs[2] = {w.rz0, w.rz1};
   double best = INFINITY;
-  int stall = 0, it = 0;
-  for (it = 1; it <= max_iter; ++it) {
-    const double* p_old = pbuf[(it + 1) & 1];
-    double* p_new = pbuf[it & 1];
-    const double* rz_new = rzs[(it + 1) & 1];
-    hipLaunchKernelGGL(k_pcg_p, dim3(nvec), dim3(256), 0, st, n, w.z32,
-                       p_old, p_new, rz_new, rzs[it & 1], nrz, w.rzx, nvec,
-                       it == 1);
-    hipLaunchKernelGGL(k_symv_rows<TK>, dim3(nsym), dim3(512), 0, st, K,
-                       ldk, n, lam, 0, p_new, w.q, nullptr, nullptr, nullptr,
-                       w.pq);
-    hipLaunchKernelGGL(k_pcg_xr, dim3(nvec), dim3(256), 0, st, n, w.pq, nsym,
-                       rz_new, nrz, p_new, w.q, w.z32, w.x, w.r, w.r32, w.rr,
-                       w.rzx);
-    rc = col_sums(w.rr, nvec, rn);
-    if (rc) return rc;
-    double rel = rel_of(rn);
-    // The recurrence drifts from b - A x on ill-conditioned systems: every
-    // 10 iterations, and before accepting convergence, r is recomputed.
-    // CG residual norms are not monotone, so stagnation is judged on these
-    // checkpoints only: 5 in a row without a new best stop the iteration.
-    if (!(rel > tol) || it % 10 == 0) {
-      rc = true_residual(w.r, w.r32);
-      if (rc) return rc;
-      rel = rel_of(rn);
-      if (!(rel > tol)) break;
-      if (rel < best) {
-        best = rel;
-        stall = 0;
-      } else if (++stall >= 5) {
-        break;
-      }
+  int stall = 0, it = 0, batches = 0, rc = 0;
+  bool converged = false;
+  while (it < max_iter && !converged) {
+    const int nb = max_iter - it < kPcgBatch ? max_iter - it : kPcgBatch;
+    for (int k = 0; k < nb; ++k) {
+      ++it;
+      const double* p_old = pbuf[(it + 1) & 1];
+      double* p_new = pbuf[it & 1];
+      hipLaunchKernelGGL(k_pcg_p, dim3(nvec), dim3(256), 0, st, n, w.z32,
+                         p_old, p_new, w.st);
+      hipLaunchKernelGGL(k_symv_rows<TK>, dim3(nsym), dim3(512), 0, st, K,
+                         ldk, n, lam, 0, p_new, w.q, nullptr, nullptr,
+                         nullptr, w.pq, done);
+      hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(256), 0, st, w.pq, nsym,
+                         w.st);
+      hipLaunchKernelGGL(k_pcg_xr, dim3(nvec), dim3(256), 0, st, n, w.st,
+                         p_new, w.q, w.z32, w.x, w.r, w.r32, w.rr, w.rzx,
+                         w.xx);
+      precond(W, WT, ldw, n, n1, w.r32, w.z32, w.r, w.rz, w.zz, done, w,
+              st);
+      hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(256), 0, st, w.rr, nvec,
+                         w.rzx, nvec, w.xx, nvec, w.rz, w.zz, nrz, nrhs, tol,
+                         it, w.st);
     }
-    precond(W, WT, ldw, n, n1, w.r32, w.z32, w.r, rzs[it & 1], w, st);
+    ++batches;
+    rc = read_state();
+    if (rc) return rc;
+    if (!hs.done && batches % 2 != 0) continue;
+    // Converged on the recurrence, or a periodic checkpoint: test the
+    // recomputed residual (the iteration count is the one that converged).
+    const int at = hs.done ? hs.iters : it;
+    recompute(at);
+    rc = read_state();
+    if (rc) return rc;
+    if (hs.done) {
+      it = at;
+      converged = true;
+      break;
+    }
+    const double q = fmin(worst(hs.rel), worst(hs.est));
+    if (q < best) {
+      best = q;
+      stall = 0;
+    } else if (++stall >= 5) {
+      break;
+    }
+    it = at;
   }
-  if (it > max_iter) it = max_iter;
-  rc = true_residual(nullptr, nullptr);
-  if (rc) return rc;
+  if (!converged) {
+    // The returned x's recomputed residual (the test above may not have
+    // run for the last iterations).
+    recompute(it);
+    rc = read_state();
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(k_pcg_io, dim3(nvec), dim3(256), 0, st, n, nrhs,
                      nullptr, nullptr, nullptr, nullptr, w.x, nullptr,
                      nullptr, Xout);
-  if (iters) *iters = it;
-  if (resid) *resid = rel_of(rn);
+  if (iters) *iters = it < max_iter ? it : max_iter;
+  if (resid) *resid = worst(hs.rel);
   return check_launch("nta_pcg64");
 }
 
@@ -1313,7 +1647,7 @@ using namespace nta;
 
 extern "C" int nta_chol_split(int n) {
   if (n <= 0 || n % 64 != 0) return 0;
-  return n <= 64 ? n : split_of(n, true);
+  return chol_split_of(n);
 }
 
 extern "C" int nta_chol_f32(float* A, int n, long long lda, float* W,
@@ -1326,11 +1660,47 @@ extern "C" int nta_chol_f32(float* A, int n, long long lda, float* W,
     return set_error(NTA_EUNSUPPORTED, "nta_chol_f32: n=%d, lda and ldw must "
                      "be multiples of 64, 4 and 4 (pad the matrix)", n);
   hipStream_t st = (hipStream_t)stream;
-  chol_node(A, lda, W, ldw, 0, n, true, info, st);
+  CholStreams* cs = chol_streams();
+  if (!cs) return NTA_EINVAL;
+  int rc = chol_factor(A, lda, W, ldw, n, info, st, *cs);
+  if (rc) return rc;
+  const std::vector<int> off = chol_offsets(n);
+  const int m = (int)off.size() - 1;
+  if (m > 1) {
+    // W11 = L11^-1, W22 = L22^-1; the block below them keeps L21.  The
+    // products' scratch is A (no longer read once factored).  When the
+    // halves are equal powers of two of 128-blocks, each level's joins of
+    // both halves run as one batch (no join of size < mh crosses the
+    // boundary); otherwise half by half.
+    const int mh = m / 2;
+    const bool pow2 = (mh & (mh - 1)) == 0;
+    if (m == 2 * mh && n % 128 == 0 && pow2) {
+      chol_invert_half(W, ldw, off, 0, m, A, lda, st, mh);
+    } else {
+      chol_invert_half(W, ldw, off, 0, mh, A, lda, st, m);
+      chol_invert_half(W, ldw, off, mh, m, A, lda, st, m);
+    }
+  }
   dim3 grid((n + 31) / 32, (n + 31) / 32);
   hipLaunchKernelGGL(k_transpose_lower, grid, dim3(256), 0, st, W, ldw, n,
                      WT, ldw);
   return check_launch("nta_chol_f32");
+}
+
+extern "C" int nta_gemm_f32_f64(const float* K, long long ldk, int m, int n,
+                                const double* X, double* Y, void* stream) {
+  if (m <= 0 || n <= 0 || ldk < n || !K || !X || !Y)
+    return set_error(NTA_EINVAL, "nta_gemm_f32_f64: bad m=%d n=%d ldk=%lld "
+                     "or null pointer", m, n, ldk);
+  if (m % 32 != 0 || n % 16 != 0 || ldk % 4 != 0 ||
+      ((uintptr_t)K & 15) != 0)
+    return set_error(NTA_EUNSUPPORTED, "nta_gemm_f32_f64: m=%d, n=%d and "
+                     "ldk=%lld must be multiples of 32, 16 and 4 and K "
+                     "16-byte aligned (pad the matrix)", m, n, ldk);
+  hipLaunchKernelGGL(k_symv_rows<float>, dim3(m / 32), dim3(512), 0,
+                     (hipStream_t)stream, K, ldk, n, 0.0, 2, X, Y, nullptr,
+                     nullptr, nullptr, nullptr, nullptr);
+  return check_launch("nta_gemm_f32_f64");
 }
 
 extern "C" long long nta_pcg64_work_bytes(int n, int nrhs) {
@@ -1367,7 +1737,8 @@ extern "C" int nta_chol_precond(const float* W, const float* WT, int n,
   const unsigned g = (unsigned)(((long long)n * RHS + 255) / 256);
   hipLaunchKernelGGL(k_pad32, dim3(g), dim3(256), 0, st, n, R, nrhs, w.r32,
                      1);
-  precond(W, WT, ldw, n, n1, w.r32, w.z32, nullptr, nullptr, w, st);
+  precond(W, WT, ldw, n, n1, w.r32, w.z32, nullptr, nullptr, nullptr,
+          nullptr, w, st);
   hipLaunchKernelGGL(k_pad32, dim3(g), dim3(256), 0, st, n, w.z32, nrhs, Z,
                      0);
   return check_launch("nta_chol_precond");
@@ -1420,8 +1791,8 @@ namespace nta {
 constexpr int kScratchMaxN = 16384;
 constexpr size_t kScratchBytes =
     6 * (size_t)kScratchMaxN * RHS * sizeof(double) +         // v64
-    6 * (size_t)(kScratchMaxN / 16) * RHS * sizeof(double) +  // slots
-    256 +                                                     // sums
+    7 * (size_t)(kScratchMaxN / 16) * RHS * sizeof(double) +  // slots
+    (sizeof(PcgState) + 255) / 256 * 256 +                    // state
     6 * (size_t)kScratchMaxN * RHS * sizeof(float);           // v32
 
 __device__ __align__(256) char scratch[kScratchBytes];

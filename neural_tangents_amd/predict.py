@@ -145,7 +145,7 @@ def gp_inference(k_train_train, y_train, diag_reg: float = 0.,
         mean = y_t.to(dtype=k_dd.dtype)
       else:
         alpha = k_inv_y(k)
-        mean = (k_td.to(alpha.device, torch.float64) @ alpha)
+        mean = _solve.kernel_times(k_td.to(alpha.device), alpha)
         if y_t.ndim == 1:
           mean = mean[:, 0]
       if k_test_test is not None:

@@ -1,5 +1,6 @@
-"""FP32 recursive Cholesky, its block preconditioner and the FP64 PCG solve
-of libnta.so against torch FP64 linear algebra (needs a GPU).
+"""FP32 blocked Cholesky, its block preconditioner, the FP64 PCG solve and
+the FP32 x FP64 test-set product of libnta.so against torch FP64 linear
+algebra (needs a GPU).
 
 The factor is FP32, so the preconditioner is checked to FP32 accuracy
 (relative error of (L L^T)^-1 R against the FP64 solve, scaled by the
@@ -66,10 +67,11 @@ def test_factor_blocks_match_fp64(n):
   assert float(Wd[n1:, n1:].triu(1).abs().max()) == 0.0
 
 
-def test_factor_large_node_uses_128_tiles():
-  """n = 6144: the top node's products run on 128x128 tiles (the smaller
-  nodes on 64x64 tiles and the 128x128 leaves), checked block by block
-  against torch FP64 on the GPU."""
+def test_factor_large_matches_fp64():
+  """n = 6144: twelve 512-column panels, so every update kind (the next
+  panel's head and middle columns, the bulk right of it beside the next
+  panel's factorisation) and the batched inversion joins of both halves
+  run; checked block by block against torch FP64 on the GPU."""
   n = 6144
   g = torch.Generator(device=DEV).manual_seed(11)
   X = torch.randn(n, n, generator=g, device=DEV, dtype=torch.float64)
@@ -154,3 +156,29 @@ def test_invalid_arguments_raise():
               _lib.ptr(W), _lib.ptr(W), None)
   assert _lib.value('nta_pcg64_work_bytes', 64, 17) == -1
   assert _lib.value('nta_chol_split', 100) == 0
+
+
+@pytest.mark.parametrize('m,n,c', [(1024, 8192, 10), (96, 640, 16),
+                                   (50, 300, 3), (64, 256, 20)])
+def test_kernel_times_matches_fp64_product(m, n, c):
+  """nta_gemm_f32_f64 (through _solve.kernel_times, padded when m, n are
+  not multiples of 32 and 16, in 16-column chunks) against the FP64 product
+  of the same fp32 entries: only the summation order differs."""
+  from neural_tangents_amd import _solve
+  g = torch.Generator(device=DEV).manual_seed(m + n + c)
+  K = torch.randn(m, n, generator=g, device=DEV)
+  X = torch.randn(n, c, generator=g, device=DEV, dtype=torch.float64)
+  Y = _solve.kernel_times(K, X)
+  want = K.double() @ X
+  assert Y.shape == (m, c) and Y.dtype == torch.float64
+  err = float((Y - want).abs().max() / want.abs().max())
+  assert err <= 1e-13, err
+
+
+def test_kernel_times_rejects_unpadded_shapes():
+  K = torch.zeros((48, 64), device=DEV)
+  X = torch.zeros((64, 16), dtype=torch.float64, device=DEV)
+  Y = torch.empty((48, 16), dtype=torch.float64, device=DEV)
+  with pytest.raises(NotImplementedError):
+    _lib.call('nta_gemm_f32_f64', _lib.ptr(K), 64, 48, 64, _lib.ptr(X),
+              _lib.ptr(Y), _lib.stream_handle(DEV))
