@@ -45,7 +45,6 @@ class MixedCholesky:
     npad = -(-n // self.NB) * self.NB
     self.npad = npad
     self.n1 = _lib.value('nta_chol_split', npad)
-    stream = _lib.stream_handle(self.device)
     self.k64 = K.dtype == torch.float64
     if npad == n and K.dtype == torch.float32 and K.is_contiguous():
       K32 = K
@@ -72,32 +71,46 @@ class MixedCholesky:
     # non-positive pivot: the kernel is too ill-conditioned for single
     # precision at this lambda), the shift grows by 10x; the FP64-residual
     # iteration still converges to the solution of the unshifted system.
-    mean_diag = None
-    shift = self.lam
-    info = torch.zeros((1,), dtype=torch.int32, device=self.device)
-    for attempt in range(8):
-      A = K32.clone()
-      A.diagonal().add_(shift)
-      W = torch.zeros_like(A)
-      WT = torch.empty_like(A)
-      info.zero_()
-      _lib.call('nta_chol_f32', _lib.ptr(A), npad, npad, _lib.ptr(W), npad,
-                _lib.ptr(WT), _lib.ptr(info), stream)
-      if int(info.item()) == 0:
-        break
-      if mean_diag is None:
-        mean_diag = float(torch.diagonal(K32)[:n].double().mean())
-      shift = max(10.0 * shift, 1e-7 * mean_diag)
-    else:
-      raise ValueError(
-          f'FP32 Cholesky failed at leading minor {int(info.item())} even '
-          f'with shift {shift:.3g}: the train kernel is not positive '
-          'definite; increase diag_reg or use solver="fp64".')
-    del A
-    self.shift = shift
-    self.W = W
-    self.WT = WT
+    # The pivot check waits for the first solve (the refinement synchronises
+    # anyway), so the factorisation and the refinement run back to back.
+    self._K32 = K32
+    self._info = torch.zeros((1,), dtype=torch.int32, device=self.device)
+    self.shift = self.lam
+    self._factor(self.shift)
+    self._checked = False
     self._work = {}
+
+  def _factor(self, shift):
+    A = self._K32.clone()
+    A.diagonal().add_(shift)
+    self.W = torch.zeros_like(A)
+    self.WT = torch.empty_like(A)
+    self._info.zero_()
+    _lib.call('nta_chol_f32', _lib.ptr(A), self.npad, self.npad,
+              _lib.ptr(self.W), self.npad, _lib.ptr(self.WT),
+              _lib.ptr(self._info), _lib.stream_handle(self.device))
+
+  def _factor_ok(self):
+    """True once the factor in W has no non-positive pivot; refactors with
+    a growing shift (synchronously) after a breakdown and returns False."""
+    if self._checked:
+      return True
+    if int(self._info.item()) == 0:
+      self._checked = True
+      return True
+    mean_diag = float(torch.diagonal(self._K32)[:self.n].double().mean())
+    shift = self.shift
+    for attempt in range(8):
+      shift = max(10.0 * shift, 1e-7 * mean_diag)
+      self._factor(shift)
+      if int(self._info.item()) == 0:
+        self.shift = shift
+        self._checked = True
+        return False
+    raise ValueError(
+        f'FP32 Cholesky failed at leading minor {int(self._info.item())} '
+        f'even with shift {shift:.3g}: the train kernel is not positive '
+        'definite; increase diag_reg or use solver="fp64".')
 
   def _workspace(self, nrhs):
     ws = self._work.get(nrhs)
@@ -118,6 +131,7 @@ class MixedCholesky:
 
   def precond(self, R):
     """(L L^T)^-1 R in fp32 (R: [npad, k], k <= 16)."""
+    self._factor_ok()
     R32 = R.to(torch.float32).contiguous()
     Z = torch.empty_like(R32)
     ws = self._workspace(R.shape[1])
@@ -148,6 +162,22 @@ class MixedCholesky:
     return max(100.0 * self.tol, 1e-8)
 
   def _solve16(self, b):
+    X, it, r = self._refine(b)
+    if not self._factor_ok():     # the factor was replaced: solve again
+      X, it, r = self._refine(b)
+    if not r <= self.accept_residual():
+      warnings.warn(
+          f'mixed-precision solve stopped at relative residual {r:.3g} '
+          f'after {it} iterations (limit {self.accept_residual():.3g}); '
+          'using the FP64 Cholesky solve instead.', RuntimeWarning,
+          stacklevel=3)
+      self.fell_back = True
+      return self._fp64_solve(b), it, r
+    return X[:self.n], it, r
+
+  def _refine(self, b):
+    """nta_pcg64 on the padded right-hand side: (X, iterations, final
+    relative residual recomputed from scratch)."""
     B = self._pad(b).contiguous()
     k = B.shape[1]
     X = torch.empty_like(B)
@@ -161,15 +191,7 @@ class MixedCholesky:
               _lib.ptr(X), self.tol, self.max_iter, _lib.ptr(ws), ws.numel(),
               ctypes.byref(iters), ctypes.byref(resid),
               _lib.stream_handle(self.device))
-    if not resid.value <= self.accept_residual():
-      warnings.warn(
-          f'mixed-precision solve stopped at relative residual '
-          f'{resid.value:.3g} after {iters.value} iterations (limit '
-          f'{self.accept_residual():.3g}); using the FP64 Cholesky solve '
-          'instead.', RuntimeWarning, stacklevel=3)
-      self.fell_back = True
-      return self._fp64_solve(b), iters.value, resid.value
-    return X[:self.n], iters.value, resid.value
+    return X, iters.value, resid.value
 
   def _fp64_solve(self, b):
     """Plain FP64 Cholesky solve of (K + lambda I) x = b (the fallback)."""

@@ -325,7 +325,8 @@ static GemmJob batched(GemmJob j, int batch, long long sA, long long sB,
   return j;
 }
 
-static void launch_gemm(const GemmJob& a, const GemmJob* b, hipStream_t st) {
+static void launch_gemm(const GemmJob& a, const GemmJob* b, hipStream_t st,
+                        int max_wgs = 0) {
   GemmJobs jobs;
   jobs.j[0] = a;
   if (b)
@@ -337,6 +338,7 @@ static void launch_gemm(const GemmJob& a, const GemmJob* b, hipStream_t st) {
   int grid = jobs.j[0].ntiles * jobs.j[0].batch +
              jobs.j[1].ntiles * jobs.j[1].batch;
   if (grid <= 0) return;
+  if (max_wgs > 0 && grid > max_wgs) grid = max_wgs;
   hipLaunchKernelGGL(k_gemm<kTile>, dim3(grid), dim3(256), 0, st, jobs);
 }
 
@@ -754,20 +756,22 @@ __global__ __launch_bounds__(256) void k_leaf128(const float* __restrict__ A,
 // or more workgroups per CU, are faster (7168^2 lower x 512: 94 against 62
 // TFLOP/s on one MI355X).  The factor's inverse is then built by recursive
 // doubling inside each half (chol_invert_half).
-// Events of the side stream: the last panel product, the last mid update,
-// the last two bulk updates, the side stream's end.
-enum { kEvPanel, kEvMid, kEvBulk0, kEvBulk1, kEvSideEnd, kCholEvents };
+// Events: the last panel product, the last mid update, the last two bulk
+// updates, the side stream's end, the end of the top half's factorisation
+// and of its inversion (on the third stream).
+enum { kEvPanel, kEvMid, kEvBulk0, kEvBulk1, kEvSideEnd, kEvSplit, kEvInvEnd,
+       kCholEvents };
 struct CholStreams {
-  hipStream_t side = nullptr;
+  hipStream_t side = nullptr, inv = nullptr;
   hipEvent_t ev[kCholEvents] = {};
 };
 constexpr int kCholDevices = 64;
 static std::mutex g_chol_mu;
 static CholStreams g_chol[kCholDevices];
 
-// The side stream and events of the current device (created on first use,
-// default priority, not synchronising with the null stream), or nullptr
-// after setting the error.
+// The side and inversion streams and the events of the current device
+// (created on first use, default priority, not synchronising with the null
+// stream), or nullptr after setting the error.
 static CholStreams* chol_streams() {
   int dev = -1;
   hipError_t e = hipGetDevice(&dev);
@@ -779,9 +783,10 @@ static CholStreams* chol_streams() {
   std::lock_guard<std::mutex> lock(g_chol_mu);
   CholStreams& c = g_chol[dev];
   if (!c.side) {
-    hipStream_t s = nullptr;
+    hipStream_t s = nullptr, iv = nullptr;
     hipEvent_t ev[kCholEvents] = {};
     e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&iv, hipStreamNonBlocking);
     for (int k = 0; k < kCholEvents && e == hipSuccess; ++k)
       e = hipEventCreateWithFlags(&ev[k], hipEventDisableTiming);
     if (e != hipSuccess) {
@@ -790,6 +795,7 @@ static CholStreams* chol_streams() {
       return nullptr;
     }
     for (int k = 0; k < kCholEvents; ++k) c.ev[k] = ev[k];
+    c.inv = iv;
     c.side = s;
   }
   return &c;
@@ -811,8 +817,28 @@ static int chol_split_of(int n) {
 
 constexpr int kOuter = 512;
 
+static void chol_invert_half(float* W, long long ldw,
+                             const std::vector<int>& off, int kb, int ke,
+                             float* tmp, long long ldt, hipStream_t st,
+                             int smax, int max_wgs = 0);
+
+// split > 0 (a panel boundary): once columns [0, split) are factored, the
+// top half [0, split) is inverted on the inversion stream (scratch: A's
+// rows above split, which the rest of the factorisation no longer touches)
+// beside the rest of the factorisation; kEvInvEnd marks its end.
+#ifndef NTA_SIDE_WGS
+#define NTA_SIDE_WGS 0
+#endif
 static int chol_factor(float* A, long long lda, float* W, long long ldw,
-                       int n, int* info, hipStream_t st, CholStreams& cs) {
+                       int n, int* info, hipStream_t st, CholStreams& cs,
+                       int split) {
+  int side_wgs = 0;
+  if (NTA_SIDE_WGS > 0) {
+    int dev = 0, ncu = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    side_wgs = NTA_SIDE_WGS * ncu;
+  }
   hipStream_t sd = cs.side;
   hipEvent_t* ev = cs.ev;
   hipError_t e = hipSuccess;
@@ -830,7 +856,7 @@ static int chol_factor(float* A, long long lda, float* W, long long ldw,
     const float* Lc = W + (long long)c0 * ldw + kc;
     launch_gemm(make_job(Lr, ldw, Lc, ldw, A + (long long)r * lda + c0, lda,
                          n - r, c1 - c0, K, -1.0f, 1.0f, 0, 0, lower),
-                nullptr, s);
+                nullptr, s, s == st ? 0 : side_wgs);
   };
   const std::vector<int> off = chol_offsets(n);
   const int m = (int)off.size() - 1;
@@ -862,6 +888,14 @@ static int chol_factor(float* A, long long lda, float* W, long long ldw,
         if (mid_pending) wait(st, kEvMid);
         mid_pending = false;
         update(r0, r0, E0, o, b, 0, st);
+      }
+    }
+    if (E0 == split) {
+      record(kEvSplit, st);
+      wait(cs.inv, kEvSplit);
+      if (e == hipSuccess) {
+        chol_invert_half(W, ldw, off, 0, ke, A, lda, cs.inv, m, side_wgs);
+        record(kEvInvEnd, cs.inv);
       }
     }
     if (E0 >= n || e != hipSuccess) break;
@@ -906,7 +940,7 @@ static int chol_factor(float* A, long long lda, float* W, long long ldw,
 static void chol_invert_half(float* W, long long ldw,
                              const std::vector<int>& off, int kb, int ke,
                              float* tmp, long long ldt, hipStream_t st,
-                             int smax) {
+                             int smax, int max_wgs) {
   const int nb = ke - kb;
   for (int s = 1; s < nb && s < smax; s *= 2) {
     // joins: left blocks [kb + 2 g s, + s), right [kb + (2 g + 1) s, + s)
@@ -944,8 +978,8 @@ static void chol_invert_half(float* W, long long ldw,
       jobs_for(g1, 1, ro - lo, re - ro);
     }
     if (nj == 0) break;
-    launch_gemm(tj[0], nj > 1 ? &tj[1] : nullptr, st);
-    launch_gemm(wj[0], nj > 1 ? &wj[1] : nullptr, st);
+    launch_gemm(tj[0], nj > 1 ? &tj[1] : nullptr, st, max_wgs);
+    launch_gemm(wj[0], nj > 1 ? &wj[1] : nullptr, st, max_wgs);
   }
 }
 
@@ -1210,7 +1244,7 @@ struct PcgState {
   int done;            // 1: converged, every later kernel is a no-op
   int iters;           // iteration that set done (or the last checked)
   int restart;         // 1: the next direction is z (beta = 0)
-  int pad;
+  int skip;            // 1: the recomputed residual alone meets tol
 };
 
 // Per-column sums of nslot partials [slot][RHS] by a 256-thread block:
@@ -1264,6 +1298,7 @@ __global__ __launch_bounds__(256) void k_pcg_init(
     st->done = 0;
     st->iters = 0;
     st->restart = 1;
+    st->skip = 0;
   }
 }
 
@@ -1347,7 +1382,7 @@ __global__ __launch_bounds__(256) void k_pcg_check(
     int nrzx, const double* __restrict__ xx, int nxx,
     const double* __restrict__ rz, const double* __restrict__ zz, int nrz,
     int nrhs, double tol, int it, PcgState* __restrict__ st) {
-  if (rzx && st->done) return;
+  if (rzx ? st->done : st->skip) return;
   __shared__ double scratch[256], srr[RHS], srx[RHS], sxx[RHS], srz[RHS],
       szz[RHS];
   __shared__ int ok[RHS];
@@ -1376,6 +1411,37 @@ __global__ __launch_bounds__(256) void k_pcg_check(
     st->done = all;
     st->iters = it;
     st->restart = rzx ? 0 : 1;
+  }
+}
+
+// The test on a recomputed residual before its preconditioning: when every
+// column's relative residual is <= tol the solve has converged (skip = 1,
+// done = 1) and the preconditioner and k_pcg_check of the recomputation
+// return at once; otherwise skip = 0 and they run.
+__global__ __launch_bounds__(256) void k_pcg_rcheck(
+    const double* __restrict__ rr, int nrr, int nrhs, double tol, int it,
+    PcgState* __restrict__ st) {
+  __shared__ double scratch[256], srr[RHS];
+  __shared__ int ok[RHS];
+  block_slot_sums(rr, nrr, scratch, srr);
+  const int t = threadIdx.x;
+  double rel = 0.0;
+  if (t < RHS) {
+    const double bn = st->bn[t];
+    rel = bn > 0.0 ? sqrt(srr[t]) / bn : 0.0;
+    ok[t] = (t >= nrhs || bn == 0.0 || rel <= tol) ? 1 : 0;
+  }
+  __syncthreads();
+  int all = 1;
+  for (int c = 0; c < RHS; ++c) all &= ok[c];
+  if (all && t < RHS) st->rel[t] = rel;
+  if (t == 0) {
+    st->skip = all;
+    if (all) {
+      st->done = 1;
+      st->iters = it;
+      st->restart = 1;
+    }
   }
 }
 
@@ -1565,8 +1631,10 @@ static int pcg_run(const TK* K, int n, long long ldk, double lam,
     hipLaunchKernelGGL(k_symv_rows<TK>, dim3(nsym), dim3(512), 0, st, K,
                        ldk, n, lam, 1, w.x, nullptr, w.b, w.r, w.r32, w.rr,
                        nullptr);
-    precond(W, WT, ldw, n, n1, w.r32, w.z32, w.r, w.rz, w.zz, nullptr, w,
-            st);
+    hipLaunchKernelGGL(k_pcg_rcheck, dim3(1), dim3(256), 0, st, w.rr, nsym,
+                       nrhs, tol, it, w.st);
+    precond(W, WT, ldw, n, n1, w.r32, w.z32, w.r, w.rz, w.zz, &w.st->skip,
+            w, st);
     hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(256), 0, st, w.rr, nsym,
                        nullptr, 0, w.xx, nvec, w.rz, w.zz, nrz, nrhs, tol,
                        it, w.st);
@@ -1662,17 +1730,28 @@ extern "C" int nta_chol_f32(float* A, int n, long long lda, float* W,
   hipStream_t st = (hipStream_t)stream;
   CholStreams* cs = chol_streams();
   if (!cs) return NTA_EINVAL;
-  int rc = chol_factor(A, lda, W, ldw, n, info, st, *cs);
-  if (rc) return rc;
   const std::vector<int> off = chol_offsets(n);
   const int m = (int)off.size() - 1;
-  if (m > 1) {
-    // W11 = L11^-1, W22 = L22^-1; the block below them keeps L21.  The
-    // products' scratch is A (no longer read once factored).  When the
-    // halves are equal powers of two of 128-blocks, each level's joins of
-    // both halves run as one batch (no join of size < mh crosses the
-    // boundary); otherwise half by half.
-    const int mh = m / 2;
+  const int mh = m / 2;
+  // W11 = L11^-1, W22 = L22^-1; the block below them keeps L21.  The
+  // products' scratch is A (no longer read once factored).  When the split
+  // is a panel boundary, the top half is inverted on the third stream while
+  // the bottom half is factored (its scratch rows [0, n1) of A are done
+  // with by then), and the bottom half after the factorisation, its scratch
+  // in rows [n1, n).
+  const bool overlap = m > 1 && mh % (kOuter / 128) == 0;
+  int rc = chol_factor(A, lda, W, ldw, n, info, st, *cs,
+                       overlap ? off[mh] : -1);
+  if (rc) return rc;
+  if (overlap) {
+    chol_invert_half(W, ldw, off, mh, m, A + (long long)off[mh] * lda, lda,
+                     st, m);
+    hipError_t e = hipStreamWaitEvent(st, cs->ev[kEvInvEnd], 0);
+    if (e != hipSuccess)
+      return set_error((int)e, "nta_chol_f32: %s", hipGetErrorString(e));
+  } else if (m > 1) {
+    // equal powers of two of 128-blocks: each level's joins of both halves
+    // run as one batch (no join of size < mh crosses the boundary)
     const bool pow2 = (mh & (mh - 1)) == 0;
     if (m == 2 * mh && n % 128 == 0 && pow2) {
       chol_invert_half(W, ldw, off, 0, m, A, lda, st, mh);

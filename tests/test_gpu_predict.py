@@ -101,10 +101,14 @@ def test_posterior_covariance_matches_oracle():
 
 
 def test_non_spd_raises():
+  """The mixed solver checks the FP32 pivots at the first solve (the
+  factorisation is not waited for in the constructor); the FP64 solver at
+  construction."""
   from neural_tangents_amd import _solve
   K = -torch.eye(128, dtype=torch.float64).cuda()
   with pytest.raises(ValueError):
-    _solve.CholeskySolver(K, 0.0, True, 'mixed')
+    _solve.CholeskySolver(K, 0.0, True, 'mixed').solve(
+        torch.ones((128, 2), dtype=torch.float64, device='cuda'))
   with pytest.raises(ValueError):
     _solve.CholeskySolver(K, 0.0, True, 'fp64')
 
