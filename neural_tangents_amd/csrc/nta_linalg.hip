@@ -325,8 +325,7 @@ static GemmJob batched(GemmJob j, int batch, long long sA, long long sB,
   return j;
 }
 
-static void launch_gemm(const GemmJob& a, const GemmJob* b, hipStream_t st,
-                        int max_wgs = 0) {
+static void launch_gemm(const GemmJob& a, const GemmJob* b, hipStream_t st) {
   GemmJobs jobs;
   jobs.j[0] = a;
   if (b)
@@ -338,7 +337,6 @@ static void launch_gemm(const GemmJob& a, const GemmJob* b, hipStream_t st,
   int grid = jobs.j[0].ntiles * jobs.j[0].batch +
              jobs.j[1].ntiles * jobs.j[1].batch;
   if (grid <= 0) return;
-  if (max_wgs > 0 && grid > max_wgs) grid = max_wgs;
   hipLaunchKernelGGL(k_gemm<kTile>, dim3(grid), dim3(256), 0, st, jobs);
 }
 
@@ -754,8 +752,10 @@ __global__ __launch_bounds__(256) void k_leaf128(const float* __restrict__ A,
 // updates use 64x64 tiles: a 128x128-tile workgroup fills a CU's registers
 // (one wave per SIMD), and on these K = 512 updates the smaller tiles, two
 // or more workgroups per CU, are faster (7168^2 lower x 512: 94 against 62
-// TFLOP/s on one MI355X).  The factor's inverse is then built by recursive
-// doubling inside each half (chol_invert_half).
+// TFLOP/s on one MI355X).  The factor's inverse is built by recursive
+// doubling inside each half (chol_invert_half): the top half on a third
+// stream as soon as it is factored, the bottom half after the
+// factorisation.
 // Events: the last panel product, the last mid update, the last two bulk
 // updates, the side stream's end, the end of the top half's factorisation
 // and of its inversion (on the third stream).
@@ -820,25 +820,15 @@ constexpr int kOuter = 512;
 static void chol_invert_half(float* W, long long ldw,
                              const std::vector<int>& off, int kb, int ke,
                              float* tmp, long long ldt, hipStream_t st,
-                             int smax, int max_wgs = 0);
+                             int smax);
 
 // split > 0 (a panel boundary): once columns [0, split) are factored, the
 // top half [0, split) is inverted on the inversion stream (scratch: A's
 // rows above split, which the rest of the factorisation no longer touches)
 // beside the rest of the factorisation; kEvInvEnd marks its end.
-#ifndef NTA_SIDE_WGS
-#define NTA_SIDE_WGS 0
-#endif
 static int chol_factor(float* A, long long lda, float* W, long long ldw,
                        int n, int* info, hipStream_t st, CholStreams& cs,
                        int split) {
-  int side_wgs = 0;
-  if (NTA_SIDE_WGS > 0) {
-    int dev = 0, ncu = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    side_wgs = NTA_SIDE_WGS * ncu;
-  }
   hipStream_t sd = cs.side;
   hipEvent_t* ev = cs.ev;
   hipError_t e = hipSuccess;
@@ -856,7 +846,7 @@ static int chol_factor(float* A, long long lda, float* W, long long ldw,
     const float* Lc = W + (long long)c0 * ldw + kc;
     launch_gemm(make_job(Lr, ldw, Lc, ldw, A + (long long)r * lda + c0, lda,
                          n - r, c1 - c0, K, -1.0f, 1.0f, 0, 0, lower),
-                nullptr, s, s == st ? 0 : side_wgs);
+                nullptr, s);
   };
   const std::vector<int> off = chol_offsets(n);
   const int m = (int)off.size() - 1;
@@ -894,7 +884,7 @@ static int chol_factor(float* A, long long lda, float* W, long long ldw,
       record(kEvSplit, st);
       wait(cs.inv, kEvSplit);
       if (e == hipSuccess) {
-        chol_invert_half(W, ldw, off, 0, ke, A, lda, cs.inv, m, side_wgs);
+        chol_invert_half(W, ldw, off, 0, ke, A, lda, cs.inv, m);
         record(kEvInvEnd, cs.inv);
       }
     }
@@ -940,7 +930,7 @@ static int chol_factor(float* A, long long lda, float* W, long long ldw,
 static void chol_invert_half(float* W, long long ldw,
                              const std::vector<int>& off, int kb, int ke,
                              float* tmp, long long ldt, hipStream_t st,
-                             int smax, int max_wgs) {
+                             int smax) {
   const int nb = ke - kb;
   for (int s = 1; s < nb && s < smax; s *= 2) {
     // joins: left blocks [kb + 2 g s, + s), right [kb + (2 g + 1) s, + s)
@@ -978,8 +968,8 @@ static void chol_invert_half(float* W, long long ldw,
       jobs_for(g1, 1, ro - lo, re - ro);
     }
     if (nj == 0) break;
-    launch_gemm(tj[0], nj > 1 ? &tj[1] : nullptr, st, max_wgs);
-    launch_gemm(wj[0], nj > 1 ? &wj[1] : nullptr, st, max_wgs);
+    launch_gemm(tj[0], nj > 1 ? &tj[1] : nullptr, st);
+    launch_gemm(wj[0], nj > 1 ? &wj[1] : nullptr, st);
   }
 }
 
