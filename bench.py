@@ -352,6 +352,63 @@ SCHEDULE_NOTES = {
 }
 
 
+def readme_cnn21_result(n_flat=512, n_gap=16, steps=3):
+  """The reference README's performance table (``README.md:370-416``):
+  21 x [Conv(1, (3, 3), (1, 1), 'SAME'), Relu] on 32x32x3 inputs, timed as
+  milliseconds per NTK entry of an N x N batch (distinct x1, x2, nngp and
+  ntk) with a Flatten readout (the diagonal-spatial path, one
+  nta_diag_chain launch) and with a GlobalAvgPool readout (21 full-spatial
+  stages), next to the README's NVIDIA V100 fp32 figures.  The error
+  against the FP64 oracle is taken on a 2x2 block (Flatten) and one entry
+  (GlobalAvgPool), outside the timed region."""
+  import numpy as np
+  import torch
+  from neural_tangents_amd import stax
+  from neural_tangents_amd.data import blurred_images
+  from oracle import stax_oracle as so
+  dev = torch.device('cuda', 0)
+  layers, olayers = [], []
+  for _ in range(21):
+    layers += [stax.Conv(1, (3, 3), (1, 1), 'SAME'), stax.Relu()]
+    olayers += [so.conv_layer((3, 3), (1, 1), 'SAME', 1.0, None),
+                so.relu_layer()]
+  res = {'architecture': "21 x [Conv(1, (3, 3), (1, 1), 'SAME'), Relu], "
+                         '32x32x3 inputs (README.md:370-378)'}
+  for name, n, v100, readout, oreadout, diag in (
+      ('flatten', n_flat, 0.0046510, stax.Flatten(), so.flatten_layer(),
+       True),
+      ('global_avg_pool', n_gap, 2.7001, stax.GlobalAvgPool(),
+       so.global_avg_pool_layer(), False)):
+    kfn = stax.serial(*layers, readout)[2]
+    x1n, x2n = blurred_images([n, n], 32, seed=1)
+    x1 = torch.from_numpy(x1n).to(dev)
+    x2 = torch.from_numpy(x2n).to(dev)
+    k = kfn(x1, x2, ('nngp', 'ntk'))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+      k = kfn(x1, x2, ('nngp', 'ntk'))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ms_entry = dt * 1e3 / (n * n)
+    rows, cols = ([0, n - 1], [0, n - 1]) if diag else ([0], [n - 1])
+    ref = so.kernel_fn(olayers + [oreadout], x1n[rows], x2n[cols],
+                       ('nngp', 'ntk'), np.float64, diagonal_spatial=diag)
+    err = {}
+    for g in ('nngp', 'ntk'):
+      got = getattr(k, g).cpu().numpy()[np.ix_(rows, cols)].astype(np.float64)
+      err[g] = float(np.linalg.norm(got - ref[g]) / np.linalg.norm(ref[g]))
+    log(f'readme cnn21 {name}: {ms_entry:.3g} ms/entry (V100 {v100})')
+    res[name] = {'batch': [n, n], 'ms_per_entry': ms_entry,
+                 'entries_per_s': 1e3 / ms_entry,
+                 'readme_v100_fp32_ms_per_entry': v100,
+                 'speedup_vs_readme_v100': v100 / ms_entry,
+                 'rel_err_vs_fp64_oracle': err}
+    del k
+    torch.cuda.empty_cache()
+  return res
+
+
 def valu_busy(kernel):
   d = _load_json(VALU_FILE)
   try:
@@ -371,6 +428,8 @@ def main():
   ap.add_argument('--no-krr', action='store_true')
   ap.add_argument('--no-m10', action='store_true')
   ap.add_argument('--no-grad', action='store_true')
+  ap.add_argument('--no-readme', action='store_true',
+                  help='skip the README 21-layer CNN ms/entry lines')
   ap.add_argument('--pmc-calibration', action='store_true',
                   help='first copy 256 MiB with torch (the reference launch '
                   'that scales FETCH_SIZE / WRITE_SIZE counters)')
@@ -450,7 +509,8 @@ def main():
                  'get': ['nngp', 'ntk'], 'parallelism': 'single GPU'},
       'path_gbs_canonical': bpp * value / 1e9,
       'path_pct_hbm_peak': 100.0 * bpp * value / 1e9 / HBM_PEAK_GBS,
-      'roofline': {'bound': 'valu' if fused else 'hbm', 'kernel': dom_name,
+      'roofline': {'bound': 'hbm', 'limiter': 'valu' if fused else 'hbm',
+                   'kernel': dom_name,
                    'achieved': achieved, 'peak': HBM_PEAK_GBS,
                    'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
                    'traffic': traffic, 'avg_launch_ms': avg_ms,
@@ -473,6 +533,8 @@ def main():
   if not args.no_grad and args.workload == 'myrtle5_32x32_256x256':
     out['grad'] = grad_result()
     torch.cuda.empty_cache()
+  if not args.no_readme and args.workload == 'myrtle5_32x32_256x256':
+    out['readme_cnn21'] = readme_cnn21_result()
   if not args.no_cpu:
     # Error block: rows on both sides of the first row-tile boundary and in
     # the last (tail) row tile; first, middle and last columns.

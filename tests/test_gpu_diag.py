@@ -84,3 +84,21 @@ def test_readme_21_layer_cnn_without_pooling():
                       ('nngp', 'ntk'), diagonal_spatial=True)
   for g in ('nngp', 'ntk'):
     assert rel_err(getattr(got, g).double().cpu().numpy(), want[g]) <= 1e-5
+
+
+def test_readme_21_layer_cnn_with_global_pooling():
+  """README.md:380-396: the same 21 layers with a GlobalAvgPool readout run
+  as 21 full-spatial Conv-Relu stages; one cross entry and the self chains
+  against the FP64 oracle (about 7 s of numpy)."""
+  x1, x2 = blurred_images([1, 1], 32, seed=6)
+  ours, ref = [], []
+  for _ in range(21):
+    ours += [stax.Conv(1, (3, 3), (1, 1), 'SAME'), stax.Relu()]
+    ref += [so.conv_layer((3, 3), (1, 1), 'SAME'), so.relu_layer()]
+  ours += [stax.GlobalAvgPool()]
+  ref += [so.global_avg_pool_layer()]
+  got = stax.serial(*ours)[2](x1, x2, ('nngp', 'ntk'))
+  want = so.kernel_fn(ref, x1.astype(np.float64), x2.astype(np.float64),
+                      ('nngp', 'ntk'))
+  for g in ('nngp', 'ntk'):
+    assert rel_err(getattr(got, g).double().cpu().numpy(), want[g]) <= 1e-5
