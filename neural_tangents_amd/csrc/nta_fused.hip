@@ -821,6 +821,10 @@ __global__ __launch_bounds__(256) void k_first16(
 // (1.5x the first-stage work), which costs far less than the 16 MiB per
 // pair of state the unfused pair of stages writes and reads back.
 constexpr int kXS = 34;               // padded image row (positions)
+// Rows past the image that the last sweep steps' operand reads reach (their
+// planes lie past the sweep end and are dropped): the image array carries
+// them as zeroed padding, so every read stays inside it.
+constexpr int kXhTail = 4 * kXS * 4;  // halves
 constexpr int kLS2 = 36;              // exchange plane row (4 rows = 16 banks)
 // Vector instructions scheduled between consecutive MFMAs of the next step's
 // patch Grams (0: leave the order to the compiler).
@@ -899,7 +903,7 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
   constexpr int NT = NTK ? 2 : 1;
   // Images i, j as FP16 hi/lo pairs: xh[img][hl][1 + h][1 + w][c] (zero
   // padded, channels padded to 4).
-  __shared__ __align__(16) _Float16 xh[2 * 2 * kXS * kXS * 4];
+  __shared__ __align__(16) _Float16 xh[2 * 2 * kXS * kXS * 4 + kXhTail];
   __shared__ __align__(16) float lp[6 * kPL2];        // exchange planes
   __shared__ unsigned smax[2];
   const bool self = n2 == 0;
@@ -984,6 +988,9 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
       *reinterpret_cast<half4v*>(xh + ((img * 2 + 1) * (kXS * kXS) + pos) *
                                           4) = lv4;
     }
+    for (int k = t; k < kXhTail / 4; k += 256)
+      *reinterpret_cast<half4v*>(xh + 2 * 2 * kXS * kXS * 4 + 4 * k) =
+          half4v{0, 0, 0, 0};
   }
   __syncthreads();
   // MFMA operands (v_mfma_f32_16x16x32_f16, 8 halves per lane; A and B
@@ -1355,7 +1362,7 @@ void k_first3_pool32(const float* __restrict__ x1,
   constexpr int S = 32, H2 = 16, SO = 16;
   constexpr long long PER_OUT = (long long)SO * SO * SO * SO;
   constexpr int NT = NTK ? 2 : 1;
-  __shared__ __align__(16) _Float16 xh[2 * 2 * kXS * kXS * 4];
+  __shared__ __align__(16) _Float16 xh[2 * 2 * kXS * kXS * 4 + kXhTail];
   __shared__ __align__(16) float lp[6 * kPL2];        // exchange planes
   __shared__ unsigned smax[2];
   const bool self = n2 == 0;
@@ -1428,6 +1435,9 @@ void k_first3_pool32(const float* __restrict__ x1,
     *reinterpret_cast<half4v*>(xh + ((img * 2 + 1) * (kXS * kXS) + pos) *
                                         4) = lv4;
   }
+  for (int k = t; k < kXhTail / 4; k += 256)
+    *reinterpret_cast<half4v*>(xh + 2 * 2 * kXS * kXS * 4 + 4 * k) =
+        half4v{0, 0, 0, 0};
   __syncthreads();
   constexpr int kRowB = kXS * 4 * 2;
   constexpr int kHlB = kXS * kXS * 4 * 2;
@@ -1441,10 +1451,9 @@ void k_first3_pool32(const float* __restrict__ x1,
                     : (st == 0 ? (kXS - 1) * 8 : 0);
   }
   const char* xhb = reinterpret_cast<const char*>(xh);
-  // Operands of image row `row` (clamped into the image: rows past it
-  // belong to planes that are dropped).
+  // Operands of image row `row` (rows past the image belong to dropped
+  // planes and read the zeroed tail of xh).
   auto opnd8 = [&](int img, int hl, int row) {
-    row = row < S ? row : S - 1;
     const char* b = xhb + (img * 2 + hl) * kHlB + row * kRowB +
                     (img ? ob[0] : oa[0]);
     const half4v lo = *reinterpret_cast<const half4v*>(b);
@@ -1452,7 +1461,6 @@ void k_first3_pool32(const float* __restrict__ x1,
     return half8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
   };
   auto opnd4 = [&](int img, int hl, int row) {
-    row = row < S ? row : S - 1;
     const char* b = xhb + (img * 2 + hl) * kHlB + row * kRowB +
                     (img ? ob[1] : oa[1]);
     const half4v lo = *reinterpret_cast<const half4v*>(b);
