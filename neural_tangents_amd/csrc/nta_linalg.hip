@@ -351,9 +351,8 @@ static void launch_gemm(const GemmJob& a, const GemmJob* b, hipStream_t st) {
 //      substitution (one column of one block per lane), the blocks below
 //      them by two 16x16 MFMA products;
 //   5. W21 = -L22^-1 (L21 L11^-1) on MFMA.
-// k_leaf64 runs this on a 64 block; leaf128_body runs it twice inside a 128
-// block, with the products that join the two halves on four wavefronts, so
-// a 128 block is one launch instead of five.
+// k_leaf64 runs this on a 64 block (the last block when n is an odd
+// multiple of 64); the 128 blocks have their own leaf (leaf128_body).
 constexpr int LP = 33;                      // LDS row stride of 32x32 tiles
 constexpr int LQ = 36;                      // 16-byte aligned LDS rows
 constexpr int XS = 65;                      // LDS row stride of 64x64 blocks
@@ -648,84 +647,312 @@ __global__ __launch_bounds__(64) void k_leaf64(const float* __restrict__ A,
   block64_out<64>(W + (long long)k0 * ldw + k0, ldw, X, false, t);
 }
 
-// The 128x128 diagonal block at (k0, k0), factored and inverted as two
-// 64x64 halves:
-// W11 = L11^-1, S = A21 W11^T, A22 -= S S^T, T = S W11, W22 = L22^-1,
-// W21 = -W22 T, in LDS (three 64x64 regions: W11 goes to memory as soon as
-// the products that read it are done, and its region then holds T).
-// Writes the lower 32-blocks of W = L^-1 at (k0, k0); A is only read.  Run
-// by a workgroup of 256 threads.
-__device__ __forceinline__ void leaf128_body(
-    const float* __restrict__ A, long long lda, float* __restrict__ W,
-    long long ldw, int k0, int* __restrict__ info, float* __restrict__ X11,
-    float* __restrict__ X21, float* __restrict__ X22, LeafLds& s) {
-  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const float* A0 = A + (long long)k0 * lda + k0;
-  block64_in<256>(X11, A0, lda, t);
-  block64_in<256>(X21, A0 + 64 * lda, lda, t);
-  block64_in<256>(X22, A0 + 64 * lda + 64, lda, t);
-  __syncthreads();
-  int bad = chol_inv64(X11, t, s);
-  // wave w: 32x32 tile (w / 2, w % 2) of the 64x64 products
-  const int ib = w >> 1, jb = w & 1;
-  {
-    // S = A21 W11^T (W11 lower: its upper-right block is zero)
-    const floatx16 d = mm32<XS, XS, 64, false, true>(
-        X21 + 32 * ib * XS, X11 + 32 * jb * XS, lane);
-    __syncthreads();                           // every read of A21 done
+// ------------------------------------------------ 128x128 leaf
+// The 128x128 diagonal block at (k0, k0) in one workgroup of four
+// wavefronts, factored first and inverted afterwards:
+//   1. four 32-column panels, right-looking.  A panel's rows are held one
+//      per lane, up to 64 rows by the leading wavefront (its first 32 lanes
+//      are the diagonal block and carry the pivot chain, as in panel32) and
+//      the rows below by a second wavefront, which takes each four-column
+//      block's pivots and columns from LDS after a workgroup barrier;
+//      between panels the trailing lower 32x32 tiles are updated on MFMA by
+//      all four wavefronts;
+//   2. W = L^-1: the eight 16x16 diagonal blocks by forward substitution
+//      (two wavefronts, one column per lane), then recursive doubling
+//      [W_l 0; L_rl W_l] -> [W_l 0; -W_r (L_rl W_l) W_r] at 16 (four joins,
+//      one per wavefront), 32 (two) and 64 (one, its four 32x32 tiles on
+//      the four wavefronts), each product on MFMA from LDS.
+// Only the pivot chains of step 1 are serial; the inverse, on one
+// wavefront inside each half when the halves were inverted on the way,
+// now runs after the factor on all four.  Writes the lower 32-blocks of
+// W = L^-1 at (k0, k0) (its upper-right 32-blocks are left untouched); A is
+// only read.
+constexpr int XL = 129;                     // LDS row stride of the block
+constexpr int LT = 20;                      // 16-byte aligned 16-blocks
+struct Leaf2Lds {
+  float X[128 * XL];                        // the block, then L, then W
+  float T[64 * 65];                         // join products
+  float Lt[8][16 * LT];                     // diagonal 16-blocks, L^T
+  float dg[128];                            // 1 / L_ii
+  float4 lc[2][32];                         // a panel block's columns
+  float rs[2][4];                           // and its pivots' 1/sqrt
+};
+
+// One 32-column panel (columns [c0, c0 + 32), rows [c0, 128)) held in a[]
+// by wavefronts 0 (rows c0 + lane) and 1 (rows c0 + 64 + lane); every
+// thread of the workgroup calls (barrier per four-column block).  Returns
+// on wavefront 0 the 1-based column of the first non-positive pivot in the
+// panel (0 if none).
+__device__ __forceinline__ int panel32_2w(float (&a)[32], int w, int lane,
+                                          Leaf2Lds& s) {
+  int bad = 0;
 #pragma unroll
-    for (int e = 0; e < 16; ++e)
-      X21[(32 * ib + mfma_row(e, lane)) * XS + 32 * jb + (lane & 31)] = d[e];
-  }
-  __syncthreads();
-  float* W0 = W + (long long)k0 * ldw + k0;
-  {
-    // T = S W11; A22 -= S S^T on the three lower tiles; W11 out; T into
-    // the W11 region once every read of it is done
-    const floatx16 d = mm32<XS, XS, 64, false, false>(
-        X21 + 32 * ib * XS, X11 + 32 * jb, lane);
-    if (jb <= ib) {
-      const floatx16 g = mm32<XS, XS, 64, false, true>(
-          X21 + 32 * ib * XS, X21 + 32 * jb * XS, lane);
+  for (int j = 0; j < 32; j += 4) {
+    const int pb = (j >> 2) & 1;
+    float4* buf = s.lc[pb];
+    if (w == 0) {
+      float rsv[4];
 #pragma unroll
-      for (int e = 0; e < 16; ++e)
-        X22[(32 * ib + mfma_row(e, lane)) * XS + 32 * jb + (lane & 31)] -=
-            g[e];
+      for (int u = 0; u < 4; ++u) {
+        const int c = j + u;
+        float piv = __int_as_float(
+            __builtin_amdgcn_readlane(__float_as_int(a[c]), c));
+        if (!(piv > 0.0f)) {                  // scale 1: output finite
+          if (bad == 0) bad = c + 1;
+          piv = 1.0f;
+        }
+        // lane c holds the pivot, so a[c] r is its sqrt; the lanes above
+        // the diagonal compute values that are never read (only the
+        // write-back masks them)
+        const float r = __builtin_amdgcn_rsqf(piv);
+        rsv[u] = r;
+        const float l = a[c] * r;
+        a[c] = l;
+#pragma unroll
+        for (int v = u + 1; v < 4; ++v) {
+          const float lk = __int_as_float(
+              __builtin_amdgcn_readlane(__float_as_int(l), j + v));
+          a[j + v] = fmaf(-l, lk, a[j + v]);
+        }
+      }
+      if (lane < 32) buf[lane] = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
+      if (lane == 0) {
+        s.rs[pb][0] = rsv[0];
+        s.rs[pb][1] = rsv[1];
+        s.rs[pb][2] = rsv[2];
+        s.rs[pb][3] = rsv[3];
+      }
     }
-    block64_out<256>(W0, ldw, X11, false, t);
     __syncthreads();
+    if (w == 1) {
+      // the block's four columns for rows below the leading wavefront
+      float l[4];
 #pragma unroll
-    for (int e = 0; e < 16; ++e)
-      X11[(32 * ib + mfma_row(e, lane)) * XS + 32 * jb + (lane & 31)] = d[e];
-  }
-  __syncthreads();
-  const int bad2 = chol_inv64(X22, t, s);
-  if (bad == 0 && bad2 != 0) bad = 64 + bad2;
-  if (t == 0 && bad != 0 && *info == 0) *info = k0 + bad;
-  {
-    // W21 = -W22 T into the S slot (S is no longer read)
-    const floatx16 d = mm32<XS, XS, 64, false, false>(
-        X22 + 32 * ib * XS, X11 + 32 * jb, lane);
+      for (int u = 0; u < 4; ++u) {
+        float x = a[j + u];
+        const float4 bu = buf[j + u];               // L[j + u][j .. j + 3]
+        if (u > 0) x = fmaf(-l[0], bu.x, x);
+        if (u > 1) x = fmaf(-l[1], bu.y, x);
+        if (u > 2) x = fmaf(-l[2], bu.z, x);
+        l[u] = x * s.rs[pb][u];
+        a[j + u] = l[u];
+      }
+    }
+    if (w < 2 && j + 4 < 32) {
+      float4 bv[28];                          // all reads in flight at once
 #pragma unroll
-    for (int e = 0; e < 16; ++e)
-      X21[(32 * ib + mfma_row(e, lane)) * XS + 32 * jb + (lane & 31)] = -d[e];
+      for (int k = j + 4; k < 32; ++k) bv[k - j - 4] = buf[k];
+#pragma unroll
+      for (int k = j + 4; k < 32; ++k) {
+        const float4 v = bv[k - j - 4];
+        float x = a[k];
+        x = fmaf(-a[j], v.x, x);
+        x = fmaf(-a[j + 1], v.y, x);
+        x = fmaf(-a[j + 2], v.z, x);
+        x = fmaf(-a[j + 3], v.w, x);
+        a[k] = x;
+      }
+    }
   }
-  __syncthreads();
-  block64_out<256>(W0 + 64 * ldw, ldw, X21, true, t);
-  block64_out<256>(W0 + 64 * ldw + 64, ldw, X22, false, t);
+  return bad;
 }
 
-constexpr int kLeafLds = 3 * 64 * XS * 4 + (int)sizeof(LeafLds);
+__device__ __forceinline__ void leaf128_body(
+    const float* __restrict__ A, long long lda, float* __restrict__ W,
+    long long ldw, int k0, int* __restrict__ info, Leaf2Lds& s) {
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  float* X = s.X;
+  const float* A0 = A + (long long)k0 * lda + k0;
+  // The block's lower triangle (and the float4 overhang into the diagonal
+  // 32-blocks, masked later); zeros in the upper-right 32-blocks of the
+  // diagonal 64-blocks, the zero blocks of W_l, W_r in the join at 64.  All
+  // loads of a thread are issued before its stores.
+  {
+    float4 v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int f = t + 256 * q, row = f >> 5, c4 = (f & 31) * 4;
+      v[q] = c4 <= row ? *(const float4*)(A0 + (long long)row * lda + c4)
+                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int f = t + 256 * q, row = f >> 5, c4 = (f & 31) * 4;
+      const bool zero_blk = (row >> 5) + 1 == (c4 >> 5) && !(row & 32);
+      if (c4 <= row || zero_blk) {
+        float* x = X + row * XL + c4;
+        x[0] = v[q].x;
+        x[1] = v[q].y;
+        x[2] = v[q].z;
+        x[3] = v[q].w;
+      }
+    }
+  }
+  __syncthreads();
+  // 1. factor
+  int bad = 0;
+#pragma unroll 1
+  for (int p = 0; p < 4; ++p) {
+    const int c0 = 32 * p;
+    const int row = c0 + 64 * w + lane;             // this lane's row
+    const bool act = w < 2 && row < 128;
+    const int i = row - c0;                         // row within the panel
+    float a[32];
+#pragma unroll
+    for (int c = 0; c < 32; ++c)
+      a[c] = (act && c <= i) ? X[row * XL + c0 + c] : 0.0f;
+    const int b = panel32_2w(a, w, lane, s);
+    if (w == 0 && bad == 0 && b != 0) bad = c0 + b;
+    if (act) {
+#pragma unroll
+      for (int c = 0; c < 32; ++c)
+        X[row * XL + c0 + c] = c <= i ? a[c] : 0.0f;
+    }
+    __syncthreads();
+    if (p < 3) {
+      // A[c0 + 32:, c0 + 32:] -= L L^T on its lower 32x32 tiles
+      const int nb = 3 - p;
+      for (int tt = w; tt < nb * (nb + 1) / 2; tt += 4) {
+        int ti = 0;
+        while ((ti + 1) * (ti + 2) / 2 <= tt) ++ti;
+        const int tj = tt - ti * (ti + 1) / 2;
+        const int R0 = c0 + 32 + 32 * ti, C0 = c0 + 32 + 32 * tj;
+        const floatx16 d = mm32<XL, XL, 32, false, true>(
+            X + R0 * XL + c0, X + C0 * XL + c0, lane);
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+          X[(R0 + mfma_row(e, lane)) * XL + C0 + (lane & 31)] -= d[e];
+      }
+      __syncthreads();
+    }
+  }
+  if (t == 0 && bad != 0 && *info == 0) *info = k0 + bad;
+  // 2. W = L^-1.  Diagonal 16-blocks transposed, reciprocal pivots.
+  for (int f = t; f < 8 * 256; f += 256) {
+    const int b = f >> 8, k = (f >> 4) & 15, m = f & 15;
+    s.Lt[b][k * LT + m] = X[(16 * b + m) * XL + 16 * b + k];
+  }
+  if (t < 128) {
+    // (a non-positive pivot was reported and left in place: 1 keeps the
+    // inverse finite)
+    const float d = X[t * XL + t];
+    s.dg[t] = d > 0.0f ? __builtin_amdgcn_rcpf(d) : 1.0f;
+  }
+  __syncthreads();
+  if (w < 2) {
+    // lane: column c of block b's inverse, right-looking on the transposed
+    // copy (the float4 reads do not depend on the solve)
+    const int b = 4 * w + (lane >> 4), c = lane & 15;
+    const float* Lt = s.Lt[b];
+    const float* dv = s.dg + 16 * b;
+    float v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = (k == c) ? 1.0f : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      v[k] *= dv[k];
+#pragma unroll
+      for (int m = (k + 1) & ~3; m < 16; m += 4) {
+        const float4 lv = *(const float4*)(Lt + k * LT + m);
+        if (m > k) v[m] = fmaf(-lv.x, v[k], v[m]);
+        if (m + 1 > k) v[m + 1] = fmaf(-lv.y, v[k], v[m + 1]);
+        if (m + 2 > k) v[m + 2] = fmaf(-lv.z, v[k], v[m + 2]);
+        if (m + 3 > k) v[m + 3] = fmaf(-lv.w, v[k], v[m + 3]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) X[(16 * b + k) * XL + 16 * b + c] = v[k];
+  }
+  __syncthreads();
+  {
+    // joins at 16 (wavefront w: the 32-block w) on v_mfma_f32_16x16x4f32
+    // (lane l: A[l % 16][k], B[k][l % 16] with k = 4 kk + l / 16; register v
+    // of the result is row 4 (l / 16) + v, column l % 16)
+    const int q = lane & 15, kq = lane >> 4, o = 32 * w;
+    float* Tq = s.T + w * (16 * 17);
+    floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int k = 4 * kk + kq;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(
+          X[(o + 16 + q) * XL + o + k], X[(o + k) * XL + o + q], acc, 0, 0,
+          0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Tq[(4 * kq + v) * 17 + q] = acc[v];
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __builtin_amdgcn_wave_barrier();
+    acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int k = 4 * kk + kq;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(
+          X[(o + 16 + q) * XL + o + 16 + k], Tq[k * 17 + q], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      X[(o + 16 + 4 * kq + v) * XL + o + q] = -acc[v];
+  }
+  __syncthreads();
+  if (w < 2) {
+    // joins at 32 (wavefront w: the 64-block w)
+    const int o = 64 * w;
+    float* Th = s.T + w * (32 * 33);
+    const floatx16 d = mm32<XL, XL, 32, false, false>(
+        X + (o + 32) * XL + o, X + o * XL + o, lane);
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+      Th[mfma_row(e, lane) * 33 + (lane & 31)] = d[e];
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __builtin_amdgcn_wave_barrier();
+    const floatx16 g = mm32<XL, 33, 32, false, false>(
+        X + (o + 32) * XL + o + 32, Th, lane);
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+      X[(o + 32 + mfma_row(e, lane)) * XL + o + (lane & 31)] = -g[e];
+  }
+  __syncthreads();
+  {
+    // the join at 64: tile (ib, jb) of the 64x64 products per wavefront
+    const int ib = w >> 1, jb = w & 1;
+    const floatx16 d = mm32<XL, XL, 64, false, false>(
+        X + (64 + 32 * ib) * XL, X + 32 * jb, lane);
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+      s.T[(32 * ib + mfma_row(e, lane)) * 65 + 32 * jb + (lane & 31)] = d[e];
+    __syncthreads();
+    const floatx16 g = mm32<XL, 65, 64, false, false>(
+        X + (64 + 32 * ib) * XL + 64, s.T + 32 * jb, lane);
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+      X[(64 + 32 * ib + mfma_row(e, lane)) * XL + 32 * jb + (lane & 31)] =
+          -g[e];
+  }
+  __syncthreads();
+  // 3. the lower 32-blocks of W out (ten blocks, 2,560 float4)
+  float* W0 = W + (long long)k0 * ldw + k0;
+#pragma unroll
+  for (int q = 0; q < 10; ++q) {
+    const int f = t + 256 * q;                 // float4 slot of the lower
+    const int blk = f >> 8, e = f & 255;       // blocks, block-row major
+    int bi = 0;
+    while ((bi + 1) * (bi + 2) / 2 <= blk) ++bi;
+    const int bj = blk - bi * (bi + 1) / 2;
+    const int row = 32 * bi + (e >> 3), c4 = 32 * bj + (e & 7) * 4;
+    const float* x = X + row * XL + c4;
+    *(float4*)(W0 + (long long)row * ldw + c4) =
+        make_float4(x[0], x[1], x[2], x[3]);
+  }
+}
 
-__global__ __launch_bounds__(256) void k_leaf128(const float* __restrict__ A,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_leaf128(const float* __restrict__ A,
                                                  long long lda,
                                                  float* __restrict__ W,
                                                  long long ldw, int k0,
                                                  int* __restrict__ info) {
-  __shared__ __align__(16) char smem[kLeafLds];
-  float* X = reinterpret_cast<float*>(smem);
-  leaf128_body(A, lda, W, ldw, k0, info, X, X + 64 * XS, X + 128 * XS,
-               *reinterpret_cast<LeafLds*>(X + 192 * XS));
+  __shared__ __align__(16) Leaf2Lds s;
+  leaf128_body(A, lda, W, ldw, k0, info, s);
 }
 
 // ------------------------------------------------ blocked factorisation
