@@ -807,6 +807,22 @@ __device__ __forceinline__ void leaf128_body(
 #pragma unroll
       for (int c = 0; c < 32; ++c)
         X[row * XL + c0 + c] = c <= i ? a[c] : 0.0f;
+      if (w == 0 && lane < 32) {
+        // the panel's two diagonal 16-blocks transposed for the inverse
+        // (only the entries below the diagonal are read there), and the
+        // reciprocal pivot (a non-positive pivot was reported and left in
+        // place: 1 keeps the inverse finite)
+        float* lt = s.Lt[2 * p + (lane >> 4)] + (lane & 15);
+        if (lane < 16) {                      // static register indices
+#pragma unroll
+          for (int k = 0; k < 16; ++k) lt[k * LT] = a[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < 16; ++k) lt[k * LT] = a[16 + k];
+        }
+        const float d = X[row * XL + row];
+        s.dg[row] = d > 0.0f ? __builtin_amdgcn_rcpf(d) : 1.0f;
+      }
     }
     __syncthreads();
     if (p < 3) {
@@ -827,18 +843,8 @@ __device__ __forceinline__ void leaf128_body(
     }
   }
   if (t == 0 && bad != 0 && *info == 0) *info = k0 + bad;
-  // 2. W = L^-1.  Diagonal 16-blocks transposed, reciprocal pivots.
-  for (int f = t; f < 8 * 256; f += 256) {
-    const int b = f >> 8, k = (f >> 4) & 15, m = f & 15;
-    s.Lt[b][k * LT + m] = X[(16 * b + m) * XL + 16 * b + k];
-  }
-  if (t < 128) {
-    // (a non-positive pivot was reported and left in place: 1 keeps the
-    // inverse finite)
-    const float d = X[t * XL + t];
-    s.dg[t] = d > 0.0f ? __builtin_amdgcn_rcpf(d) : 1.0f;
-  }
-  __syncthreads();
+  // 2. W = L^-1 (the transposed diagonal 16-blocks and the reciprocal
+  // pivots were written with the panels)
   if (w < 2) {
     // lane: column c of block b's inverse, right-looking on the transposed
     // copy (the float4 reads do not depend on the solve)
@@ -912,6 +918,20 @@ __device__ __forceinline__ void leaf128_body(
       X[(o + 32 + mfma_row(e, lane)) * XL + o + (lane & 31)] = -g[e];
   }
   __syncthreads();
+  // 3. W out: its diagonal 64-blocks W_l, W_r (lower 32-blocks) now, the
+  // join at 64 straight from its product registers
+  float* W0 = W + (long long)k0 * ldw + k0;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const int f = t + 256 * q;                 // float4 slot of the six
+    const int blk = f >> 8, e = f & 255;       // lower blocks of W_l, W_r
+    const int h = blk >= 3, bb = blk - 3 * h;  // (0,0) (1,0) (1,1)
+    const int bi = 2 * h + (bb > 0), bj = 2 * h + (bb > 1);
+    const int row = 32 * bi + (e >> 3), c4 = 32 * bj + (e & 7) * 4;
+    const float* x = X + row * XL + c4;
+    *(float4*)(W0 + (long long)row * ldw + c4) =
+        make_float4(x[0], x[1], x[2], x[3]);
+  }
   {
     // the join at 64: tile (ib, jb) of the 64x64 products per wavefront
     const int ib = w >> 1, jb = w & 1;
@@ -925,23 +945,8 @@ __device__ __forceinline__ void leaf128_body(
         X + (64 + 32 * ib) * XL + 64, s.T + 32 * jb, lane);
 #pragma unroll
     for (int e = 0; e < 16; ++e)
-      X[(64 + 32 * ib + mfma_row(e, lane)) * XL + 32 * jb + (lane & 31)] =
-          -g[e];
-  }
-  __syncthreads();
-  // 3. the lower 32-blocks of W out (ten blocks, 2,560 float4)
-  float* W0 = W + (long long)k0 * ldw + k0;
-#pragma unroll
-  for (int q = 0; q < 10; ++q) {
-    const int f = t + 256 * q;                 // float4 slot of the lower
-    const int blk = f >> 8, e = f & 255;       // blocks, block-row major
-    int bi = 0;
-    while ((bi + 1) * (bi + 2) / 2 <= blk) ++bi;
-    const int bj = blk - bi * (bi + 1) / 2;
-    const int row = 32 * bi + (e >> 3), c4 = 32 * bj + (e & 7) * 4;
-    const float* x = X + row * XL + c4;
-    *(float4*)(W0 + (long long)row * ldw + c4) =
-        make_float4(x[0], x[1], x[2], x[3]);
+      W0[(long long)(64 + 32 * ib + mfma_row(e, lane)) * ldw + 32 * jb +
+         (lane & 31)] = -g[e];
   }
 }
 
