@@ -330,12 +330,20 @@ int nta_pcg64_k64(const double* K, int n, long long ldk, double lam,
  * ldk, 16-byte aligned) and X [n][16] fp64 (up to 16 columns, zero padded):
  * K's entries are exact in FP64 and the products accumulate on fp64 MFMA.
  * Y [m][16] fp64.  m, n and ldk must be multiples of 32, 16 and 4
- * (NTA_EUNSUPPORTED otherwise: pad).
+ * (NTA_EUNSUPPORTED otherwise: pad).  With a device workspace of
+ * nta_gemm_f32_f64_work_bytes(m, n) bytes the columns are split over
+ * several workgroups per row block (for short K: m = 1,024 rows are only
+ * 32 row blocks) and the parts summed in a fixed order; without it (NULL or
+ * too small) one workgroup per row block.
  * Replaces: the test-set mean jnp.tensordot(k_td, k_inv_y, ...)
  * (predict.py:696) with the FP32 test-train kernel and the FP64 solution.
  */
 int nta_gemm_f32_f64(const float* K, long long ldk, int m, int n,
-                     const double* X, double* Y, void* stream);
+                     const double* X, double* Y, void* work,
+                     long long work_bytes, void* stream);
+
+/* Workspace bytes of nta_gemm_f32_f64 (0: none used; -1: invalid sizes). */
+long long nta_gemm_f32_f64_work_bytes(int m, int n);
 
 /*
  * nta_pcg64 without a workspace argument: the iteration's vectors live in a

@@ -48,7 +48,7 @@ SIGNATURES = {
                   _LL, _P, _P, _P],
     'nta_pcg64_k64': [_P, _I, _LL, _D, _P, _P, _LL, _I, _P, _I, _P, _D, _I,
                       _P, _LL, _P, _P, _P],
-    'nta_gemm_f32_f64': [_P, _LL, _I, _I, _P, _P, _P],
+    'nta_gemm_f32_f64': [_P, _LL, _I, _I, _P, _P, _P, _LL, _P],
     'nta_pcg64_noalloc': [_P, _I, _LL, _D, _P, _P, _LL, _I, _P, _I, _P, _D,
                           _I, _P, _P, _P],
     'nta_diag_chain': [_P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P,
@@ -67,6 +67,7 @@ SIGNATURES = {
 VALUE_FNS = {
     'nta_chol_split': (_I, [_I]),
     'nta_pcg64_work_bytes': (_LL, [_I, _I]),
+    'nta_gemm_f32_f64_work_bytes': (_LL, [_I, _I]),
     'nta_relu_bwd_parts': (_LL, [_I, _I, _I]),
 }
 

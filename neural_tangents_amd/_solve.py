@@ -55,6 +55,8 @@ def kernel_times(K: torch.Tensor, X: torch.Tensor) -> torch.Tensor:
     Kp = torch.zeros((mp, npad), dtype=torch.float32, device=K.device)
     Kp[:m, :n] = K
   stream = _lib.stream_handle(K.device)
+  wb = _lib.value('nta_gemm_f32_f64_work_bytes', mp, npad)
+  work = torch.empty((max(wb, 1),), dtype=torch.uint8, device=K.device)
   cols = []
   for c0 in range(0, X.shape[1], 16):
     xc = X[:, c0:c0 + 16]
@@ -62,7 +64,7 @@ def kernel_times(K: torch.Tensor, X: torch.Tensor) -> torch.Tensor:
     Xp[:n, :xc.shape[1]] = xc
     Y = torch.empty((mp, 16), dtype=torch.float64, device=K.device)
     _lib.call('nta_gemm_f32_f64', _lib.ptr(Kp), npad, mp, npad,
-              _lib.ptr(Xp), _lib.ptr(Y), stream)
+              _lib.ptr(Xp), _lib.ptr(Y), _lib.ptr(work), wb, stream)
     cols.append(Y[:m, :xc.shape[1]])
   return torch.cat(cols, dim=1) if len(cols) > 1 else cols[0]
 

@@ -1341,7 +1341,9 @@ __device__ __forceinline__ void row_stripe_product(
 // through LDS in a fixed order.
 // mode 0: Q = (K + lam I) P, part[wg][c] = sum_rows P Q.
 // mode 2: Q = K P for a rectangular K (rows = 32 gridDim.x, n columns):
-//         neither P's rows of Q's index nor part are touched.
+//         neither P's rows of Q's index nor part are touched; with
+//         gridDim.y > 1, part y of the 16-column chunks into Q + y (32
+//         gridDim.x) RHS.
 // mode 1: D = B - (K + lam I) P; R = D, R32 = D (when R is given),
 //         part[wg][c] = sum_rows D^2.
 template <typename TK>
@@ -1358,9 +1360,14 @@ __global__ __launch_bounds__(512) void k_symv_rows(
   const int s = w & 1, qd = w >> 1;
   const int r = lane & 15, g = lane >> 4;
   const int i0 = blockIdx.x * 32 + 16 * s;
+  // mode 2 may split the columns over gridDim.y parts (partial products
+  // into Q's part blockIdx.y)
+  const int nch = n / 16, per = (nch + gridDim.y - 1) / gridDim.y;
+  const int cb = blockIdx.y * per, ce = min(nch, cb + per);
+  if (gridDim.y > 1) Q += (long long)blockIdx.y * gridDim.x * 32 * RHS;
   doublex4 acc = {0.0, 0.0, 0.0, 0.0};
   row_stripe_product(acc, K + (long long)(i0 + r) * ldk + 4 * g,
-                     P + (long long)(4 * g) * RHS + r, qd, n / 16, 4);
+                     P + (long long)(4 * g) * RHS + r, cb + qd, ce, 4);
   red[qd][s][lane] = acc;
   __syncthreads();
   if (qd == 0) {
@@ -1988,8 +1995,34 @@ extern "C" int nta_chol_f32(float* A, int n, long long lda, float* W,
   return check_launch("nta_chol_f32");
 }
 
+// Y = sum over the parts of the column-split products, in a fixed order.
+__global__ __launch_bounds__(256) void k_sum_parts(
+    const double* __restrict__ parts, int nparts, long long count,
+    double* __restrict__ Y) {
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= count) return;
+  double v = parts[e];
+  for (int q = 1; q < nparts; ++q) v += parts[(long long)q * count + e];
+  Y[e] = v;
+}
+
+// Column parts of nta_gemm_f32_f64: enough row-block x part workgroups to
+// cover the CUs twice (one part when the rows alone do).
+static int gemm_f32_f64_parts(int m, int n) {
+  int parts = (512 + m / 32 - 1) / (m / 32);
+  parts = std::min(parts, std::max(1, n / 256));    // >= 16 chunks a part
+  return std::max(1, std::min(parts, 16));
+}
+
+extern "C" long long nta_gemm_f32_f64_work_bytes(int m, int n) {
+  if (m <= 0 || n <= 0 || m % 32 != 0 || n % 16 != 0) return -1;
+  const int parts = gemm_f32_f64_parts(m, n);
+  return parts > 1 ? (long long)parts * m * RHS * sizeof(double) : 0;
+}
+
 extern "C" int nta_gemm_f32_f64(const float* K, long long ldk, int m, int n,
-                                const double* X, double* Y, void* stream) {
+                                const double* X, double* Y, void* work,
+                                long long work_bytes, void* stream) {
   if (m <= 0 || n <= 0 || ldk < n || !K || !X || !Y)
     return set_error(NTA_EINVAL, "nta_gemm_f32_f64: bad m=%d n=%d ldk=%lld "
                      "or null pointer", m, n, ldk);
@@ -1998,9 +2031,22 @@ extern "C" int nta_gemm_f32_f64(const float* K, long long ldk, int m, int n,
     return set_error(NTA_EUNSUPPORTED, "nta_gemm_f32_f64: m=%d, n=%d and "
                      "ldk=%lld must be multiples of 32, 16 and 4 and K "
                      "16-byte aligned (pad the matrix)", m, n, ldk);
-  hipLaunchKernelGGL(k_symv_rows<float>, dim3(m / 32), dim3(512), 0,
-                     (hipStream_t)stream, K, ldk, n, 0.0, 2, X, Y, nullptr,
-                     nullptr, nullptr, nullptr, nullptr);
+  hipStream_t st = (hipStream_t)stream;
+  const int parts = gemm_f32_f64_parts(m, n);
+  const long long need = parts > 1 ? (long long)parts * m * RHS * 8 : 0;
+  if (parts == 1 || !work || work_bytes < need) {
+    hipLaunchKernelGGL(k_symv_rows<float>, dim3(m / 32), dim3(512), 0, st,
+                       K, ldk, n, 0.0, 2, X, Y, nullptr, nullptr, nullptr,
+                       nullptr, nullptr);
+    return check_launch("nta_gemm_f32_f64");
+  }
+  double* pw = (double*)work;
+  hipLaunchKernelGGL(k_symv_rows<float>, dim3(m / 32, parts), dim3(512), 0,
+                     st, K, ldk, n, 0.0, 2, X, pw, nullptr, nullptr, nullptr,
+                     nullptr, nullptr);
+  const long long count = (long long)m * RHS;
+  hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((count + 255) / 256)),
+                     dim3(256), 0, st, pw, parts, count, Y);
   return check_launch("nta_gemm_f32_f64");
 }
 

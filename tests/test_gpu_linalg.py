@@ -159,7 +159,8 @@ def test_invalid_arguments_raise():
 
 
 @pytest.mark.parametrize('m,n,c', [(1024, 8192, 10), (96, 640, 16),
-                                   (50, 300, 3), (64, 256, 20)])
+                                   (50, 300, 3), (64, 256, 20),
+                                   (32, 4000, 5), (8192, 1024, 10)])
 def test_kernel_times_matches_fp64_product(m, n, c):
   """nta_gemm_f32_f64 (through _solve.kernel_times, padded when m, n are
   not multiples of 32 and 16, in 16-column chunks) against the FP64 product
@@ -181,4 +182,5 @@ def test_kernel_times_rejects_unpadded_shapes():
   Y = torch.empty((48, 16), dtype=torch.float64, device=DEV)
   with pytest.raises(NotImplementedError):
     _lib.call('nta_gemm_f32_f64', _lib.ptr(K), 64, 48, 64, _lib.ptr(X),
-              _lib.ptr(Y), _lib.stream_handle(DEV))
+              _lib.ptr(Y), None, 0, _lib.stream_handle(DEV))
+  assert _lib.value('nta_gemm_f32_f64_work_bytes', 48, 64) == -1
