@@ -1547,30 +1547,24 @@ __global__ __launch_bounds__(256) void k_pcg_p(
   }
 }
 
-// alpha = (r . z) / (p . A p) per column from the product's partials.
-__global__ __launch_bounds__(256) void k_pcg_alpha(
-    const double* __restrict__ pq, int npq, PcgState* __restrict__ st) {
-  if (st->done) return;
-  __shared__ double scratch[256], den[RHS];
-  block_slot_sums(pq, npq, scratch, den);
-  const int t = threadIdx.x;
-  if (t < RHS) st->alpha[t] = den[t] != 0.0 ? st->rz[t] / den[t] : 0.0;
-}
-
-// X += alpha P; R -= alpha Q; R32 = R; partials per workgroup of
-// rr = sum_rows R^2, rzx = sum_rows R . Z (Z: the previous preconditioned
-// residual, for the Polak-Ribiere beta) and xx = sum_rows X^2.
+// alpha = (r . z) / (p . A p) per column (every workgroup sums the
+// product's npq partials pq itself, in a fixed order); X += alpha P;
+// R -= alpha Q; R32 = R; partials per workgroup of rr = sum_rows R^2,
+// rzx = sum_rows R . Z (Z: the previous preconditioned residual, for the
+// Polak-Ribiere beta) and xx = sum_rows X^2.
 __global__ __launch_bounds__(256) void k_pcg_xr(
-    int n, const PcgState* __restrict__ st, const double* __restrict__ P,
-    const double* __restrict__ Q, const float* __restrict__ Z,
-    double* __restrict__ X, double* __restrict__ R, float* __restrict__ R32,
-    double* __restrict__ rr, double* __restrict__ rzx,
-    double* __restrict__ xx) {
+    int n, PcgState* __restrict__ st, const double* __restrict__ pq,
+    int npq, const double* __restrict__ P, const double* __restrict__ Q,
+    const float* __restrict__ Z, double* __restrict__ X,
+    double* __restrict__ R, float* __restrict__ R32, double* __restrict__ rr,
+    double* __restrict__ rzx, double* __restrict__ xx) {
   if (st->done) return;
-  __shared__ double red[256];
+  __shared__ double red[256], den[RHS];
+  block_slot_sums(pq, npq, red, den);
   const int t = threadIdx.x;
   const int c = t % RHS;
-  const double alpha = st->alpha[c];
+  const double alpha = den[c] != 0.0 ? st->rz[c] / den[c] : 0.0;
+  if (blockIdx.x == 0 && t < RHS) st->alpha[t] = alpha;
   const long long e0 = (long long)blockIdx.x * VROWS * RHS;
   double s = 0.0, x = 0.0, q = 0.0;
   for (int e = t; e < VROWS * RHS; e += 256) {
@@ -1888,11 +1882,9 @@ static int pcg_run(const TK* K, int n, long long ldk, double lam,
       hipLaunchKernelGGL(k_symv_rows<TK>, dim3(nsym), dim3(512), 0, st, K,
                          ldk, n, lam, 0, p_new, w.q, nullptr, nullptr,
                          nullptr, w.pq, done);
-      hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(256), 0, st, w.pq, nsym,
-                         w.st);
       hipLaunchKernelGGL(k_pcg_xr, dim3(nvec), dim3(256), 0, st, n, w.st,
-                         p_new, w.q, w.z32, w.x, w.r, w.r32, w.rr, w.rzx,
-                         w.xx);
+                         w.pq, nsym, p_new, w.q, w.z32, w.x, w.r, w.r32,
+                         w.rr, w.rzx, w.xx);
       precond(W, WT, ldw, n, n1, w.r32, w.z32, w.r, w.rz, w.zz, done, w,
               st);
       hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(256), 0, st, w.rr, nvec,
