@@ -1473,7 +1473,9 @@ struct PcgState {
   int done;            // 1: converged, every later kernel is a no-op
   int iters;           // iteration that set done (or the last checked)
   int restart;         // 1: the next direction is z (beta = 0)
-  int skip;            // 1: the recomputed residual alone meets tol
+  int skip;            // 1: skip the rest of a residual recomputation
+  int notdone;         // 1 - done (gate of the speculative verification)
+  int pad;
 };
 
 // Per-column sums of nslot partials [slot][RHS] by a 256-thread block:
@@ -1525,6 +1527,7 @@ __global__ __launch_bounds__(256) void k_pcg_init(
   }
   if (t == 0) {
     st->done = 0;
+    st->notdone = 1;
     st->iters = 0;
     st->restart = 1;
     st->skip = 0;
@@ -1632,7 +1635,8 @@ __global__ __launch_bounds__(256) void k_pcg_check(
     int all = 1;
     for (int c = 0; c < RHS; ++c) all &= ok[c];
     st->done = all;
-    st->iters = it;
+    st->notdone = !all;
+    if (rzx) st->iters = it;     // a recomputation keeps the iteration count
     st->restart = rzx ? 0 : 1;
   }
 }
@@ -1640,10 +1644,16 @@ __global__ __launch_bounds__(256) void k_pcg_check(
 // The test on a recomputed residual before its preconditioning: when every
 // column's relative residual is <= tol the solve has converged (skip = 1,
 // done = 1) and the preconditioner and k_pcg_check of the recomputation
-// return at once; otherwise skip = 0 and they run.
+// return at once; otherwise skip = 0 and they run.  With a gate that is
+// set (the recurrence has not converged: the speculative recomputation did
+// not run), skip = 1 and nothing else changes.
 __global__ __launch_bounds__(256) void k_pcg_rcheck(
-    const double* __restrict__ rr, int nrr, int nrhs, double tol, int it,
-    PcgState* __restrict__ st) {
+    const double* __restrict__ rr, int nrr, int nrhs, double tol,
+    const int* __restrict__ gate, PcgState* __restrict__ st) {
+  if (gate && *gate) {
+    if (threadIdx.x == 0) st->skip = 1;
+    return;
+  }
   __shared__ double scratch[256], srr[RHS];
   __shared__ int ok[RHS];
   block_slot_sums(rr, nrr, scratch, srr);
@@ -1662,7 +1672,7 @@ __global__ __launch_bounds__(256) void k_pcg_rcheck(
     st->skip = all;
     if (all) {
       st->done = 1;
-      st->iters = it;
+      st->notdone = 0;
       st->restart = 1;
     }
   }
@@ -1812,10 +1822,13 @@ static void precond(const float* W, const float* WT, long long ld, int n,
 // The refinement loop.  Iterations are launched in batches without host
 // synchronisation: every kernel of an iteration returns at once when the
 // device-side state says the solve has converged, so a batch that passes
-// convergence costs a few empty launches.  After each batch the host reads
-// the state (one synchronisation); on convergence, and every second batch,
-// the residual is recomputed as b - A x (the recurrence drifts on
-// ill-conditioned systems) and tested again, and the direction restarts.
+// convergence costs a few empty launches.  Each batch is followed by a
+// recomputation of the residual as b - A x and its test (the recurrence
+// drifts on ill-conditioned systems) that the device runs only if the
+// recurrence converged, and by the solution's copy out; then the host reads
+// the state (one synchronisation per batch).  Every second batch without
+// convergence the residual is recomputed unconditionally (a checkpoint:
+// drift, stall test), and the direction restarts.
 // Stop: every column's relative residual ||r|| / ||b|| or estimated
 // relative error ||M^-1 r|| / ||x|| <= tol (M^-1 r is the error x* - x to
 // the accuracy of the FP32 factor), or no progress of the recomputed
@@ -1849,18 +1862,25 @@ static int pcg_run(const TK* K, int n, long long ldk, double lam,
     return m;
   };
   // r <- b - A x (FP64), z <- M^-1 r, then the test on the recomputed
-  // residual (restarts the direction).
-  auto recompute = [&](int it) {
+  // residual (restarts the direction).  Speculative: the device runs it only
+  // if the recurrence has converged (gate notdone).
+  auto recompute = [&](bool speculative) {
+    const int* gate = speculative ? &w.st->notdone : nullptr;
     hipLaunchKernelGGL(k_symv_rows<TK>, dim3(nsym), dim3(512), 0, st, K,
                        ldk, n, lam, 1, w.x, nullptr, w.b, w.r, w.r32, w.rr,
-                       nullptr);
+                       gate);
     hipLaunchKernelGGL(k_pcg_rcheck, dim3(1), dim3(256), 0, st, w.rr, nsym,
-                       nrhs, tol, it, w.st);
+                       nrhs, tol, gate, w.st);
     precond(W, WT, ldw, n, n1, w.r32, w.z32, w.r, w.rz, w.zz, &w.st->skip,
             w, st);
     hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(256), 0, st, w.rr, nsym,
                        nullptr, 0, w.xx, nvec, w.rz, w.zz, nrz, nrhs, tol,
-                       it, w.st);
+                       0, w.st);
+  };
+  auto copy_out = [&]() {
+    hipLaunchKernelGGL(k_pcg_io, dim3(nvec), dim3(256), 0, st, n, nrhs,
+                       nullptr, nullptr, nullptr, nullptr, w.x, nullptr,
+                       nullptr, Xout);
   };
   hipLaunchKernelGGL(k_pcg_io, dim3(nvec), dim3(256), 0, st, n, nrhs, Bin,
                      w.b, w.r, w.r32, w.x, w.p0, w.bb, nullptr);
@@ -1870,7 +1890,7 @@ static int pcg_run(const TK* K, int n, long long ldk, double lam,
   double* pbuf[2] = {w.p0, w.p1};
   double best = INFINITY;
   int stall = 0, it = 0, batches = 0, rc = 0;
-  bool converged = false;
+  bool converged = false, copied = false;
   while (it < max_iter && !converged) {
     const int nb = max_iter - it < kPcgBatch ? max_iter - it : kPcgBatch;
     for (int k = 0; k < nb; ++k) {
@@ -1892,17 +1912,26 @@ static int pcg_run(const TK* K, int n, long long ldk, double lam,
                          it, w.st);
     }
     ++batches;
+    // If the recurrence converged in this batch, the device confirms it on
+    // the recomputed residual and copies the solution out before the one
+    // synchronisation of the batch.
+    recompute(true);
+    copy_out();
     rc = read_state();
     if (rc) return rc;
-    if (!hs.done && batches % 2 != 0) continue;
-    // Converged on the recurrence, or a periodic checkpoint: test the
-    // recomputed residual (the iteration count is the one that converged).
-    const int at = hs.done ? hs.iters : it;
-    recompute(at);
+    if (hs.done) {                    // confirmed (iteration that converged)
+      it = hs.iters;
+      converged = copied = true;
+      break;
+    }
+    if (batches % 2 != 0) continue;
+    // a periodic checkpoint on the recomputed residual (the recurrence
+    // drifts on ill-conditioned systems), also the stall test
+    recompute(false);
     rc = read_state();
     if (rc) return rc;
     if (hs.done) {
-      it = at;
+      it = hs.iters;
       converged = true;
       break;
     }
@@ -1913,18 +1942,15 @@ static int pcg_run(const TK* K, int n, long long ldk, double lam,
     } else if (++stall >= 5) {
       break;
     }
-    it = at;
   }
   if (!converged) {
     // The returned x's recomputed residual (the test above may not have
     // run for the last iterations).
-    recompute(it);
+    recompute(false);
     rc = read_state();
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_pcg_io, dim3(nvec), dim3(256), 0, st, n, nrhs,
-                     nullptr, nullptr, nullptr, nullptr, w.x, nullptr,
-                     nullptr, Xout);
+  if (!copied) copy_out();
   if (iters) *iters = it < max_iter ? it : max_iter;
   if (resid) *resid = worst(hs.rel);
   return check_launch("nta_pcg64");
