@@ -301,8 +301,10 @@ int nta_chol_precond(const float* W, const float* WT, int n, long long ldw,
  * and *resid the largest relative residual of the returned X, recomputed
  * from scratch (host pointers, may be NULL).  Iterations run in batches of
  * 4 without host synchronisation (the kernels of an iteration after
- * convergence return at once); the stream is synchronised once per batch
- * and per recomputed residual.
+ * convergence return at once), each batch followed by the confirmation on
+ * the recomputed residual, which the device runs only once the recurrence
+ * has converged; the stream is synchronised once per batch and per
+ * checkpoint.
  * Replaces: cho_solve (predict.py:1254) of the regularised train kernel.
  */
 int nta_pcg64(const float* K, int n, long long ldk, double lam,
