@@ -183,32 +183,45 @@ int nta_head(const float* nngp, const float* ntk, int n1, int n2, int H,
              void* stream);
 
 /*
- * Adjoint of one [Conv] -> ReLU -> [AvgPool p x p] stage, per pre-pool
- * element of a tile (no counterpart in the reference, which differentiates
- * its XLA graph with jax.grad; this is the reverse of stax/elementwise.py:
- * 444-455 with the _sqrt/_arctan2 JVP rules of 1278-1310 and of the pool
+ * Adjoint of one [Conv] -> ReLU -> [AvgPool p x p] stage over a tile of
+ * pairs (no counterpart in the reference, which differentiates its XLA graph
+ * with jax.grad; this is the reverse of stax/elementwise.py:444-455 with the
+ * _sqrt/_arctan2 JVP rules of 1278-1310 and of the pool
  * stax/linear.py:3551-3571).  aN, aT: stencil means of the stage's nngp and
  * ntk inputs [T][S][S][S][S] (aT = NULL: first stage, ntk input 0); gN, gT:
- * pooled cotangents [T][S/p]^4.  Writes the cotangents of aN (gaN, already
- * times w_var), aT, of the normalisers per element (gqa, gqb; to be summed
- * over the partner axes) and, into gwb (device doubles, 2 per workgroup:
- * nta_relu_bwd_parts(n1, n2, S) pairs), the workgroups' partial sums of the
- * d/dw_var and d/db_var terms (the caller adds the slots: a fixed-order,
- * deterministic reduction).  Pairs (a, b) with a + self_off == b (and every
- * pair of a self tile, n2 == 0) are pairs of identical images: their spatial
- * diagonal is treated as rho = 1 exactly.  Pass self_off = INT_MAX for
- * none.  act_k1, act_k0: the ABRelu constants, as in nta_first_stage.
+ * pooled cotangents [T][S/p]^4 (NULL: zero).  Writes
+ *   gaN: the cotangent of aN (already times w_var), in the stage layout,
+ *     or with gram_scale != 0 (cross tiles only) times gram_scale in the
+ *     patch-Gram layout [n1][S][S][n2][S][S] = (a, h, w, b, h', w'): the
+ *     first stage's Gram adjoint is then two plain GEMMs;
+ *   gaT: the cotangent of aT (NULL: not wanted);
+ *   gqa, gqb [T][S][S]: the cotangents of the normalisers q1 (at (h, w),
+ *     summed over (h', w')) and q2 (at (h', w'), summed over (h, w));
+ *   gwb: device doubles, 2 per workgroup (nta_relu_bwd_parts(n1, n2, S)
+ *     pairs), the workgroups' partial sums of the d/dw_var and d/db_var
+ *     terms (the caller adds the slots: a fixed-order, deterministic
+ *     reduction).
+ * work: device workspace of nta_relu_bwd_work_bytes(n1, n2, S) bytes.
+ * Pairs (a, b) with a + self_off == b (and every pair of a self tile,
+ * n2 == 0) are pairs of identical images: their spatial diagonal is treated
+ * as rho = 1 exactly.  Pass self_off = INT_MAX for none.  act_k1, act_k0:
+ * the ABRelu constants, as in nta_first_stage.  T < 2^24.
  */
 int nta_relu_bwd(const float* aN, const float* aT, int n1, int n2, int S,
                  int pool, int self_off, float w_var, float b_var,
                  float act_k1, float act_k0, const float* q1,
                  const float* q2, const float* gN, const float* gT,
-                 float* gaN, float* gaT, float* gqa, float* gqb, double* gwb,
+                 float gram_scale, float* gaN, float* gaT, float* gqa,
+                 float* gqb, double* gwb, void* work, long long work_bytes,
                  void* stream);
 
 /* Partial-sum slots (pairs of doubles) of an nta_relu_bwd call on a tile of
  * n1 x n2 pairs (n2 == 0: n1 self pairs) of side S; -1 for bad sizes. */
 long long nta_relu_bwd_parts(int n1, int n2, int S);
+
+/* Workspace bytes of an nta_relu_bwd call (S^3 floats per pair for sides 8,
+ * 16 and 32, 2 S^4 otherwise); -1 for bad sizes. */
+long long nta_relu_bwd_work_bytes(int n1, int n2, int S);
 
 /*
  * Diagonal-spatial chain: a network of nstage [Conv fh x fw, stride 1]

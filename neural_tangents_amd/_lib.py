@@ -60,7 +60,7 @@ SIGNATURES = {
                               _P],
     'nta_qnorm': [_P, _LL, _P, _P],
     'nta_relu_bwd': [_P, _P, _I, _I, _I, _I, _I, _F, _F, _F, _F, _P, _P, _P,
-                     _P, _P, _P, _P, _P, _P, _P],
+                     _P, _F, _P, _P, _P, _P, _P, _P, _LL, _P],
 }
 
 # Entry points that return a value instead of a status.
@@ -69,6 +69,7 @@ VALUE_FNS = {
     'nta_pcg64_work_bytes': (_LL, [_I, _I]),
     'nta_gemm_f32_f64_work_bytes': (_LL, [_I, _I]),
     'nta_relu_bwd_parts': (_LL, [_I, _I, _I]),
+    'nta_relu_bwd_work_bytes': (_LL, [_I, _I, _I]),
 }
 
 

@@ -69,29 +69,30 @@ NO_SELF = 2 ** 31 - 1
 
 
 def _relu_bwd(aN, aT, n1, n2, S, pool, w_var, b_var, q1, q2, gN, gT,
-              stream, self_off=NO_SELF, act=(1.0, 0.0)):
+              stream, self_off=NO_SELF, act=(1.0, 0.0), gram_scale=0.0):
   """Adjoint of one stage's ReLU (+ pool).  Returns (gaN, gaT, gqa, gqb,
-  gw, gb) with gw, gb the d/dw_var, d/db_var contributions as 0-d FP64
-  device tensors (summed from the kernel's per-workgroup partials, with no
-  host synchronisation)."""
+  gw, gb): gqa, gqb [T, S, S] the normaliser cotangents summed over the
+  partner's positions, gw, gb the d/dw_var, d/db_var contributions as 0-d
+  FP64 device tensors (summed from the kernel's per-workgroup partials,
+  with no host synchronisation).  gram_scale != 0: gaN in the patch-Gram
+  layout [n1, S, S, n2, S, S] times gram_scale (cross tiles)."""
   T = aN.shape[0]
   gaN = torch.empty_like(aN)
   gaT = torch.empty_like(aN) if aT is not None else None
-  gqa = torch.empty_like(aN)
-  gqb = torch.empty_like(aN)
+  gqa = torch.empty((T, S, S), dtype=torch.float32, device=aN.device)
+  gqb = torch.empty_like(gqa)
   parts = _lib.value('nta_relu_bwd_parts', n1, n2, S)
   gwb = torch.empty((max(parts, 1), 2), dtype=torch.float64,
                     device=aN.device)
+  nwork = _lib.value('nta_relu_bwd_work_bytes', n1, n2, S)
+  work = torch.empty((max(nwork, 1),), dtype=torch.uint8, device=aN.device)
   _lib.call('nta_relu_bwd', _lib.ptr(aN), _lib.ptr(aT), n1, n2, S, pool,
             self_off, w_var, b_var, act[0], act[1], _lib.ptr(q1),
-            _lib.ptr(q2), _lib.ptr(gN),
-            _lib.ptr(gT), _lib.ptr(gaN), _lib.ptr(gaT), _lib.ptr(gqa),
-            _lib.ptr(gqb), _lib.ptr(gwb), stream)
-  v = lambda t: t.reshape(T, S, S, S, S)   # (h, h', w, w')
-  gqa_img = v(gqa).sum(dim=(2, 4))          # over (h', w') -> [T, S, S]
-  gqb_img = v(gqb).sum(dim=(1, 3))          # over (h, w)
+            _lib.ptr(q2), _lib.ptr(gN), _lib.ptr(gT), gram_scale,
+            _lib.ptr(gaN), _lib.ptr(gaT), _lib.ptr(gqa), _lib.ptr(gqb),
+            _lib.ptr(gwb), _lib.ptr(work), work.numel(), stream)
   sums = gwb[:parts].sum(dim=0)
-  return gaN, gaT, gqa_img, gqb_img, sums[0], sums[1]
+  return gaN, gaT, gqa, gqb, sums[0], sums[1]
 
 
 def _check_plan(plan):
@@ -184,24 +185,27 @@ def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=64):
           aN = _first_gram(x1[i0:i0 + n1], xb_all[j0:j0 + n2], n1, n2, S,
                            C, fh, fw, stream)
           aT = None
+        first_stage = si == 0
         gaN, gaT, gqa, gqb, dw, db = _relu_bwd(
             aN, aT, n1, n2, S, pool, w_var, b_var, qa[si], qb[si], gN, gT,
-            stream, self_off=(i0 - j0) if sym else NO_SELF, act=s.act)
+            stream, self_off=(i0 - j0) if sym else NO_SELF, act=s.act,
+            gram_scale=1.0 / (fh * fw * C) if first_stage else 0.0)
         gwv[si] += dw
         gbv[si] += db
         gq1[si][i0:i0 + n1] += gqa.reshape(n1, n2, S, S).sum(1)
         gq2[si][j0:j0 + n2] += gqb.reshape(n1, n2, S, S).sum(0)
-        if si > 0:
+        if not first_stage:
           gN = _stencil(gaN, T, S, fh_s, fw_s, valid, stream)
           gT = _stencil(gaT, T, S, fh_s, fw_s, valid, stream)
         else:
-          # aN = G / (fh fw C), G = P1 P2^T per pair
-          M = _to_hw_pairs(gaN, S) / (fh * fw * C)     # [T, HW, HW]
-          M = M.reshape(n1, n2, S * S, S * S)
-          P1 = P_all1[i0:i0 + n1]                      # [n1, HW, K]
-          P2 = P_all2[j0:j0 + n2]                      # [n2, HW, K]
-          gP1 = torch.einsum('ijab,jbk->iak', M, P2)
-          gP2 = torch.einsum('ijab,iak->jbk', M, P1)
+          # aN = G / (fh fw C), G = P1 P2^T per pair; gaN holds
+          # M = dL/dG as [(a, h, w), (b, h', w')]
+          HW = S * S
+          M = gaN.reshape(n1 * HW, n2 * HW)
+          P1 = P_all1[i0:i0 + n1].reshape(n1 * HW, -1)
+          P2 = P_all2[j0:j0 + n2].reshape(n2 * HW, -1)
+          gP1 = torch.mm(M, P2).reshape(n1, HW, -1)
+          gP2 = torch.mm(M.t(), P1).reshape(n2, HW, -1)
           gx1[i0:i0 + n1] += _fold(gP1, (n1,) + tuple(x1.shape[1:]), fh, fw)
           gx2[j0:j0 + n2] += _fold(gP2, (n2,) + tuple(x1.shape[1:]), fh, fw)
   # Self chains: push the normaliser cotangents through each image's chain.
