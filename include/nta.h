@@ -205,7 +205,8 @@ int nta_head(const float* nngp, const float* ntk, int n1, int n2, int H,
  * Pairs (a, b) with a + self_off == b (and every pair of a self tile,
  * n2 == 0) are pairs of identical images: their spatial diagonal is treated
  * as rho = 1 exactly.  Pass self_off = INT_MAX for none.  act_k1, act_k0:
- * the ABRelu constants, as in nta_first_stage.  T < 2^24.
+ * the ABRelu constants, as in nta_first_stage.  T < 2^24 (and n2 S^3 < 2^31
+ * with the patch-Gram layout).
  */
 int nta_relu_bwd(const float* aN, const float* aT, int n1, int n2, int S,
                  int pool, int self_off, float w_var, float b_var,

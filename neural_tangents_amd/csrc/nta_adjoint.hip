@@ -36,7 +36,32 @@
 
 namespace nta {
 
+// atan2(y, x) for y >= 0, in [0, pi]: the smaller of |x|, y over the
+// larger, t in [0, 1], is reduced to |u| <= tan(pi / 8) by
+// u = (t - 1) / (t + 1) above tan(pi / 8), and atan(u) = u + u z P(z)
+// (z = u^2, P of degree 3) -- the single-precision arctangent of the Cephes
+// library (S. L. Moshier) -- before the octant is restored.  Absolute error
+// below 3e-7 (hardware reciprocals).  (0, 0) gives 0.
+__device__ __forceinline__ float atan2_pos(float y, float x) {
+  const float ax = fabsf(x);
+  const float mx = fmaxf(ax, y), mn = fminf(ax, y);
+  const float t = mx > 0.0f ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
+  const bool big = t > 0.41421356237f;
+  const float u = big ? (t - 1.0f) * __builtin_amdgcn_rcpf(t + 1.0f) : t;
+  const float z = u * u;
+  const float pz = fmaf(fmaf(fmaf(8.05374449538e-2f, z, -1.38776856032e-1f),
+                             z, 1.99777106478e-1f),
+                        z, -3.33329491539e-1f);
+  float r = fmaf(pz * z, u, u);
+  if (big) r += 0.5f * 0.5f * kPi;
+  if (y > ax) r = 0.5f * kPi - r;
+  if (x < 0.0f) r = kPi - r;
+  return r;
+}
+
 // Cotangents of one element.  rqa, rqb: k1 / (4 pi q) (0 where q <= 0).
+// Square root and reciprocals on the hardware instructions (~1 ulp): the
+// cotangents feed gradients, not the kernels themselves.
 struct BwdElem {
   float gct;     // gc + gt
   float gt;      // gTD ds
@@ -52,12 +77,13 @@ __device__ __forceinline__ BwdElem bwd_elem(float an, float at, bool has_t,
   const float c = fmaf(w_var, an, b_var);
   const float t = has_t ? fmaf(w_var, at, c) : c;
   const float d = prod_minus_sq(qa, qb, c);
-  const float s = same ? 0.0f : sqrtf(fmaxf(d, 0.0f));
-  const float theta = (s == 0.0f && c == 0.0f) ? 0.5f * kPi : atan2f(s, c);
+  const float s = same ? 0.0f : fsqrt(fmaxf(d, 0.0f));
+  const float th = atan2_pos(s, c);                    // no branch
+  const float theta = (s == 0.0f && c == 0.0f) ? 0.5f * kPi : th;
   const float ds = fmaf(k1, (kPi - theta) * kInv2Pi, k0);   // dot_sigma'
   // d(ds')/dc = k1 / (2 pi s) and d(ds')/dq = -k1 c / (4 pi s q) vanish at
   // s = 0; dJ'/dc = ds', dJ'/dq = k1 s / (4 pi q)
-  const float rs = s > 0.0f ? 1.0f / s : 0.0f;
+  const float rs = s > 0.0f ? __builtin_amdgcn_rcpf(s) : 0.0f;
   const float gtt = gtd * t;
   const float gt = gtd * ds;
   const float gc = fmaf(gtt, k1 * kInv2Pi * rs, gj * ds);
@@ -71,7 +97,7 @@ __device__ __forceinline__ BwdElem bwd_elem(float an, float at, bool has_t,
 }
 
 __device__ __forceinline__ float bwd_rq(float q, float k1) {
-  return q > 0.0f ? k1 * (0.25f / kPi) / q : 0.0f;
+  return q > 0.0f ? k1 * (0.25f / kPi) * __builtin_amdgcn_rcpf(q) : 0.0f;
 }
 
 // Sum of a workgroup's (sw, sb) in a fixed order (wave shuffles, then the
@@ -106,7 +132,10 @@ __device__ __forceinline__ void bwd_wb_sums(double sw, double sb,
 // sweep in registers, then over w' by shuffles) into gqa[p][h][w].
 // gram_scale != 0 (cross tiles): gaN goes to the patch-Gram layout
 // [n1][S][S][n2][S][S] = (a, h, w, b, h', w'), times gram_scale.
-template <int S>
+// HT, HN, HG: aT, gN, gT given (else zero).  With the presence of the
+// inputs fixed at compile time the loop has no branches, so the loads of a
+// thread's elements are all in flight together.
+template <int S, bool HT, bool HN, bool HG>
 __global__ __launch_bounds__(256) void k_relu_bwd_rows(
     const float* __restrict__ aN, const float* __restrict__ aT, int n2,
     int pool_sh, int self_off, float w_var, float b_var, float k1, float k0,
@@ -130,31 +159,39 @@ __global__ __launch_bounds__(256) void k_relu_bwd_rows(
   const int i = self ? p : p / n2;
   const int j = self ? p : p - i * n2;
   const bool self_pair = self || (long long)i + self_off == (long long)j;
-  const bool has_t = aT != nullptr;
   const int w2 = t & (S - 1);
   const int r = t / S;
   const int w1g = r % kW1;
   const int hoff = r / kW1;
   const int So = S >> pool_sh;
   const float inv_pool = pool_sh ? 1.0f / 16.0f : 1.0f;
-  constexpr long long S2 = (long long)S * S;
-  const long long base = (long long)p * S2 * S2 + (long long)h1 * S2 * S;
-  const long long obase = (long long)p * So * So * So * So;
+  constexpr int S2 = S * S;
+  // the workgroup's slabs (uniform bases; 32-bit offsets inside them)
+  const long long slab = (long long)p * S2 * S2 + (long long)h1 * S2 * S;
+  const float* aNs = aN + slab;
+  const float* aTs = HT ? aT + slab : nullptr;
+  float* gaTs = gaT ? gaT + slab : nullptr;
+  const long long oslab = (long long)p * So * So * So * So +
+                          (long long)(h1 >> pool_sh) * So * So * So;
+  const float* gNs = HN ? gN + oslab : nullptr;
+  const float* gTs = HG ? gT + oslab : nullptr;
   const bool gram = gram_scale != 0.0f;
   const float wg = gram ? w_var * gram_scale : w_var;
+  // gaN: stage layout as the inputs, or (a, h, w, b, h', w')
+  float* gaNs = gram ? gaN + ((long long)i * S + h1) * S * n2 * S2 +
+                           (long long)j * S2
+                     : gaN + slab;
+  const int gw = gram ? n2 * S2 : S;      // gaN stride of w
+  const int gh = gram ? S : S2;           // and of h'
   const float* qa_row = q1 + ((long long)i * S + h1) * S;
   const float* qb_img = (self ? q1 : q2) + (long long)j * S2;
   float qa[kU], rqa[kU], acc_a[kU];
-  long long gbase[kU];
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
     const int w1 = w1g + kW1 * u;
     qa[u] = qa_row[w1];
     rqa[u] = bwd_rq(qa[u], k1);
     acc_a[u] = 0.0f;
-    gbase[u] = gram ? (((long long)i * S + h1) * S + w1) * n2 * S2 +
-                          (long long)j * S2 + w2
-                    : base + (long long)w1 * S + w2;
   }
   double sw = 0.0, sb = 0.0;
   constexpr int kUnroll = kU >= 4 ? 1 : 2;        // S = 32: 4 elements
@@ -163,27 +200,33 @@ __global__ __launch_bounds__(256) void k_relu_bwd_rows(
     const int h2 = hoff + kH * it;
     const float qb = qb_img[h2 * S + w2];
     const float rqb = bwd_rq(qb, k1);
-    const long long orow =
-        obase + ((long long)(h1 >> pool_sh) * So + (h2 >> pool_sh)) * So * So +
-        (w2 >> pool_sh);
+    const unsigned orow = (unsigned)((h2 >> pool_sh) * So * So +
+                                     (w2 >> pool_sh));
+    float an[kU], at[kU], gj[kU], gtd[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {              // all loads first
+      const int w1 = w1g + kW1 * u;
+      const unsigned e = (unsigned)((h2 * S + w1) * S + w2);
+      const unsigned o = orow + (unsigned)((w1 >> pool_sh) * So);
+      an[u] = aNs[e];
+      at[u] = HT ? aTs[e] : 0.0f;
+      gj[u] = HN ? gNs[o] * inv_pool : 0.0f;
+      gtd[u] = HG ? gTs[o] * inv_pool : 0.0f;
+    }
     float gbs = 0.0f;
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int w1 = w1g + kW1 * u;
-      const long long e = base + ((long long)h2 * S + w1) * S + w2;
-      const float an = aN[e];
-      const float at = has_t ? aT[e] : 0.0f;
-      const long long o = orow + (long long)(w1 >> pool_sh) * So;
-      const float gj = gN ? gN[o] * inv_pool : 0.0f;
-      const float gtd = gT ? gT[o] * inv_pool : 0.0f;
+      const unsigned e = (unsigned)((h2 * S + w1) * S + w2);
       const bool same = self_pair && h1 == h2 && w1 == w2;
-      const BwdElem el = bwd_elem(an, at, has_t, qa[u], qb, rqa[u], rqb, same,
-                                  gj, gtd, w_var, b_var, k1, k0);
-      gaN[gbase[u] + (long long)h2 * S * (gram ? 1 : S)] = wg * el.gct;
-      if (gaT) gaT[e] = has_t ? w_var * el.gt : 0.0f;
+      const BwdElem el = bwd_elem(an[u], at[u], HT, qa[u], qb, rqa[u], rqb,
+                                  same, gj[u], gtd[u], w_var, b_var, k1, k0);
+      gaNs[(unsigned)(w1 * gw + h2 * gh + w2)] = wg * el.gct;
+      if (HT) gaTs[e] = w_var * el.gt;
+      else if (gaTs) gaTs[e] = 0.0f;
       acc_a[u] += el.gq_a;
       gbs += el.gq_b;
-      sw += (double)(el.gct * an + el.gt * at);
+      sw += (double)(el.gct * an[u] + el.gt * at[u]);
       sb += (double)el.gct;
     }
     // over the other rows of the wave (w = w1g + ..., same h', w')
@@ -375,18 +418,39 @@ extern "C" int nta_relu_bwd(const float* aN, const float* aT, int n1, int n2,
   const long long T = bwd_pairs(n1, n2);
   if (T >= (1LL << 24))                 // grid.x * 256 work-items < 2^32
     return set_error(NTA_EINVAL, "nta_relu_bwd: %lld pairs in one tile", T);
+  if (gram_scale != 0.0f && (long long)n2 * S * S * S >= (1LL << 31))
+    return set_error(NTA_EINVAL, "nta_relu_bwd: patch-Gram layout with "
+                     "n2 * S^3 >= 2^31");
   hipStream_t st = (hipStream_t)stream;
   float* part = (float*)work;
   if (bwd_rows(S)) {
     const int ps = pool == 2 ? 1 : 0;
     const dim3 grid((unsigned)T, (unsigned)S);
-#define NTA_BWD_ROWS(SS)                                                    \
-    hipLaunchKernelGGL(k_relu_bwd_rows<SS>, grid, dim3(256), 0, st, aN, aT, \
-                       n2, ps, self_off, w_var, b_var, act_k1, act_k0, q1,  \
-                       q2, gN, gT, gram_scale, gaN, gaT, gqa, part, gwb)
-    if (S == 32) NTA_BWD_ROWS(32);
-    else if (S == 16) NTA_BWD_ROWS(16);
-    else NTA_BWD_ROWS(8);
+#define NTA_BWD_ROWS(SS, HT, HN, HG)                                        \
+    hipLaunchKernelGGL((k_relu_bwd_rows<SS, HT, HN, HG>), grid, dim3(256),  \
+                       0, st, aN, aT, n2, ps, self_off, w_var, b_var,       \
+                       act_k1, act_k0, q1, q2, gN, gT, gram_scale, gaN,     \
+                       gaT, gqa, part, gwb)
+#define NTA_BWD_ROWS_G(SS, HT)                                              \
+    if (gN && gT) NTA_BWD_ROWS(SS, HT, true, true);                         \
+    else if (gN) NTA_BWD_ROWS(SS, HT, true, false);                         \
+    else if (gT) NTA_BWD_ROWS(SS, HT, false, true);                         \
+    else NTA_BWD_ROWS(SS, HT, false, false)
+#define NTA_BWD_ROWS_S(SS)                                                  \
+    if (aT) {                                                               \
+      NTA_BWD_ROWS_G(SS, true);                                             \
+    } else {                                                                \
+      NTA_BWD_ROWS_G(SS, false);                                            \
+    }
+    if (S == 32) {
+      NTA_BWD_ROWS_S(32);
+    } else if (S == 16) {
+      NTA_BWD_ROWS_S(16);
+    } else {
+      NTA_BWD_ROWS_S(8);
+    }
+#undef NTA_BWD_ROWS_S
+#undef NTA_BWD_ROWS_G
 #undef NTA_BWD_ROWS
     hipLaunchKernelGGL(k_bwd_sum_h, dim3(grid_for(T * S * S)), dim3(256), 0,
                        st, part, T, S, gqb);

@@ -1047,6 +1047,28 @@ static int chol_split_of(int n) {
   return m == 1 ? n : off[m / 2];
 }
 
+// Transpose of the lower blocks of W's first ncols columns (32x32 tiles on
+// or below the diagonal, rows < n) into WT (WT's tiles strictly below the
+// diagonal are left untouched; the preconditioner never reads them).  Grid:
+// (ncols / 32, n / 32) tiles, rounded up.
+__global__ __launch_bounds__(256) void k_transpose_lower(
+    const float* __restrict__ W, long long ldw, int n, int ncols,
+    float* __restrict__ WT, long long ldt) {
+  __shared__ float tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;   // W tile (by, bx)
+  if (bx > by || bx >= ncols) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    const int i = by + r, j = bx + tx;
+    tile[r][tx] = (i < n && j < n) ? W[(long long)i * ldw + j] : 0.0f;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = bx + r, j = by + tx;
+    if (i < n && j < n) WT[(long long)i * ldt + j] = tile[tx][r];
+  }
+}
+
 constexpr int kOuter = 512;
 
 static void chol_invert_half(float* W, long long ldw,
@@ -1059,8 +1081,8 @@ static void chol_invert_half(float* W, long long ldw,
 // rows above split, which the rest of the factorisation no longer touches)
 // beside the rest of the factorisation; kEvInvEnd marks its end.
 static int chol_factor(float* A, long long lda, float* W, long long ldw,
-                       int n, int* info, hipStream_t st, CholStreams& cs,
-                       int split) {
+                       float* WT, int n, int* info, hipStream_t st,
+                       CholStreams& cs, int split) {
   hipStream_t sd = cs.side;
   hipEvent_t* ev = cs.ev;
   hipError_t e = hipSuccess;
@@ -1117,6 +1139,10 @@ static int chol_factor(float* A, long long lda, float* W, long long ldw,
       wait(cs.inv, kEvSplit);
       if (e == hipSuccess) {
         chol_invert_half(W, ldw, off, 0, ke, A, lda, cs.inv, m);
+        // W11 and L21 (columns [0, split)) are final: their transposes too
+        const dim3 grid((split + 31) / 32, (n + 31) / 32);
+        hipLaunchKernelGGL(k_transpose_lower, grid, dim3(256), 0, cs.inv, W,
+                           ldw, n, split, WT, ldw);
         record(kEvInvEnd, cs.inv);
       }
     }
@@ -1202,27 +1228,6 @@ static void chol_invert_half(float* W, long long ldw,
     if (nj == 0) break;
     launch_gemm(tj[0], nj > 1 ? &tj[1] : nullptr, st);
     launch_gemm(wj[0], nj > 1 ? &wj[1] : nullptr, st);
-  }
-}
-
-// Transpose of the lower blocks of W (tiles on or below the diagonal) into
-// WT (WT's tiles strictly below the diagonal are left untouched; the
-// preconditioner never reads them).
-__global__ __launch_bounds__(256) void k_transpose_lower(
-    const float* __restrict__ W, long long ldw, int n, float* __restrict__ WT,
-    long long ldt) {
-  __shared__ float tile[32][33];
-  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;   // W tile (by, bx)
-  if (bx > by) return;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int r = ty; r < 32; r += 8) {
-    const int i = by + r, j = bx + tx;
-    tile[r][tx] = (i < n && j < n) ? W[(long long)i * ldw + j] : 0.0f;
-  }
-  __syncthreads();
-  for (int r = ty; r < 32; r += 8) {
-    const int i = bx + r, j = by + tx;
-    if (i < n && j < n) WT[(long long)i * ldt + j] = tile[tx][r];
   }
 }
 
@@ -1334,11 +1339,11 @@ __device__ __forceinline__ void row_stripe_product(
   }
 }
 
-// (K + lam I) P in FP64 for 32 rows per workgroup of 8 waves: wave w
-// computes rows [i0 + 16 (w & 1), +16) over the 16-column chunks
-// c = w >> 1 (mod 4) on v_mfma_f64_16x16x4f64 (K read in fp32 -- exact in
-// fp64 -- or in fp64); the four partial tiles of a row stripe are added
-// through LDS in a fixed order.
+// (K + lam I) P in FP64 for 32 rows per workgroup of 2 kSvSets waves:
+// wave w computes rows [i0 + 16 (w & 1), +16) over the 16-column chunks
+// c = w >> 1 (mod kSvSets) on v_mfma_f64_16x16x4f64 (K read in fp32 --
+// exact in fp64 -- or in fp64); the kSvSets partial tiles of a row stripe
+// are added through LDS in a fixed order.
 // mode 0: Q = (K + lam I) P, part[wg][c] = sum_rows P Q.
 // mode 2: Q = K P for a rectangular K (rows = 32 gridDim.x, n columns):
 //         neither P's rows of Q's index nor part are touched; with
@@ -1346,15 +1351,17 @@ __device__ __forceinline__ void row_stripe_product(
 //         gridDim.x) RHS.
 // mode 1: D = B - (K + lam I) P; R = D, R32 = D (when R is given),
 //         part[wg][c] = sum_rows D^2.
+constexpr int kSvSets = 4;
+constexpr int kSvThreads = 128 * kSvSets;
 template <typename TK>
-__global__ __launch_bounds__(512) void k_symv_rows(
+__global__ __launch_bounds__(kSvThreads) void k_symv_rows(
     const TK* __restrict__ K, long long ldk, int n, double lam, int mode,
     const double* __restrict__ P, double* __restrict__ Q,
     const double* __restrict__ B, double* __restrict__ R,
     float* __restrict__ R32, double* __restrict__ part,
     const int* __restrict__ done) {
   if (done && *done) return;           // converged: the rest is a no-op
-  __shared__ doublex4 red[4][2][64];
+  __shared__ doublex4 red[kSvSets][2][64];
   __shared__ double dots[2][RHS];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int s = w & 1, qd = w >> 1;
@@ -1367,12 +1374,11 @@ __global__ __launch_bounds__(512) void k_symv_rows(
   if (gridDim.y > 1) Q += (long long)blockIdx.y * gridDim.x * 32 * RHS;
   doublex4 acc = {0.0, 0.0, 0.0, 0.0};
   row_stripe_product(acc, K + (long long)(i0 + r) * ldk + 4 * g,
-                     P + (long long)(4 * g) * RHS + r, cb + qd, ce, 4);
+                     P + (long long)(4 * g) * RHS + r, cb + qd, ce,
+                     kSvSets);
   red[qd][s][lane] = acc;
   __syncthreads();
   if (qd == 0) {
-    const doublex4 a1 = red[1][s][lane], a2 = red[2][s][lane],
-                   a3 = red[3][s][lane];
     double d = 0.0;
     // v_mfma_f64_16x16x4f64 output: register v of lane l is row
     // 4 v + l / 16, column l % 16 (the f32 16x16x4 layout is row
@@ -1380,7 +1386,9 @@ __global__ __launch_bounds__(512) void k_symv_rows(
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const long long e = (long long)(i0 + 4 * v + g) * RHS + r;
-      const double kp = ((acc[v] + a1[v]) + a2[v]) + a3[v];
+      double kp = acc[v];
+#pragma unroll
+      for (int q = 1; q < kSvSets; ++q) kp += red[q][s][lane][v];
       if (mode == 2) {
         Q[e] = kp;
         continue;
@@ -1409,12 +1417,12 @@ __global__ __launch_bounds__(512) void k_symv_rows(
 }
 
 // Out = sa (A X) + sv V (fp32) for a row-major block A (m x k) on
-// v_mfma_f32_16x16x4f32: a workgroup of 8 waves owns 16 output rows, the
+// v_mfma_f32_16x16x4f32: a workgroup of 16 waves owns 16 output rows, the
 // waves take the 16-column chunks round-robin and are added through LDS in
 // a fixed order.  tri: A is lower (1) or upper (2) triangular.  With r64
 // given, rzp[wg][c] = sum_rows r64 * Out and zzp[wg][c] = sum_rows Out^2
 // (fp64).
-constexpr int GV_WAVES = 8;
+constexpr int GV_WAVES = 16;
 __global__ __launch_bounds__(64 * GV_WAVES) void k_gemv_rows(
     const float* __restrict__ A, long long lda, int m, int k, int tri,
     const float* __restrict__ X, float* __restrict__ Out, float sa,
@@ -1866,9 +1874,9 @@ static int pcg_run(const TK* K, int n, long long ldk, double lam,
   // if the recurrence has converged (gate notdone).
   auto recompute = [&](bool speculative) {
     const int* gate = speculative ? &w.st->notdone : nullptr;
-    hipLaunchKernelGGL(k_symv_rows<TK>, dim3(nsym), dim3(512), 0, st, K,
-                       ldk, n, lam, 1, w.x, nullptr, w.b, w.r, w.r32, w.rr,
-                       gate);
+    hipLaunchKernelGGL(k_symv_rows<TK>, dim3(nsym), dim3(kSvThreads), 0, st,
+                       K, ldk, n, lam, 1, w.x, nullptr, w.b, w.r, w.r32,
+                       w.rr, gate);
     hipLaunchKernelGGL(k_pcg_rcheck, dim3(1), dim3(256), 0, st, w.rr, nsym,
                        nrhs, tol, gate, w.st);
     precond(W, WT, ldw, n, n1, w.r32, w.z32, w.r, w.rz, w.zz, &w.st->skip,
@@ -1899,8 +1907,8 @@ static int pcg_run(const TK* K, int n, long long ldk, double lam,
       double* p_new = pbuf[it & 1];
       hipLaunchKernelGGL(k_pcg_p, dim3(nvec), dim3(256), 0, st, n, w.z32,
                          p_old, p_new, w.st);
-      hipLaunchKernelGGL(k_symv_rows<TK>, dim3(nsym), dim3(512), 0, st, K,
-                         ldk, n, lam, 0, p_new, w.q, nullptr, nullptr,
+      hipLaunchKernelGGL(k_symv_rows<TK>, dim3(nsym), dim3(kSvThreads), 0,
+                         st, K, ldk, n, lam, 0, p_new, w.q, nullptr, nullptr,
                          nullptr, w.pq, done);
       hipLaunchKernelGGL(k_pcg_xr, dim3(nvec), dim3(256), 0, st, n, w.st,
                          w.pq, nsym, p_new, w.q, w.z32, w.x, w.r, w.r32,
@@ -1987,7 +1995,7 @@ extern "C" int nta_chol_f32(float* A, int n, long long lda, float* W,
   // with by then), and the bottom half after the factorisation, its scratch
   // in rows [n1, n).
   const bool overlap = m > 1 && mh % (kOuter / 128) == 0;
-  int rc = chol_factor(A, lda, W, ldw, n, info, st, *cs,
+  int rc = chol_factor(A, lda, W, ldw, WT, n, info, st, *cs,
                        overlap ? off[mh] : -1);
   if (rc) return rc;
   if (overlap) {
@@ -2007,9 +2015,13 @@ extern "C" int nta_chol_f32(float* A, int n, long long lda, float* W,
       chol_invert_half(W, ldw, off, mh, m, A, lda, st, m);
     }
   }
-  dim3 grid((n + 31) / 32, (n + 31) / 32);
-  hipLaunchKernelGGL(k_transpose_lower, grid, dim3(256), 0, st, W, ldw, n,
-                     WT, ldw);
+  // WT = W^T: with the overlap, columns [0, n1) were transposed on the
+  // inversion stream; W22 remains
+  const int t0 = overlap ? off[mh] : 0, nt = n - t0;
+  const dim3 grid((nt + 31) / 32, (nt + 31) / 32);
+  hipLaunchKernelGGL(k_transpose_lower, grid, dim3(256), 0, st,
+                     W + (long long)t0 * ldw + t0, ldw, nt, nt,
+                     WT + (long long)t0 * ldw + t0, ldw);
   return check_launch("nta_chol_f32");
 }
 
@@ -2053,15 +2065,15 @@ extern "C" int nta_gemm_f32_f64(const float* K, long long ldk, int m, int n,
   const int parts = gemm_f32_f64_parts(m, n);
   const long long need = parts > 1 ? (long long)parts * m * RHS * 8 : 0;
   if (parts == 1 || !work || work_bytes < need) {
-    hipLaunchKernelGGL(k_symv_rows<float>, dim3(m / 32), dim3(512), 0, st,
-                       K, ldk, n, 0.0, 2, X, Y, nullptr, nullptr, nullptr,
-                       nullptr, nullptr);
+    hipLaunchKernelGGL(k_symv_rows<float>, dim3(m / 32), dim3(kSvThreads), 0,
+                       st, K, ldk, n, 0.0, 2, X, Y, nullptr, nullptr,
+                       nullptr, nullptr, nullptr);
     return check_launch("nta_gemm_f32_f64");
   }
   double* pw = (double*)work;
-  hipLaunchKernelGGL(k_symv_rows<float>, dim3(m / 32, parts), dim3(512), 0,
-                     st, K, ldk, n, 0.0, 2, X, pw, nullptr, nullptr, nullptr,
-                     nullptr, nullptr);
+  hipLaunchKernelGGL(k_symv_rows<float>, dim3(m / 32, parts),
+                     dim3(kSvThreads), 0, st, K, ldk, n, 0.0, 2, X, pw,
+                     nullptr, nullptr, nullptr, nullptr, nullptr);
   const long long count = (long long)m * RHS;
   hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((count + 255) / 256)),
                      dim3(256), 0, st, pw, parts, count, Y);
