@@ -26,6 +26,8 @@ steps.  Extra keys, all measured after the timed steps:
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
        [--no-krr] [--no-m10] [--no-grad]
+       python bench.py --krr [--krr-reps R]      (the solve alone)
+       python bench.py --grad [--grad-targets N] (the gradients alone)
 """
 import argparse
 import json
@@ -297,18 +299,18 @@ def m10_result(steps):
           'kernels': table}
 
 
-def grad_result():
+def grad_result(n_target=2048):
   """Differentiation form: loss 0.5 mean((K_ts (K_ss + lambda I)^-1 Y_s -
   Y_t)^2) of the Myrtle-5 32x32 NTK with 100 support and 2,048 target
-  images, gradients with respect to the support images, W_std and b_std
-  (one forward + backward, wall time)."""
+  images (n_target: fewer, for a profile), gradients with respect to the
+  support images, W_std and b_std (one forward + backward, wall time)."""
   import torch
   from neural_tangents_amd import grad
   from neural_tangents_amd.data import blurred_images, one_hot_targets
   dev = torch.device('cuda', 0)
-  xs, xt = blurred_images([100, 2048], 32, seed=5)
+  xs, xt = blurred_images([100, n_target], 32, seed=5)
   ys = torch.from_numpy(one_hot_targets(100, seed=6)).to(dev)
-  yt = torch.from_numpy(one_hot_targets(2048, seed=7)).to(dev)
+  yt = torch.from_numpy(one_hot_targets(n_target, seed=7)).to(dev)
   builder = grad.myrtle_builder((2, 1, 1))
 
   def run(ns, nt):
@@ -326,11 +328,11 @@ def grad_result():
   run(4, 8)                                    # warm-up
   torch.cuda.synchronize()
   t0 = time.perf_counter()
-  loss = run(100, 2048)
+  loss = run(100, n_target)
   dt = time.perf_counter() - t0
   log(f'grad: {dt:.1f} s')
-  pairs = 100 * 2048 + 100 * 101 // 2
-  return {'workload': 'grad_myrtle5_32x32_support100_target2048',
+  pairs = 100 * n_target + 100 * 101 // 2
+  return {'workload': f'grad_myrtle5_32x32_support100_target{n_target}',
           'seconds': dt, 'loss': loss, 'kernel_pairs': pairs,
           'pairs_per_s_fwd_bwd': pairs / dt}
 
@@ -439,6 +441,9 @@ def main():
   ap.add_argument('--krr-test', type=int, default=1024)
   ap.add_argument('--krr-reps', type=int, default=3)
   ap.add_argument('--krr-tol', type=float, default=1e-7)
+  ap.add_argument('--grad', action='store_true',
+                  help='only the differentiation benchmark')
+  ap.add_argument('--grad-targets', type=int, default=2048)
   args = ap.parse_args()
   if args.gpus != 1:
     print('bench.py runs on one GPU only: this engine has no multi-GPU path '
@@ -451,6 +456,12 @@ def main():
     out.update({'n_gpus': 1, 'higher_is_better': True,
                 'dtype': 'fp32+fp64',
                 'data': 'synthetic (blurred Gaussian images, seed 0)'})
+    print(json.dumps(out), flush=True)
+    return
+  if args.grad:
+    out = grad_result(args.grad_targets)
+    out.update({'n_gpus': 1, 'dtype': 'fp32 (FP64 loss and solve)',
+                'data': 'synthetic (blurred Gaussian images, seed 5)'})
     print(json.dumps(out), flush=True)
     return
 
