@@ -274,9 +274,11 @@ int nta_chol_split(int n);
  * trailing updates overlap the next panel on a second stream of the
  * library's (events order it against `stream`, which the call returns on).
  * Each half of the factor is then inverted by recursive doubling.  W
- * (n x n) must be zero on entry.  On return,
- * with L L^T = A and n1 = nta_chol_split(n):
- *   W[0:n1, 0:n1] = L11^-1, W[n1:n, n1:n] = L22^-1 (lower, zeros above),
+ * (n x n) need not be initialised.  On return, with L L^T = A and
+ * n1 = nta_chol_split(n):
+ *   W[0:n1, 0:n1] = L11^-1, W[n1:n, n1:n] = L22^-1 (lower; zeros above the
+ *     diagonal inside each diagonal 64x64 block, the rest of the upper
+ *     triangles left as it was),
  *   W[n1:n, 0:n1] = L21,
  * the blocks from which the preconditioner (L L^T)^-1 is applied, and WT
  * (n x n, leading dimension ldw) holds W^T on and above its diagonal.

@@ -83,7 +83,7 @@ class MixedCholesky:
   def _factor(self, shift):
     A = self._K32.clone()
     A.diagonal().add_(shift)
-    self.W = torch.zeros_like(A)
+    self.W = torch.empty_like(A)        # the factorisation writes what is read
     self.WT = torch.empty_like(A)
     self._info.zero_()
     _lib.call('nta_chol_f32', _lib.ptr(A), self.npad, self.npad,

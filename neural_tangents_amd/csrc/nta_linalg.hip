@@ -629,9 +629,9 @@ __device__ __forceinline__ void block64_out(float* __restrict__ G,
 }
 
 // Factors and inverts the 64x64 diagonal block of A at (k0, k0) (lower
-// triangle read) into W = L^-1 at (k0, k0) of W (lower; the upper-right
-// 32x32 block of W is left untouched).  *info receives the 1-based global
-// index of the first non-positive pivot.
+// triangle read) into W = L^-1 at (k0, k0) of W (lower, with zeros in the
+// upper-right 32x32 block).  *info receives the 1-based global index of the
+// first non-positive pivot.
 __global__ __launch_bounds__(64) void k_leaf64(const float* __restrict__ A,
                                                long long lda,
                                                float* __restrict__ W,
@@ -645,6 +645,11 @@ __global__ __launch_bounds__(64) void k_leaf64(const float* __restrict__ A,
   const int bad = chol_inv64(X, t, s);
   if (t == 0 && bad != 0 && *info == 0) *info = k0 + bad;
   block64_out<64>(W + (long long)k0 * ldw + k0, ldw, X, false, t);
+  // zeros in the upper-right 32-block (triangular products read whole
+  // 64-tiles)
+  for (int f = t; f < 32 * 8; f += 64)
+    *(float4*)(W + (long long)(k0 + (f >> 3)) * ldw + k0 + 32 + (f & 7) * 4) =
+        make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // ------------------------------------------------ 128x128 leaf
@@ -664,9 +669,9 @@ __global__ __launch_bounds__(64) void k_leaf64(const float* __restrict__ A,
 //      the four wavefronts), each product on MFMA from LDS.
 // Only the pivot chains of step 1 are serial; the inverse, on one
 // wavefront inside each half when the halves were inverted on the way,
-// now runs after the factor on all four.  Writes the lower 32-blocks of
-// W = L^-1 at (k0, k0) (its upper-right 32-blocks are left untouched); A is
-// only read.
+// now runs after the factor on all four.  Writes W = L^-1 at (k0, k0): its
+// lower 32-blocks and zeros in the upper-right 32-blocks of its two diagonal
+// 64-blocks (the upper-right 64-block is left untouched); A is only read.
 constexpr int XL = 129;                     // LDS row stride of the block
 constexpr int LT = 20;                      // 16-byte aligned 16-blocks
 struct Leaf2Lds {
@@ -918,19 +923,28 @@ __device__ __forceinline__ void leaf128_body(
       X[(o + 32 + mfma_row(e, lane)) * XL + o + (lane & 31)] = -g[e];
   }
   __syncthreads();
-  // 3. W out: its diagonal 64-blocks W_l, W_r (lower 32-blocks) now, the
-  // join at 64 straight from its product registers
+  // 3. W out: its diagonal 64-blocks W_l, W_r now (the lower 32-blocks,
+  // and zeros in the upper-right ones: triangular products read whole
+  // 64-tiles), the join at 64 straight from its product registers
   float* W0 = W + (long long)k0 * ldw + k0;
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    const int f = t + 256 * q;                 // float4 slot of the six
-    const int blk = f >> 8, e = f & 255;       // lower blocks of W_l, W_r
-    const int h = blk >= 3, bb = blk - 3 * h;  // (0,0) (1,0) (1,1)
-    const int bi = 2 * h + (bb > 0), bj = 2 * h + (bb > 1);
+  for (int q = 0; q < 8; ++q) {
+    const int f = t + 256 * q;                 // float4 slot; block q
+    const int blk = f >> 8, e = f & 255;
+    int bi, bj;
+    if (blk < 6) {                             // lower blocks of W_l, W_r
+      const int h = blk >= 3, bb = blk - 3 * h;  // (0,0) (1,0) (1,1)
+      bi = 2 * h + (bb > 0);
+      bj = 2 * h + (bb > 1);
+    } else {                                   // (0,1) of W_l, W_r
+      bi = 2 * (blk - 6);
+      bj = bi + 1;
+    }
     const int row = 32 * bi + (e >> 3), c4 = 32 * bj + (e & 7) * 4;
     const float* x = X + row * XL + c4;
     *(float4*)(W0 + (long long)row * ldw + c4) =
-        make_float4(x[0], x[1], x[2], x[3]);
+        blk < 6 ? make_float4(x[0], x[1], x[2], x[3])
+                : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
   {
     // the join at 64: tile (ib, jb) of the 64x64 products per wavefront

@@ -30,7 +30,7 @@ def _spd(n, seed, lam):
 def _factor(K32, lam):
   n = K32.shape[0]
   A = (K32 + lam * torch.eye(n)).to(DEV)
-  W = torch.zeros_like(A)
+  W = torch.full_like(A, float('nan'))    # written where it is read
   WT = torch.empty_like(A)
   info = torch.zeros((1,), dtype=torch.int32, device=DEV)
   _lib.call('nta_chol_f32', _lib.ptr(A), n, n, _lib.ptr(W), n,
