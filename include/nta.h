@@ -268,6 +268,17 @@ int nta_diag_chain(const float* x1, const float* x2, int n1, int n2, int sym,
 int nta_chol_split(int n);
 
 /*
+ * A = K + shift I on the 64x64 tiles on and below the diagonal of an n x n
+ * fp32 matrix (the tiles above are not written): the input nta_chol_f32
+ * factors, at half the bytes of a full copy.  n a multiple of 64; K, A
+ * 16-byte aligned, ldk, lda multiples of 4.
+ * Replaces: _add_diagonal_regularizer (predict.py:1224-1232) on the copy
+ * that cho_factor (predict.py:1244) consumes.
+ */
+int nta_copy_lower_shift(const float* K, long long ldk, int n, float shift,
+                         float* A, long long lda, void* stream);
+
+/*
  * FP32 Cholesky of A (lower triangle read; A is overwritten with
  * intermediates): right-looking, blocked on 128 columns (leaves factored
  * and inverted in LDS) and 512-column panels, with fp32 MFMA products; the

@@ -43,6 +43,7 @@ SIGNATURES = {
     'nta_head': [_P, _P, _I, _I, _I, _I, _I, _D, _D, _D, _D, _D, _P, _P, _LL,
                  _P],
     'nta_chol_f32': [_P, _I, _LL, _P, _LL, _P, _P, _P],
+    'nta_copy_lower_shift': [_P, _LL, _I, _F, _P, _LL, _P],
     'nta_chol_precond': [_P, _P, _I, _LL, _I, _P, _I, _P, _P, _LL, _P],
     'nta_pcg64': [_P, _I, _LL, _D, _P, _P, _LL, _I, _P, _I, _P, _D, _I, _P,
                   _LL, _P, _P, _P],

@@ -3,7 +3,8 @@ FP64 residuals, preconditioned conjugate gradients.
 
 For ``A = K + lambda I`` (``predict.py:1224-1258``):
 
-1. ``nta_chol_f32`` factors ``fp32(K) + shift I`` in FP32 (right-looking
+1. ``nta_copy_lower_shift`` forms ``fp32(K) + shift I`` (lower tiles) and
+   ``nta_chol_f32`` factors it in FP32 (right-looking
    blocked Cholesky on fp32 MFMA, 128-column leaves inverted in LDS) and
    inverts each half of the factor; the preconditioner ``(L L^T)^-1`` is
    applied from the blocks it leaves (``nta_chol_precond``).
@@ -46,7 +47,8 @@ class MixedCholesky:
     self.npad = npad
     self.n1 = _lib.value('nta_chol_split', npad)
     self.k64 = K.dtype == torch.float64
-    if npad == n and K.dtype == torch.float32 and K.is_contiguous():
+    if (npad == n and K.dtype == torch.float32 and K.is_contiguous() and
+        K.data_ptr() % 16 == 0):
       K32 = K
     else:
       K32 = torch.zeros((npad, npad), dtype=torch.float32, device=self.device)
@@ -81,8 +83,11 @@ class MixedCholesky:
     self._work = {}
 
   def _factor(self, shift):
-    A = self._K32.clone()
-    A.diagonal().add_(shift)
+    # fp32(K) + shift I, lower tiles only (all the factorisation reads)
+    A = torch.empty_like(self._K32)
+    _lib.call('nta_copy_lower_shift', _lib.ptr(self._K32), self.npad,
+              self.npad, shift, _lib.ptr(A), self.npad,
+              _lib.stream_handle(self.device))
     self.W = torch.empty_like(A)        # the factorisation writes what is read
     self.WT = torch.empty_like(A)
     self._info.zero_()

@@ -1978,6 +1978,30 @@ static int pcg_run(const TK* K, int n, long long ldk, double lam,
   return check_launch("nta_pcg64");
 }
 
+// A = K + shift I on the 64x64 tiles on and below the diagonal (the tiles
+// above are not touched: the factorisation reads only the lower triangle).
+// Grid (n / 64, n / 64); one tile per workgroup of 256 threads.
+__global__ __launch_bounds__(256) void k_copy_lower_shift(
+    const float* __restrict__ K, long long ldk, int n, float shift,
+    float* __restrict__ A, long long lda) {
+  const int bx = blockIdx.x * 64, by = blockIdx.y * 64;   // tile (by, bx)
+  if (bx > by) return;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int f = t + 256 * q, row = by + (f >> 4), c = bx + (f & 15) * 4;
+    float4 v = *(const float4*)(K + (long long)row * ldk + c);
+    if (bx == by) {
+      const int d = row - c;                  // diagonal inside this float4
+      if (d == 0) v.x += shift;
+      if (d == 1) v.y += shift;
+      if (d == 2) v.z += shift;
+      if (d == 3) v.w += shift;
+    }
+    *(float4*)(A + (long long)row * lda + c) = v;
+  }
+}
+
 }  // namespace nta
 
 using namespace nta;
@@ -1985,6 +2009,23 @@ using namespace nta;
 extern "C" int nta_chol_split(int n) {
   if (n <= 0 || n % 64 != 0) return 0;
   return chol_split_of(n);
+}
+
+extern "C" int nta_copy_lower_shift(const float* K, long long ldk, int n,
+                                    float shift, float* A, long long lda,
+                                    void* stream) {
+  if (n <= 0 || ldk < n || lda < n || !K || !A)
+    return set_error(NTA_EINVAL, "nta_copy_lower_shift: bad n=%d ldk=%lld "
+                     "lda=%lld or null pointer", n, ldk, lda);
+  if (n % 64 != 0 || ldk % 4 != 0 || lda % 4 != 0 ||
+      ((uintptr_t)K & 15) != 0 || ((uintptr_t)A & 15) != 0)
+    return set_error(NTA_EUNSUPPORTED, "nta_copy_lower_shift: n=%d, ldk and "
+                     "lda must be multiples of 64, 4 and 4 and the matrices "
+                     "16-byte aligned", n);
+  const dim3 grid(n / 64, n / 64);
+  hipLaunchKernelGGL(k_copy_lower_shift, grid, dim3(256), 0,
+                     (hipStream_t)stream, K, ldk, n, shift, A, lda);
+  return check_launch("nta_copy_lower_shift");
 }
 
 extern "C" int nta_chol_f32(float* A, int n, long long lda, float* W,

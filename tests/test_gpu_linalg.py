@@ -188,3 +188,25 @@ def test_kernel_times_rejects_unpadded_shapes():
     _lib.call('nta_gemm_f32_f64', _lib.ptr(K), 64, 48, 64, _lib.ptr(X),
               _lib.ptr(Y), None, 0, _lib.stream_handle(DEV))
   assert _lib.value('nta_gemm_f32_f64_work_bytes', 48, 64) == -1
+
+
+def test_copy_lower_shift_writes_lower_tiles_only():
+  """nta_copy_lower_shift: K + shift I on the 64x64 tiles on and below the
+  diagonal, the tiles above untouched; the solver factors this copy."""
+  n = 320
+  K = torch.randn(n, n, generator=torch.Generator().manual_seed(4)).to(DEV)
+  A = torch.full_like(K, 7.0)
+  _lib.call('nta_copy_lower_shift', _lib.ptr(K), n, n, 0.25, _lib.ptr(A), n,
+            _lib.stream_handle(DEV))
+  torch.cuda.synchronize()
+  t = torch.arange(n, device=DEV) // 64
+  lower = t[:, None] >= t[None, :]
+  want = K + 0.25 * torch.eye(n, device=DEV)
+  assert torch.equal(A[lower], want[lower])
+  assert bool((A[~lower] == 7.0).all())
+  with pytest.raises(NotImplementedError):
+    _lib.call('nta_copy_lower_shift', _lib.ptr(K), n, 300, 0.0, _lib.ptr(A),
+              n, _lib.stream_handle(DEV))
+  with pytest.raises(ValueError):
+    _lib.call('nta_copy_lower_shift', None, n, n, 0.0, _lib.ptr(A), n,
+              _lib.stream_handle(DEV))
