@@ -324,9 +324,10 @@ int nta_chol_precond(const float* W, const float* WT, int n, long long ldw,
  * x to the factor's accuracy) is <= tol -- tested on the recurrence
  * residual and confirmed on the residual recomputed as B - (K + lam I) X --
  * or the recomputed residual stops improving over 5 checkpoints (every 8
- * iterations), or max_iter is reached.  *iters receives the iteration count
- * and *resid the largest relative residual of the returned X, recomputed
- * from scratch (host pointers, may be NULL).  Iterations run in batches of
+ * iterations) or is NaN (a broken-down factor), or max_iter is reached.
+ * *iters receives the iteration count and *resid the largest relative
+ * residual of the returned X, recomputed from scratch (NaN if any column's
+ * is; host pointers, may be NULL).  Iterations run in batches of
  * 4 without host synchronisation (the kernels of an iteration after
  * convergence return at once), each batch followed by the confirmation on
  * the recomputed residual, which the device runs only once the recurrence
@@ -355,23 +356,26 @@ int nta_pcg64_k64(const double* K, int n, long long ldk, double lam,
                   double* resid, void* stream);
 
 /*
- * Y = K X in FP64 for an fp32 m x n matrix K (row-major, leading dimension
- * ldk, 16-byte aligned) and X [n][16] fp64 (up to 16 columns, zero padded):
- * K's entries are exact in FP64 and the products accumulate on fp64 MFMA.
- * Y [m][16] fp64.  m, n and ldk must be multiples of 32, 16 and 4
+ * Y[:, :ncols] = K X[:, :ncols] in FP64 for an fp32 m x n matrix K
+ * (row-major, leading dimension ldk, 16-byte aligned) and X [n][ldx] fp64
+ * (1 <= ncols <= 16 columns read): K's entries are exact in FP64 and the
+ * products accumulate on fp64 MFMA.  Y [m][ldy] fp64 (the first ncols
+ * columns written).  m, n and ldk must be multiples of 32, 16 and 4
  * (NTA_EUNSUPPORTED otherwise: pad).  With a device workspace of
- * nta_gemm_f32_f64_work_bytes(m, n) bytes the columns are split over
+ * nta_gemm_f32_f64_work_bytes(m, n) bytes the columns of K are split over
  * several workgroups per row block (for short K: m = 1,024 rows are only
- * 32 row blocks) and the parts summed in a fixed order; without it (NULL or
- * too small) one workgroup per row block.
+ * 32 row blocks), the parts summed in a fixed order, and X, Y may be
+ * narrower than 16 columns; without it (NULL or too small) ncols, ldx and
+ * ldy must be 16 and one workgroup per row block runs.
  * Replaces: the test-set mean jnp.tensordot(k_td, k_inv_y, ...)
  * (predict.py:696) with the FP32 test-train kernel and the FP64 solution.
  */
 int nta_gemm_f32_f64(const float* K, long long ldk, int m, int n,
-                     const double* X, double* Y, void* work,
-                     long long work_bytes, void* stream);
+                     const double* X, long long ldx, int ncols, double* Y,
+                     long long ldy, void* work, long long work_bytes,
+                     void* stream);
 
-/* Workspace bytes of nta_gemm_f32_f64 (0: none used; -1: invalid sizes). */
+/* Workspace bytes of nta_gemm_f32_f64 (-1: invalid sizes). */
 long long nta_gemm_f32_f64_work_bytes(int m, int n);
 
 /*

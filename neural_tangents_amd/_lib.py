@@ -49,7 +49,8 @@ SIGNATURES = {
                   _LL, _P, _P, _P],
     'nta_pcg64_k64': [_P, _I, _LL, _D, _P, _P, _LL, _I, _P, _I, _P, _D, _I,
                       _P, _LL, _P, _P, _P],
-    'nta_gemm_f32_f64': [_P, _LL, _I, _I, _P, _P, _P, _LL, _P],
+    'nta_gemm_f32_f64': [_P, _LL, _I, _I, _P, _LL, _I, _P, _LL, _P, _LL,
+                         _P],
     'nta_pcg64_noalloc': [_P, _I, _LL, _D, _P, _P, _LL, _I, _P, _I, _P, _D,
                           _I, _P, _P, _P],
     'nta_diag_chain': [_P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P,

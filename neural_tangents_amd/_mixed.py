@@ -73,10 +73,13 @@ class MixedCholesky:
     # non-positive pivot: the kernel is too ill-conditioned for single
     # precision at this lambda), the shift grows by 10x; the FP64-residual
     # iteration still converges to the solution of the unshifted system.
-    # The pivot check waits for the first solve (the refinement synchronises
-    # anyway), so the factorisation and the refinement run back to back.
+    # The pivot check waits for the first solve: the factorisation's status
+    # goes to pinned host memory by an asynchronous copy behind it, which
+    # the refinement's synchronisation completes, so the factorisation and
+    # the refinement run back to back and the check costs no round trip.
     self._K32 = K32
     self._info = torch.zeros((1,), dtype=torch.int32, device=self.device)
+    self._info_host = torch.zeros((1,), dtype=torch.int32, pin_memory=True)
     self.shift = self.lam
     self._factor(self.shift)
     self._checked = False
@@ -94,13 +97,17 @@ class MixedCholesky:
     _lib.call('nta_chol_f32', _lib.ptr(A), self.npad, self.npad,
               _lib.ptr(self.W), self.npad, _lib.ptr(self.WT),
               _lib.ptr(self._info), _lib.stream_handle(self.device))
+    self._info_host.copy_(self._info, non_blocking=True)
 
-  def _factor_ok(self):
+  def _factor_ok(self, synced=False):
     """True once the factor in W has no non-positive pivot; refactors with
-    a growing shift (synchronously) after a breakdown and returns False."""
+    a growing shift (synchronously) after a breakdown and returns False.
+    synced: the stream was synchronised after the factorisation (its status
+    is then in the pinned host copy)."""
     if self._checked:
       return True
-    if int(self._info.item()) == 0:
+    info = int(self._info_host[0]) if synced else int(self._info.item())
+    if info == 0:
       self._checked = True
       return True
     mean_diag = float(torch.diagonal(self._K32)[:self.n].double().mean())
@@ -167,8 +174,8 @@ class MixedCholesky:
     return max(100.0 * self.tol, 1e-8)
 
   def _solve16(self, b):
-    X, it, r = self._refine(b)
-    if not self._factor_ok():     # the factor was replaced: solve again
+    X, it, r = self._refine(b)       # synchronises the stream
+    if not self._factor_ok(synced=True):   # replaced: solve again
       X, it, r = self._refine(b)
     if not r <= self.accept_residual():
       warnings.warn(

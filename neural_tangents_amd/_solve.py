@@ -47,6 +47,8 @@ def kernel_times(K: torch.Tensor, X: torch.Tensor) -> torch.Tensor:
     return K.to(X.device, torch.float64) @ X.to(torch.float64)
   m, n = K.shape
   X = X.to(K.device, torch.float64)
+  if X.stride(1) != 1 or X.data_ptr() % 8 != 0:
+    X = X.contiguous()
   mp, npad = -(-m // 32) * 32, -(-n // 16) * 16
   if (mp == m and npad == n and K.is_contiguous() and n % 4 == 0 and
       K.data_ptr() % 16 == 0):
@@ -54,19 +56,22 @@ def kernel_times(K: torch.Tensor, X: torch.Tensor) -> torch.Tensor:
   else:
     Kp = torch.zeros((mp, npad), dtype=torch.float32, device=K.device)
     Kp[:m, :n] = K
+  if npad != n:                      # X's rows must cover K's columns
+    Xp = torch.zeros((npad, X.shape[1]), dtype=torch.float64,
+                     device=K.device)
+    Xp[:n] = X
+    X = Xp
   stream = _lib.stream_handle(K.device)
   wb = _lib.value('nta_gemm_f32_f64_work_bytes', mp, npad)
-  work = torch.empty((max(wb, 1),), dtype=torch.uint8, device=K.device)
-  cols = []
-  for c0 in range(0, X.shape[1], 16):
-    xc = X[:, c0:c0 + 16]
-    Xp = torch.zeros((npad, 16), dtype=torch.float64, device=K.device)
-    Xp[:n, :xc.shape[1]] = xc
-    Y = torch.empty((mp, 16), dtype=torch.float64, device=K.device)
+  work = torch.empty((wb,), dtype=torch.uint8, device=K.device)
+  C = X.shape[1]
+  Y = torch.empty((mp, C), dtype=torch.float64, device=K.device)
+  for c0 in range(0, C, 16):           # 16 columns per call, strided views
+    nc = min(16, C - c0)
     _lib.call('nta_gemm_f32_f64', _lib.ptr(Kp), npad, mp, npad,
-              _lib.ptr(Xp), _lib.ptr(Y), _lib.ptr(work), wb, stream)
-    cols.append(Y[:m, :xc.shape[1]])
-  return torch.cat(cols, dim=1) if len(cols) > 1 else cols[0]
+              X.data_ptr() + 8 * c0, X.stride(0), nc,
+              Y.data_ptr() + 8 * c0, Y.stride(0), _lib.ptr(work), wb, stream)
+  return Y[:m] if mp != m else Y
 
 
 class CholeskySolver:

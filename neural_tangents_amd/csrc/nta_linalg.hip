@@ -1877,10 +1877,15 @@ static int pcg_run(const TK* K, int n, long long ldk, double lam,
       return set_error((int)e, "nta_pcg64: %s", hipGetErrorString(e));
     return 0;
   };
+  // Largest value over the columns with b != 0; NaN if any is NaN (a
+  // broken-down preconditioner must never read as converged).
   auto worst = [&](const double* v) {
     double m = 0.0;
-    for (int c = 0; c < nrhs; ++c)
-      if (hs.bn[c] > 0.0) m = fmax(m, v[c]);
+    for (int c = 0; c < nrhs; ++c) {
+      if (!(hs.bn[c] > 0.0)) continue;
+      if (v[c] != v[c]) return v[c];
+      m = fmax(m, v[c]);
+    }
     return m;
   };
   // r <- b - A x (FP64), z <- M^-1 r, then the test on the recomputed
@@ -1957,7 +1962,9 @@ static int pcg_run(const TK* K, int n, long long ldk, double lam,
       converged = true;
       break;
     }
-    const double q = fmin(worst(hs.rel), worst(hs.est));
+    const double wr = worst(hs.rel);
+    if (wr != wr) break;              // NaN residual: give up at once
+    const double q = fmin(wr, worst(hs.est));
     if (q < best) {
       best = q;
       stall = 0;
@@ -2080,15 +2087,32 @@ extern "C" int nta_chol_f32(float* A, int n, long long lda, float* W,
   return check_launch("nta_chol_f32");
 }
 
-// Y = sum over the parts of the column-split products, in a fixed order.
+// Y[row][c] (c < ncols, leading dimension ldy) = sum over the parts of the
+// column-split products ([nparts][rows][16]), in a fixed order.
 __global__ __launch_bounds__(256) void k_sum_parts(
-    const double* __restrict__ parts, int nparts, long long count,
-    double* __restrict__ Y) {
+    const double* __restrict__ parts, int nparts, long long rows,
+    double* __restrict__ Y, long long ldy, int ncols) {
   const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long count = rows * RHS;
   if (e >= count) return;
+  const long long row = e / RHS;
+  const int c = (int)(e - row * RHS);
+  if (c >= ncols) return;
   double v = parts[e];
   for (int q = 1; q < nparts; ++q) v += parts[(long long)q * count + e];
-  Y[e] = v;
+  Y[row * ldy + c] = v;
+}
+
+// Xp[k][c] = X[k][c] (c < ncols, leading dimension ldx), zero for c >= ncols:
+// the [n][16] operand layout of the row kernel.
+__global__ __launch_bounds__(256) void k_pad_cols(
+    const double* __restrict__ X, long long ldx, int ncols, long long n,
+    double* __restrict__ Xp) {
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= n * RHS) return;
+  const long long k = e / RHS;
+  const int c = (int)(e - k * RHS);
+  Xp[e] = c < ncols ? X[k * ldx + c] : 0.0;
 }
 
 // Column parts of nta_gemm_f32_f64: enough row-block x part workgroups to
@@ -2102,36 +2126,53 @@ static int gemm_f32_f64_parts(int m, int n) {
 extern "C" long long nta_gemm_f32_f64_work_bytes(int m, int n) {
   if (m <= 0 || n <= 0 || m % 32 != 0 || n % 16 != 0) return -1;
   const int parts = gemm_f32_f64_parts(m, n);
-  return parts > 1 ? (long long)parts * m * RHS * sizeof(double) : 0;
+  return ((long long)parts * m + n) * RHS * (long long)sizeof(double);
 }
 
 extern "C" int nta_gemm_f32_f64(const float* K, long long ldk, int m, int n,
-                                const double* X, double* Y, void* work,
+                                const double* X, long long ldx, int ncols,
+                                double* Y, long long ldy, void* work,
                                 long long work_bytes, void* stream) {
-  if (m <= 0 || n <= 0 || ldk < n || !K || !X || !Y)
+  if (m <= 0 || n <= 0 || ldk < n || !K || !X || !Y || ncols < 1 ||
+      ncols > RHS || ldx < ncols || ldy < ncols)
     return set_error(NTA_EINVAL, "nta_gemm_f32_f64: bad m=%d n=%d ldk=%lld "
-                     "or null pointer", m, n, ldk);
+                     "ncols=%d ldx=%lld ldy=%lld or null pointer", m, n,
+                     ldk, ncols, ldx, ldy);
   if (m % 32 != 0 || n % 16 != 0 || ldk % 4 != 0 ||
       ((uintptr_t)K & 15) != 0)
     return set_error(NTA_EUNSUPPORTED, "nta_gemm_f32_f64: m=%d, n=%d and "
                      "ldk=%lld must be multiples of 32, 16 and 4 and K "
                      "16-byte aligned (pad the matrix)", m, n, ldk);
   hipStream_t st = (hipStream_t)stream;
-  const int parts = gemm_f32_f64_parts(m, n);
-  const long long need = parts > 1 ? (long long)parts * m * RHS * 8 : 0;
-  if (parts == 1 || !work || work_bytes < need) {
+  const bool dense = ncols == RHS && ldx == RHS && ldy == RHS;
+  const long long need = nta_gemm_f32_f64_work_bytes(m, n);
+  if (!work || work_bytes < need) {
+    if (!dense)
+      return set_error(NTA_EINVAL, "nta_gemm_f32_f64: ncols=%d, ldx=%lld, "
+                       "ldy=%lld need a workspace of %lld bytes", ncols,
+                       ldx, ldy, need);
     hipLaunchKernelGGL(k_symv_rows<float>, dim3(m / 32), dim3(kSvThreads), 0,
                        st, K, ldk, n, 0.0, 2, X, Y, nullptr, nullptr,
                        nullptr, nullptr, nullptr);
     return check_launch("nta_gemm_f32_f64");
   }
+  const int parts = gemm_f32_f64_parts(m, n);
   double* pw = (double*)work;
+  const double* Xp = X;
+  if (!(ncols == RHS && ldx == RHS)) {
+    double* xw = pw + (long long)parts * m * RHS;
+    const long long cx = (long long)n * RHS;
+    hipLaunchKernelGGL(k_pad_cols, dim3((unsigned)((cx + 255) / 256)),
+                       dim3(256), 0, st, X, ldx, ncols, (long long)n, xw);
+    Xp = xw;
+  }
   hipLaunchKernelGGL(k_symv_rows<float>, dim3(m / 32, parts),
-                     dim3(kSvThreads), 0, st, K, ldk, n, 0.0, 2, X, pw,
+                     dim3(kSvThreads), 0, st, K, ldk, n, 0.0, 2, Xp, pw,
                      nullptr, nullptr, nullptr, nullptr, nullptr);
   const long long count = (long long)m * RHS;
   hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((count + 255) / 256)),
-                     dim3(256), 0, st, pw, parts, count, Y);
+                     dim3(256), 0, st, pw, parts, (long long)m, Y, ldy,
+                     ncols);
   return check_launch("nta_gemm_f32_f64");
 }
 

@@ -185,9 +185,42 @@ def test_kernel_times_rejects_unpadded_shapes():
   X = torch.zeros((64, 16), dtype=torch.float64, device=DEV)
   Y = torch.empty((48, 16), dtype=torch.float64, device=DEV)
   with pytest.raises(NotImplementedError):
-    _lib.call('nta_gemm_f32_f64', _lib.ptr(K), 64, 48, 64, _lib.ptr(X),
-              _lib.ptr(Y), None, 0, _lib.stream_handle(DEV))
+    _lib.call('nta_gemm_f32_f64', _lib.ptr(K), 64, 48, 64, _lib.ptr(X), 16,
+              16, _lib.ptr(Y), 16, None, 0, _lib.stream_handle(DEV))
   assert _lib.value('nta_gemm_f32_f64_work_bytes', 48, 64) == -1
+  # narrower than 16 columns needs the workspace
+  K = torch.zeros((64, 64), device=DEV)
+  with pytest.raises(ValueError, match='workspace'):
+    _lib.call('nta_gemm_f32_f64', _lib.ptr(K), 64, 64, 64, _lib.ptr(X), 16,
+              10, _lib.ptr(Y), 16, None, 0, _lib.stream_handle(DEV))
+
+
+@pytest.mark.parametrize('ncols,ldx,ldy', [(10, 10, 10), (16, 16, 16),
+                                           (3, 7, 5), (16, 20, 18)])
+def test_gemm_f32_f64_strided_columns(ncols, ldx, ldy):
+  """Y[:, :ncols] = K X[:, :ncols] with leading dimensions ldx, ldy: the
+  columns past ncols of Y are left untouched; with and without the
+  workspace when the layout is dense."""
+  m, n = 96, 512
+  g = torch.Generator(device=DEV).manual_seed(ncols + ldx)
+  K = torch.randn(m, n, generator=g, device=DEV)
+  X = torch.randn(n, ldx, generator=g, device=DEV, dtype=torch.float64)
+  Y = torch.full((m, ldy), 5.0, dtype=torch.float64, device=DEV)
+  wb = _lib.value('nta_gemm_f32_f64_work_bytes', m, n)
+  work = torch.empty((wb,), dtype=torch.uint8, device=DEV)
+  _lib.call('nta_gemm_f32_f64', _lib.ptr(K), n, m, n, _lib.ptr(X), ldx,
+            ncols, _lib.ptr(Y), ldy, _lib.ptr(work), wb,
+            _lib.stream_handle(DEV))
+  want = K.double() @ X[:, :ncols]
+  err = float((Y[:, :ncols] - want).abs().max() / want.abs().max())
+  assert err <= 1e-13, err
+  assert bool((Y[:, ncols:] == 5.0).all())
+  if ncols == ldx == ldy == 16:
+    Y2 = torch.empty_like(Y)
+    _lib.call('nta_gemm_f32_f64', _lib.ptr(K), n, m, n, _lib.ptr(X), 16,
+              16, _lib.ptr(Y2), 16, None, 0, _lib.stream_handle(DEV))
+    err = float((Y2 - want).abs().max() / want.abs().max())
+    assert err <= 1e-13, err
 
 
 def test_copy_lower_shift_writes_lower_tiles_only():
