@@ -168,9 +168,9 @@ def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=64):
       gn = g_nngp[i0:i0 + n1, j0:j0 + n2].reshape(T)
       gt = g_ntk[i0:i0 + n1, j0:j0 + n2].reshape(T)
       shp = (T, last.hout, last.hout, last.wout, last.wout)
-      gN = ((a * gn + c * gt) / nlast)[:, None].expand(T, nlast).reshape(
-          shp).contiguous()
-      gT = ((d * gt) / nlast)[:, None].expand(T, nlast).reshape(
+      gN = gn.mul(a / nlast).add_(gt, alpha=c / nlast)[:, None].expand(
+          T, nlast).reshape(shp).contiguous()
+      gT = gt.mul(d / nlast)[:, None].expand(T, nlast).reshape(
           shp).contiguous()
       for si in range(L - 1, -1, -1):
         s = plan.stages[si]
