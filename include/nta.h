@@ -225,6 +225,22 @@ long long nta_relu_bwd_parts(int n1, int n2, int S);
 long long nta_relu_bwd_work_bytes(int n1, int n2, int S);
 
 /*
+ * Accumulates the outputs of an nta_relu_bwd call on the same tile into
+ * per-image and per-stage totals, in a fixed order (each argument may be
+ * NULL to skip it):
+ *   wb_acc[0], wb_acc[1] += the sums of gwb's d/dw_var, d/db_var slots;
+ *   gq1_acc [n1][S][S] += gqa summed over the tile's second images (a self
+ *     tile, n2 == 0: += gqa + gqb, both normalisers being q1's);
+ *   gq2_acc [n2][S][S] += gqb summed over the tile's first images (cross
+ *     tiles), after gq1_acc, so the two may be rows of one array.
+ * No counterpart in the reference (jax.grad accumulates these itself).
+ */
+int nta_relu_bwd_accumulate(const float* gqa, const float* gqb,
+                            const double* gwb, int n1, int n2, int S,
+                            double* wb_acc, float* gq1_acc, float* gq2_acc,
+                            void* stream);
+
+/*
  * Diagonal-spatial chain: a network of nstage [Conv fh x fw, stride 1]
  * [-> ReLU] stages without pooling, read out by Flatten and the composed
  * Dense map (a, b, c, d, e as in nta_head), on the spatial diagonal

@@ -63,6 +63,7 @@ SIGNATURES = {
     'nta_qnorm': [_P, _LL, _P, _P],
     'nta_relu_bwd': [_P, _P, _I, _I, _I, _I, _I, _F, _F, _F, _F, _P, _P, _P,
                      _P, _F, _P, _P, _P, _P, _P, _P, _LL, _P],
+    'nta_relu_bwd_accumulate': [_P, _P, _P, _I, _I, _I, _P, _P, _P, _P],
 }
 
 # Entry points that return a value instead of a status.

@@ -358,6 +358,49 @@ __global__ void k_bwd_qsum(const float* __restrict__ ea,
   }
 }
 
+// acc[0], acc[1] += the sums of the (d/dw_var, d/db_var) partial slots
+// gwb[2 s], gwb[2 s + 1], s < parts, in a fixed order (one workgroup).
+__global__ __launch_bounds__(256) void k_bwd_acc_wb(
+    const double* __restrict__ gwb, long long parts, double* __restrict__ acc) {
+  __shared__ double red[2][256];
+  const int t = threadIdx.x;
+  double w = 0.0, b = 0.0;
+  for (long long q = t; q < parts; q += 256) {
+    w += gwb[2 * q];
+    b += gwb[2 * q + 1];
+  }
+  red[0][t] = w;
+  red[1][t] = b;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) {
+      red[0][t] += red[0][t + h];
+      red[1][t] += red[1][t + h];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    acc[0] += red[0][0];
+    acc[1] += red[1][0];
+  }
+}
+
+// acc[o][k] += sum over b < nb of src[(o so + b sb) S2 + k] (k < S2), in
+// order of b: a normaliser cotangent summed over the partner images.
+__global__ __launch_bounds__(256) void k_bwd_acc_q(
+    const float* __restrict__ src, int no, int nb, long long so,
+    long long sb, int S2, float* __restrict__ acc) {
+  const long long n = (long long)no * S2;
+  for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < n;
+       x += (long long)gridDim.x * blockDim.x) {
+    const long long o = x / S2;
+    const int k = (int)(x - o * S2);
+    float v = 0.0f;
+    for (int b = 0; b < nb; ++b) v += src[(o * so + b * sb) * S2 + k];
+    acc[x] += v;
+  }
+}
+
 static bool bwd_rows(int S) { return S == 8 || S == 16 || S == 32; }
 
 static long long bwd_pairs(int n1, int n2) {
@@ -465,4 +508,41 @@ extern "C" int nta_relu_bwd(const float* aN, const float* aT, int n1, int n2,
                        st, part, part + T * per, T, S, gqa, gqb);
   }
   return check_launch("nta_relu_bwd");
+}
+
+extern "C" int nta_relu_bwd_accumulate(const float* gqa, const float* gqb,
+                                       const double* gwb, int n1, int n2,
+                                       int S, double* wb_acc, float* gq1_acc,
+                                       float* gq2_acc, void* stream) {
+  if (n1 < 0 || n2 < 0 || S <= 0)
+    return set_error(NTA_EINVAL, "nta_relu_bwd_accumulate: bad n1=%d n2=%d "
+                     "S=%d", n1, n2, S);
+  if (n1 == 0) return NTA_OK;
+  if ((wb_acc && !gwb) || ((gq1_acc || gq2_acc) && (!gqa || !gqb)) ||
+      (gq2_acc && n2 == 0))
+    return set_error(NTA_EINVAL, "nta_relu_bwd_accumulate: null pointer or "
+                     "a second-image accumulator for a self tile");
+  hipStream_t st = (hipStream_t)stream;
+  const int S2 = S * S;
+  if (wb_acc)
+    hipLaunchKernelGGL(k_bwd_acc_wb, dim3(1), dim3(256), 0, st, gwb,
+                       nta_relu_bwd_parts(n1, n2, S), wb_acc);
+  if (gq1_acc) {
+    if (n2 == 0) {                     // self tile: both normalisers are q1
+      hipLaunchKernelGGL(k_bwd_acc_q, dim3(grid_for((long long)n1 * S2)),
+                         dim3(256), 0, st, gqa, n1, 1, 1LL, 0LL, S2, gq1_acc);
+      hipLaunchKernelGGL(k_bwd_acc_q, dim3(grid_for((long long)n1 * S2)),
+                         dim3(256), 0, st, gqb, n1, 1, 1LL, 0LL, S2, gq1_acc);
+    } else {
+      hipLaunchKernelGGL(k_bwd_acc_q, dim3(grid_for((long long)n1 * S2)),
+                         dim3(256), 0, st, gqa, n1, n2, (long long)n2, 1LL,
+                         S2, gq1_acc);
+    }
+  }
+  // after gq1_acc (the two may alias for a tile of K(x, x))
+  if (gq2_acc)
+    hipLaunchKernelGGL(k_bwd_acc_q, dim3(grid_for((long long)n2 * S2)),
+                       dim3(256), 0, st, gqb, n2, n1, 1LL, (long long)n2, S2,
+                       gq2_acc);
+  return check_launch("nta_relu_bwd_accumulate");
 }

@@ -69,13 +69,17 @@ NO_SELF = 2 ** 31 - 1
 
 
 def _relu_bwd(aN, aT, n1, n2, S, pool, w_var, b_var, q1, q2, gN, gT,
-              stream, self_off=NO_SELF, act=(1.0, 0.0), gram_scale=0.0):
+              stream, self_off=NO_SELF, act=(1.0, 0.0), gram_scale=0.0,
+              acc=None):
   """Adjoint of one stage's ReLU (+ pool).  Returns (gaN, gaT, gqa, gqb,
   gw, gb): gqa, gqb [T, S, S] the normaliser cotangents summed over the
   partner's positions, gw, gb the d/dw_var, d/db_var contributions as 0-d
   FP64 device tensors (summed from the kernel's per-workgroup partials,
   with no host synchronisation).  gram_scale != 0: gaN in the patch-Gram
-  layout [n1, S, S, n2, S, S] times gram_scale (cross tiles)."""
+  layout [n1, S, S, n2, S, S] times gram_scale (cross tiles).
+  acc = (wb_acc [2] FP64, gq1_acc [n1, S, S], gq2_acc [n2, S, S]): the
+  library adds gw, gb and the partner-summed gqa, gqb into these instead
+  (nta_relu_bwd_accumulate), and gw, gb come back as None."""
   T = aN.shape[0]
   gaN = torch.empty_like(aN)
   gaT = torch.empty_like(aN) if aT is not None else None
@@ -91,6 +95,12 @@ def _relu_bwd(aN, aT, n1, n2, S, pool, w_var, b_var, q1, q2, gN, gT,
             _lib.ptr(q2), _lib.ptr(gN), _lib.ptr(gT), gram_scale,
             _lib.ptr(gaN), _lib.ptr(gaT), _lib.ptr(gqa), _lib.ptr(gqb),
             _lib.ptr(gwb), _lib.ptr(work), work.numel(), stream)
+  if acc is not None:
+    wb_acc, gq1_acc, gq2_acc = acc
+    _lib.call('nta_relu_bwd_accumulate', _lib.ptr(gqa), _lib.ptr(gqb),
+              _lib.ptr(gwb), n1, n2, S, _lib.ptr(wb_acc), _lib.ptr(gq1_acc),
+              _lib.ptr(gq2_acc), stream)
+    return gaN, gaT, gqa, gqb, None, None
   sums = gwb[:parts].sum(dim=0)
   return gaN, gaT, gqa, gqb, sums[0], sums[1]
 
@@ -140,6 +150,7 @@ def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=64):
   # d/dw_var, d/db_var per stage, accumulated on the device (FP64)
   gwv = [torch.zeros((), dtype=torch.float64, device=dev) for _ in range(L)]
   gbv = [torch.zeros((), dtype=torch.float64, device=dev) for _ in range(L)]
+  wb_acc = torch.zeros((L, 2), dtype=torch.float64, device=dev)  # cross
   gx1 = torch.zeros_like(x1)
   gx2 = gx1 if sym else torch.zeros_like(x2)
   first = plan.stages[0]
@@ -186,14 +197,11 @@ def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=64):
                            C, fh, fw, stream)
           aT = None
         first_stage = si == 0
-        gaN, gaT, gqa, gqb, dw, db = _relu_bwd(
+        gaN, gaT, _, _, _, _ = _relu_bwd(
             aN, aT, n1, n2, S, pool, w_var, b_var, qa[si], qb[si], gN, gT,
             stream, self_off=(i0 - j0) if sym else NO_SELF, act=s.act,
-            gram_scale=1.0 / (fh * fw * C) if first_stage else 0.0)
-        gwv[si] += dw
-        gbv[si] += db
-        gq1[si][i0:i0 + n1] += gqa.reshape(n1, n2, S, S).sum(1)
-        gq2[si][j0:j0 + n2] += gqb.reshape(n1, n2, S, S).sum(0)
+            gram_scale=1.0 / (fh * fw * C) if first_stage else 0.0,
+            acc=(wb_acc[si], gq1[si][i0:i0 + n1], gq2[si][j0:j0 + n2]))
         if not first_stage:
           gN = _stencil(gaN, T, S, fh_s, fw_s, valid, stream)
           gT = _stencil(gaT, T, S, fh_s, fw_s, valid, stream)
@@ -211,7 +219,8 @@ def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=64):
   # Self chains: push the normaliser cotangents through each image's chain.
   for x, gq, gx in ((x1, gq1, gx1),) + (() if sym else ((x2, gq2, gx2),)):
     _self_chain_vjp(plan, x, gq, gx, gwv, gbv, stream, C, fh, fw)
-  sums = torch.stack(gwv + gbv).cpu()       # one synchronisation
+  sums = torch.cat([torch.stack(gwv) + wb_acc[:, 0],
+                    torch.stack(gbv) + wb_acc[:, 1]]).cpu()   # one sync
   return (gx1, (None if sym else gx2), [float(v) for v in sums[:L]],
           [float(v) for v in sums[L:]])
 

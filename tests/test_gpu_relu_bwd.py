@@ -138,3 +138,42 @@ def test_relu_bwd_rejects_bad_arguments():
   assert _lib.value('nta_relu_bwd_work_bytes', 2, 3, 32) == 6 * 32 ** 3 * 4
   assert _lib.value('nta_relu_bwd_work_bytes', 2, 3, 6) == 2 * 6 * 6 ** 4 * 4
   assert _lib.value('nta_relu_bwd_parts', 2, 3, 16) == 6 * 16
+
+
+@pytest.mark.parametrize('n1,n2,S', [(2, 3, 8), (3, 0, 16), (1, 5, 32)])
+def test_relu_bwd_accumulate_matches_torch_sums(n1, n2, S):
+  """nta_relu_bwd_accumulate: the partner sums of gqa / gqb and the slot
+  sums of gwb added into the accumulators (self tiles: gqa + gqb into
+  q1's), and the two accumulators as rows of one array (K(x, x) tiles)."""
+  dev = torch.device('cuda', 0)
+  g = torch.Generator(device=dev).manual_seed(n1 + n2 + S)
+  T = n1 if n2 == 0 else n1 * n2
+  gqa = torch.randn((T, S, S), generator=g, device=dev)
+  gqb = torch.randn((T, S, S), generator=g, device=dev)
+  parts = _lib.value('nta_relu_bwd_parts', n1, n2, S)
+  gwb = torch.randn((parts, 2), generator=g, device=dev, dtype=torch.float64)
+  wb = torch.randn((2,), generator=g, device=dev, dtype=torch.float64)
+  q1 = torch.randn((n1, S, S), generator=g, device=dev)
+  q2 = torch.randn((max(n2, 1), S, S), generator=g, device=dev)
+  want_wb = wb + gwb.sum(0)
+  if n2 == 0:
+    want1 = q1 + gqa + gqb
+  else:
+    want1 = q1 + gqa.reshape(n1, n2, S, S).sum(1)
+    want2 = q2 + gqb.reshape(n1, n2, S, S).sum(0)
+  _lib.call('nta_relu_bwd_accumulate', _lib.ptr(gqa), _lib.ptr(gqb),
+            _lib.ptr(gwb), n1, n2, S, _lib.ptr(wb), _lib.ptr(q1),
+            _lib.ptr(q2) if n2 else None, _lib.stream_handle(dev))
+  torch.testing.assert_close(wb, want_wb, rtol=1e-12, atol=1e-12)
+  torch.testing.assert_close(q1, want1, rtol=1e-5, atol=1e-5)
+  if n2:
+    torch.testing.assert_close(q2, want2, rtol=1e-5, atol=1e-5)
+    # one array, overlapping rows: q1 rows [0, n1), q2 rows [1, 1 + n2)
+    Q = torch.randn((1 + max(n1, n2), S, S), generator=g, device=dev)
+    want = Q.clone()
+    want[:n1] += gqa.reshape(n1, n2, S, S).sum(1)
+    want[1:1 + n2] += gqb.reshape(n1, n2, S, S).sum(0)
+    _lib.call('nta_relu_bwd_accumulate', _lib.ptr(gqa), _lib.ptr(gqb),
+              _lib.ptr(gwb), n1, n2, S, None, _lib.ptr(Q[:n1]),
+              _lib.ptr(Q[1:1 + n2]), _lib.stream_handle(dev))
+    torch.testing.assert_close(Q, want, rtol=1e-5, atol=1e-5)
