@@ -240,17 +240,23 @@ def krr_result(ntr, nte, reps, tol=1e-7):
     return time.perf_counter() - t, alpha, f, s
 
   # One untimed solve per form first (library streams and events, code
-  # loading, the allocator's first blocks), then the two forms alternate so
-  # that both see the same clocks; the median of each is reported.
+  # loading, the allocator's first blocks), then reps runs of each form in
+  # a row (each form's steady state when solving repeatedly); the median of
+  # each is the reported time.  A second, interleaved set (each run right
+  # after the other form's, so the caches hold the other form's data) is
+  # reported beside it.
   times = {'fp64': [], 'mixed': []}
   last = {}
   for mode in ('fp64', 'mixed'):
     solve_once(mode)
-  for _ in range(reps):
-    for mode in ('fp64', 'mixed'):
+    for _ in range(reps):
       dt, alpha, f, s = solve_once(mode)
       times[mode].append(dt)
       last[mode] = (alpha, f, s)
+  inter = {'fp64': [], 'mixed': []}
+  for _ in range(3):
+    for mode in ('fp64', 'mixed'):
+      inter[mode].append(solve_once(mode)[0])
   res = {}
   for mode in ('fp64', 'mixed'):
     alpha, f, s = last[mode]
@@ -282,11 +288,18 @@ def krr_result(ntr, nte, reps, tol=1e-7):
       'fell_back_to_fp64': bool(smx.impl.fell_back),
       'rel_err_alpha_vs_fp64': err_a, 'rel_err_ftest_vs_fp64': err_f,
       'kernel_seconds': t_kernel,
-      'timing': f'median of {reps} runs of each form (wall time, inputs '
-                'resident, kernel build excluded), the forms alternating '
-                'after one untimed run of each',
+      'timing': f'median of {reps} consecutive runs of each form after one '
+                'untimed run (wall time, inputs resident, kernel build '
+                'excluded)',
       'mixed_ms_all': [1e3 * t for t in sorted(times['mixed'])],
       'plain_fp64_ms_all': [1e3 * t for t in sorted(times['fp64'])],
+      'interleaved': {
+          'note': 'medians of 3 runs of each form, each right after a run '
+                  'of the other form (caches holding the other form\'s '
+                  'data)',
+          'plain_fp64_ms': 1e3 * sorted(inter['fp64'])[1],
+          'mixed_ms': 1e3 * sorted(inter['mixed'])[1],
+          'value': sorted(inter['fp64'])[1] / sorted(inter['mixed'])[1]},
   }
 
 
