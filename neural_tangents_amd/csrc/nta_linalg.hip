@@ -1666,9 +1666,9 @@ __global__ __launch_bounds__(256) void k_pcg_check(
 // The test on a recomputed residual before its preconditioning: when every
 // column's relative residual is <= tol the solve has converged (skip = 1,
 // done = 1) and the preconditioner and k_pcg_check of the recomputation
-// return at once; otherwise skip = 0 and they run.  With a gate that is
-// set (the recurrence has not converged: the speculative recomputation did
-// not run), skip = 1 and nothing else changes.
+// return at once; otherwise skip = 0, done = 0 and they run.  With a gate
+// that is set (the recurrence has not converged: the speculative
+// recomputation did not run), skip = 1 and nothing else changes.
 __global__ __launch_bounds__(256) void k_pcg_rcheck(
     const double* __restrict__ rr, int nrr, int nrhs, double tol,
     const int* __restrict__ gate, PcgState* __restrict__ st) {
@@ -1692,8 +1692,8 @@ __global__ __launch_bounds__(256) void k_pcg_rcheck(
   if (all && t < RHS) st->rel[t] = rel;
   if (t == 0) {
     st->skip = all;
+    st->done = all;     // a failed test overrides the recurrence's verdict
     if (all) {
-      st->done = 1;
       st->notdone = 0;
       st->restart = 1;
     }
@@ -1891,18 +1891,26 @@ static int pcg_run(const TK* K, int n, long long ldk, double lam,
   // r <- b - A x (FP64), z <- M^-1 r, then the test on the recomputed
   // residual (restarts the direction).  Speculative: the device runs it only
   // if the recurrence has converged (gate notdone).
-  auto recompute = [&](bool speculative) {
+  // r <- b - A x and the test of its norm alone (verify), then, unless
+  // that test passed, its preconditioning and the full test (finish).
+  auto verify = [&](bool speculative) {
     const int* gate = speculative ? &w.st->notdone : nullptr;
     hipLaunchKernelGGL(k_symv_rows<TK>, dim3(nsym), dim3(kSvThreads), 0, st,
                        K, ldk, n, lam, 1, w.x, nullptr, w.b, w.r, w.r32,
                        w.rr, gate);
     hipLaunchKernelGGL(k_pcg_rcheck, dim3(1), dim3(256), 0, st, w.rr, nsym,
                        nrhs, tol, gate, w.st);
+  };
+  auto finish = [&]() {
     precond(W, WT, ldw, n, n1, w.r32, w.z32, w.r, w.rz, w.zz, &w.st->skip,
             w, st);
     hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(256), 0, st, w.rr, nsym,
                        nullptr, 0, w.xx, nvec, w.rz, w.zz, nrz, nrhs, tol,
                        0, w.st);
+  };
+  auto recompute = [&]() {
+    verify(false);
+    finish();
   };
   auto copy_out = [&]() {
     hipLaunchKernelGGL(k_pcg_io, dim3(nvec), dim3(256), 0, st, n, nrhs,
@@ -1942,7 +1950,7 @@ static int pcg_run(const TK* K, int n, long long ldk, double lam,
     // If the recurrence converged in this batch, the device confirms it on
     // the recomputed residual and copies the solution out before the one
     // synchronisation of the batch.
-    recompute(true);
+    verify(true);
     copy_out();
     rc = read_state();
     if (rc) return rc;
@@ -1951,10 +1959,23 @@ static int pcg_run(const TK* K, int n, long long ldk, double lam,
       converged = copied = true;
       break;
     }
+    if (!hs.skip) {
+      // The recurrence converged but the recomputed residual alone does
+      // not meet tol: its preconditioning and the full test (the copy-out
+      // above stays valid: x is unchanged).
+      finish();
+      rc = read_state();
+      if (rc) return rc;
+      if (hs.done) {
+        it = hs.iters;
+        converged = copied = true;
+        break;
+      }
+    }
     if (batches % 2 != 0) continue;
     // a periodic checkpoint on the recomputed residual (the recurrence
     // drifts on ill-conditioned systems), also the stall test
-    recompute(false);
+    recompute();
     rc = read_state();
     if (rc) return rc;
     if (hs.done) {
@@ -1975,7 +1996,7 @@ static int pcg_run(const TK* K, int n, long long ldk, double lam,
   if (!converged) {
     // The returned x's recomputed residual (the test above may not have
     // run for the last iterations).
-    recompute(false);
+    recompute();
     rc = read_state();
     if (rc) return rc;
   }
