@@ -1692,7 +1692,7 @@ __global__ __launch_bounds__(256) void k_pcg_rcheck(
   if (all && t < RHS) st->rel[t] = rel;
   if (t == 0) {
     st->skip = all;
-    st->done = all;     // a failed test overrides the recurrence's verdict
+    st->done = all;     // a failed test overrides the recurrence's result
     if (all) {
       st->notdone = 0;
       st->restart = 1;
