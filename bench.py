@@ -355,7 +355,7 @@ def grad_result(n_target=2048):
                          yt[:nt], W, b, builder, diag_reg=1e-4)
     loss.backward()
     torch.cuda.synchronize()
-    return float(loss)
+    return float(loss.detach())
 
   run(4, 8)                                    # warm-up
   torch.cuda.synchronize()

@@ -401,6 +401,35 @@ __global__ __launch_bounds__(256) void k_bwd_acc_q(
   }
 }
 
+// k_bwd_acc_q for many partners (a tile of one image against a row of
+// others): a workgroup owns 64 consecutive k of one o, its four waves sum
+// the partners b = w, w + 4, ... and the four sums are added in a fixed
+// order.
+__global__ __launch_bounds__(256) void k_bwd_acc_q4(
+    const float* __restrict__ src, int no, int nb, long long so,
+    long long sb, int S2, float* __restrict__ acc) {
+  __shared__ float red[3][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long long kblocks = (S2 + 63) / 64, nblk = (long long)no * kblocks;
+  for (long long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const long long o = blk / kblocks;
+    const int k = (int)(blk - o * kblocks) * 64 + lane;
+    float v = 0.0f;
+    if (k < S2) {
+      const float* s = src + o * so * S2 + k;
+      const long long step = sb * S2;
+#pragma unroll 4
+      for (int b = w; b < nb; b += 4) v += s[b * step];
+    }
+    if (w) red[w - 1][lane] = v;
+    __syncthreads();
+    if (w == 0 && k < S2)
+      acc[o * S2 + k] += (v + red[0][lane]) + (red[1][lane] + red[2][lane]);
+    __syncthreads();                  // red is rewritten by the next block
+  }
+}
+
+
 static bool bwd_rows(int S) { return S == 8 || S == 16 || S == 32; }
 
 static long long bwd_pairs(int n1, int n2) {
@@ -418,6 +447,20 @@ static long long relu_bwd_blocks(int n1, int n2, int S) {
 static unsigned grid_for(long long n) {
   long long g = (n + 255) / 256;
   return (unsigned)(g < 1 ? 1 : (g > 65536 ? 65536 : g));
+}
+
+// acc[o] += sum over b of src[o so + b sb] (rows of S2 floats).
+static void acc_q(const float* src, int no, int nb, long long so,
+                  long long sb, int S2, float* acc, hipStream_t st) {
+  if (nb >= 8) {
+    const long long nblk = (long long)no * ((S2 + 63) / 64);
+    hipLaunchKernelGGL(k_bwd_acc_q4,
+                       dim3((unsigned)(nblk < 65536 ? nblk : 65536)),
+                       dim3(256), 0, st, src, no, nb, so, sb, S2, acc);
+  } else {
+    hipLaunchKernelGGL(k_bwd_acc_q, dim3(grid_for((long long)no * S2)),
+                       dim3(256), 0, st, src, no, nb, so, sb, S2, acc);
+  }
 }
 
 }  // namespace nta
@@ -529,20 +572,13 @@ extern "C" int nta_relu_bwd_accumulate(const float* gqa, const float* gqb,
                        nta_relu_bwd_parts(n1, n2, S), wb_acc);
   if (gq1_acc) {
     if (n2 == 0) {                     // self tile: both normalisers are q1
-      hipLaunchKernelGGL(k_bwd_acc_q, dim3(grid_for((long long)n1 * S2)),
-                         dim3(256), 0, st, gqa, n1, 1, 1LL, 0LL, S2, gq1_acc);
-      hipLaunchKernelGGL(k_bwd_acc_q, dim3(grid_for((long long)n1 * S2)),
-                         dim3(256), 0, st, gqb, n1, 1, 1LL, 0LL, S2, gq1_acc);
+      acc_q(gqa, n1, 1, 1LL, 0LL, S2, gq1_acc, st);
+      acc_q(gqb, n1, 1, 1LL, 0LL, S2, gq1_acc, st);
     } else {
-      hipLaunchKernelGGL(k_bwd_acc_q, dim3(grid_for((long long)n1 * S2)),
-                         dim3(256), 0, st, gqa, n1, n2, (long long)n2, 1LL,
-                         S2, gq1_acc);
+      acc_q(gqa, n1, n2, (long long)n2, 1LL, S2, gq1_acc, st);
     }
   }
   // after gq1_acc (the two may alias for a tile of K(x, x))
-  if (gq2_acc)
-    hipLaunchKernelGGL(k_bwd_acc_q, dim3(grid_for((long long)n2 * S2)),
-                       dim3(256), 0, st, gqb, n2, n1, 1LL, (long long)n2, S2,
-                       gq2_acc);
+  if (gq2_acc) acc_q(gqb, n2, n1, 1LL, (long long)n2, S2, gq2_acc, st);
   return check_launch("nta_relu_bwd_accumulate");
 }

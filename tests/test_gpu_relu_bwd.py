@@ -140,7 +140,8 @@ def test_relu_bwd_rejects_bad_arguments():
   assert _lib.value('nta_relu_bwd_parts', 2, 3, 16) == 6 * 16
 
 
-@pytest.mark.parametrize('n1,n2,S', [(2, 3, 8), (3, 0, 16), (1, 5, 32)])
+@pytest.mark.parametrize('n1,n2,S', [(2, 3, 8), (3, 0, 16), (1, 5, 32),
+                                     (1, 64, 32), (37, 9, 8)])
 def test_relu_bwd_accumulate_matches_torch_sums(n1, n2, S):
   """nta_relu_bwd_accumulate: the partner sums of gqa / gqb and the slot
   sums of gwb added into the accumulators (self tiles: gqa + gqb into
