@@ -117,6 +117,23 @@ int nta_stage(const float* in_nngp, const float* in_ntk, int n1, int n2,
               float* out_ntk, void* stream);
 
 /*
+ * nta_stage for Conv(3x3 SAME) -> Relu -> AvgPool(2x2 / 2) on S x S states
+ * (S = 8, 16, 32; flags NTA_CONV | NTA_RELU [| NTA_NTK]) that also writes
+ * the stencil means of the inputs at every pre-pool position,
+ * pre[T][S][S][S][S] = (1/9) sum of the nine taps -- the pre-activations
+ * before the w_var, b_var affine map, which the stage's adjoint
+ * (nta_relu_bwd: aN, aT) consumes.  NTA_EUNSUPPORTED for other stages.
+ * Replaces: the same reference functions as nta_stage; the saved means
+ * stand in for what jax.grad keeps of the forward pass of Conv.kernel_fn
+ * (linear.py:1387-1398) for its reverse pass.
+ */
+int nta_stage_keep_pre(const float* in_nngp, const float* in_ntk, int n1,
+                       int n2, int S, int flags, float w_var, float b_var,
+                       const float* q1, const float* q2, float* out_nngp,
+                       float* out_ntk, float* pre_nngp, float* pre_ntk,
+                       void* stream);
+
+/*
  * The first two stages of a Myrtle-type network as one launch on a tile of
  * pairs: Conv(3x3, SAME) -> Relu -> Conv(3x3, SAME) -> Relu ->
  * AvgPool(2x2, stride 2) on 32x32 images with 1..4 channels; the first

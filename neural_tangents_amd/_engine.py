@@ -342,9 +342,18 @@ def _state_elems(h, w):
   return h * h * w * w
 
 
+def _keeps_pre(s) -> bool:
+  """Conv(3x3 SAME) -> Relu -> AvgPool(2x2) on an 8x8, 16x16 or 32x32
+  state: nta_stage_keep_pre can also return the stage's stencil means."""
+  return (not s.first and s.conv is not None and s.conv[:3] == (3, 3, False)
+          and not s.circular and s.relu and s.act == (1.0, 0.0)
+          and s.act_kind == 0 and s.pool == (2, 2) and s.hin == s.win
+          and s.hin in (8, 16, 32))
+
+
 def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
                collect_q=None, head_out=None, ld=0, keep=None, qn=None,
-               init=None):
+               init=None, pre=None):
   """Runs all stages for one tile.  Self tile when ``n2 == 0`` (then
   ``collect_q`` receives the normaliser of each ReLU stage, computed from the
   self state on the way).  Returns the final (nngp, ntk) state when the plan
@@ -355,7 +364,10 @@ def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
   selects the fused launch of the first two stages (cross tiles only).
   ``init`` = (nngp, ntk) input state of the tile ([T, H, H, W, W], ntk may
   be None for zero) for plans compiled ``from_state``; then ``xa`` and
-  ``xb`` are unused."""
+  ``xb`` are unused.  ``pre`` (a dict) receives, for every stage index
+  whose stage ``_keeps_pre`` accepts, the stencil means of its inputs
+  (nngp, ntk) at the conv-output resolution (the adjoint's ``aN``,
+  ``aT``)."""
   T = n1 if n2 == 0 else n1 * n2
   nngp = ntk = None
   fl_ntk = want_ntk and n2 != 0
@@ -451,11 +463,21 @@ def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
                       dtype=torch.float32, device=dev)
     out_t = torch.empty_like(out) if fl_ntk else None
     nt = 2 if fl_ntk else 1
-    _launch(f'nta_stage[{s.hin}->{s.hout}]',
-            T * nt * 4 * (nngp[0].numel() + out[0].numel()), T,
-            _lib.ptr(nngp), _lib.ptr(ntk), n1, n2, s.hin, s.win, fh, fw, ph,
-            pw, flags, w_var, b_var, s.act[0], s.act[1], _lib.ptr(q1),
-            _lib.ptr(q2), _lib.ptr(out), _lib.ptr(out_t), stream)
+    if pre is not None and _keeps_pre(s):
+      pN = torch.empty_like(nngp)
+      pT = torch.empty_like(nngp) if fl_ntk else None
+      _launch(f'nta_stage_keep_pre[{s.hin}->{s.hout}]',
+              T * nt * 4 * (2 * nngp[0].numel() + out[0].numel()), T,
+              _lib.ptr(nngp), _lib.ptr(ntk), n1, n2, s.hin, flags, w_var,
+              b_var, _lib.ptr(q1), _lib.ptr(q2), _lib.ptr(out),
+              _lib.ptr(out_t), _lib.ptr(pN), _lib.ptr(pT), stream)
+      pre[si] = (pN, pT)
+    else:
+      _launch(f'nta_stage[{s.hin}->{s.hout}]',
+              T * nt * 4 * (nngp[0].numel() + out[0].numel()), T,
+              _lib.ptr(nngp), _lib.ptr(ntk), n1, n2, s.hin, s.win, fh, fw,
+              ph, pw, flags, w_var, b_var, s.act[0], s.act[1], _lib.ptr(q1),
+              _lib.ptr(q2), _lib.ptr(out), _lib.ptr(out_t), stream)
     nngp, ntk = out, out_t
     if keep is not None:
       keep.append((nngp, ntk))

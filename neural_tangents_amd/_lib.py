@@ -40,6 +40,8 @@ SIGNATURES = {
     'nta_stage_diag': [_P, _I, _I, _I, _I, _I, _I, _F, _F, _P, _P],
     'nta_stage': [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _F, _F, _F, _F,
                   _P, _P, _P, _P, _P],
+    'nta_stage_keep_pre': [_P, _P, _I, _I, _I, _I, _F, _F, _P, _P, _P, _P,
+                           _P, _P, _P],
     'nta_head': [_P, _P, _I, _I, _I, _I, _I, _D, _D, _D, _D, _D, _P, _P, _LL,
                  _P],
     'nta_chol_f32': [_P, _I, _LL, _P, _LL, _P, _P, _P],

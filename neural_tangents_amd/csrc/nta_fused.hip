@@ -193,12 +193,18 @@ __device__ __forceinline__ int stage16_diag(int slot, int k) {
 //   2: as 1, but issued right after the step's planes are written to LDS
 //      (both tensors share one exchange: two barriers per step);
 //   3: as 2 with two steps in flight (two register sets, even/odd steps).
-template <int S, int POOL, bool NTK, int PF, int SL>
+// KP (POOL == 2): the stencil means of the inputs at every pre-pool
+// position, (1/9) sum of the nine taps -- the pre-activations before the
+// w_var, b_var affine map -- also go to pre_n, pre_t ([T][S][S][S][S]),
+// for the adjoint of the stage.
+template <int S, int POOL, bool NTK, int PF, int SL, bool KP = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_stage_fast(
     const float* __restrict__ in_n, const float* __restrict__ in_t, int n1,
     int n2, int ndiag, int ngroups, int nblk, float wv9, float b_var,
     const float* __restrict__ q1, const float* __restrict__ q2,
-    float* __restrict__ out_n, float* __restrict__ out_t) {
+    float* __restrict__ out_n, float* __restrict__ out_t,
+    float* __restrict__ pre_n, float* __restrict__ pre_t) {
+  static_assert(!KP || POOL == 2, "pre-activations of pooled stages only");
   constexpr int H2 = S / 2;
   constexpr int TPP = H2 * H2;            // threads per plane
   constexpr int JPW = 256 / TPP;          // pairs per workgroup
@@ -357,6 +363,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
           for (int tt = 0; tt < NT; ++tt)
             issue_planes<S, 6>(tt ? rt : rn, hh, hp, tofs, r0[tt], r1[tt]);
+        }
+      }
+      if constexpr (KP) {
+        // plane pl at rows (hr, hc) of (i, j); element e at w = 2W + e / 2,
+        // w' = 2Wp + e % 2
+        if (valid) {
+          const int hr[4] = {2 * H, 2 * H + 1, 2 * H + 1, 2 * H};
+          const int hc[4] = {2 * Hp, 2 * Hp + 1, 2 * Hp, 2 * Hp + 1};
+          constexpr float kNinth = 1.0f / 9.0f;
+#pragma unroll
+          for (int tt = 0; tt < NT; ++tt) {
+            float* dst = (tt ? pre_t : pre_n) + p * PER_IN;
+#pragma unroll
+            for (int pl = 0; pl < 4; ++pl) {
+              float* row = dst + ((hr[pl] * S + hc[pl]) * S + 2 * W) * S +
+                           2 * Wp;
+              *reinterpret_cast<float2*>(row) = make_float2(
+                  pre[tt][pl][0] * kNinth, pre[tt][pl][1] * kNinth);
+              *reinterpret_cast<float2*>(row + S) = make_float2(
+                  pre[tt][pl][2] * kNinth, pre[tt][pl][3] * kNinth);
+            }
+          }
         }
       }
       // (sq, rq) at rows 2H, 2H+1 of image i and 2Hp, 2Hp+1 of image j,
@@ -1883,7 +1911,8 @@ template <int S, int POOL, int PF, int SL = 0>
 static int launch_stage_s(const float* in_n, const float* in_t, int n1,
                           int n2, bool ntk, float wv9, float b_var,
                           const float* q1, const float* q2, float* out_n,
-                          float* out_t, hipStream_t st) {
+                          float* out_t, hipStream_t st,
+                          float* pre_n = nullptr, float* pre_t = nullptr) {
   static_assert(SL == 0 || (S == 16 && POOL == 2 && SL == 4),
                 "diagonal slots are laid out for the 16x16 pooled stage");
   constexpr int JPW = 256 / ((S / 2) * (S / 2));
@@ -1893,16 +1922,31 @@ static int launch_stage_s(const float* in_n, const float* in_t, int n1,
   if (ngroups * ndiag > 0x7ffffff0LL)
     return set_error(NTA_EINVAL, "nta_stage: tile too large (%lld pairs)", T);
   const int nblk = pad8(ngroups * ndiag);
+  if constexpr (POOL == 2) {
+    if (pre_n) {
+      if (ntk)
+        hipLaunchKernelGGL((k_stage_fast<S, POOL, true, PF, SL, true>),
+                           dim3(nblk), dim3(256), 0, st, in_n, in_t, n1, n2,
+                           ndiag, (int)ngroups, nblk, wv9, b_var, q1, q2,
+                           out_n, out_t, pre_n, pre_t);
+      else
+        hipLaunchKernelGGL((k_stage_fast<S, POOL, false, PF, SL, true>),
+                           dim3(nblk), dim3(256), 0, st, in_n, in_t, n1, n2,
+                           ndiag, (int)ngroups, nblk, wv9, b_var, q1, q2,
+                           out_n, out_t, pre_n, pre_t);
+      return check_launch("nta_stage_keep_pre");
+    }
+  }
   if (ntk)
     hipLaunchKernelGGL((k_stage_fast<S, POOL, true, PF, SL>), dim3(nblk),
                        dim3(256),
                        0, st, in_n, in_t, n1, n2, ndiag, (int)ngroups, nblk,
-                       wv9, b_var, q1, q2, out_n, out_t);
+                       wv9, b_var, q1, q2, out_n, out_t, nullptr, nullptr);
   else
     hipLaunchKernelGGL((k_stage_fast<S, POOL, false, PF, SL>), dim3(nblk),
                        dim3(256),
                        0, st, in_n, in_t, n1, n2, ndiag, (int)ngroups, nblk,
-                       wv9, b_var, q1, q2, out_n, out_t);
+                       wv9, b_var, q1, q2, out_n, out_t, nullptr, nullptr);
   return check_launch("nta_stage");
 }
 
@@ -2009,9 +2053,62 @@ int launch_first_fast(const float* x1, const float* x2, int n1, int n2, int S,
   return check_launch("nta_first_stage");
 }
 
+// The pooled stage of launch_stage_fast that also writes the inputs'
+// stencil means (pre_n, pre_t).
+static int launch_stage_keep_pre(const float* in_n, const float* in_t,
+                                 int n1, int n2, int S, bool ntk, float w_var,
+                                 float b_var, const float* q1,
+                                 const float* q2, float* out_n, float* out_t,
+                                 float* pre_n, float* pre_t, hipStream_t st) {
+  const float wv9 = w_var / 9.0f;
+  switch (S) {
+    case 32: return launch_stage_s<32, 2, kPrefetch>(
+        in_n, in_t, n1, n2, ntk, wv9, b_var, q1, q2, out_n, out_t, st, pre_n,
+        pre_t);
+    case 16: return launch_stage_s<16, 2, kPrefetch16, kSlots16>(
+        in_n, in_t, n1, n2, ntk, wv9, b_var, q1, q2, out_n, out_t, st, pre_n,
+        pre_t);
+    case 8: return launch_stage_s<8, 2, 1>(
+        in_n, in_t, n1, n2, ntk, wv9, b_var, q1, q2, out_n, out_t, st, pre_n,
+        pre_t);
+  }
+  return kNotHandled;
+}
+
 }  // namespace nta
 
 using namespace nta;
+
+// nta_stage for Conv(3x3 SAME) -> ReLU -> AvgPool(2x2 / 2) on S x S states
+// (S = 8, 16, 32), which also writes the stencil means of the inputs at
+// every pre-pool position (the pre-activations before w_var, b_var) for the
+// stage's adjoint.
+extern "C" int nta_stage_keep_pre(const float* in_nngp, const float* in_ntk,
+                                  int n1, int n2, int S, int flags,
+                                  float w_var, float b_var, const float* q1,
+                                  const float* q2, float* out_nngp,
+                                  float* out_ntk, float* pre_nngp,
+                                  float* pre_ntk, void* stream) {
+  if (n1 < 0 || n2 < 0)
+    return set_error(NTA_EINVAL, "nta_stage_keep_pre: bad n1=%d n2=%d", n1,
+                     n2);
+  if ((S != 8 && S != 16 && S != 32) || !(flags & NTA_CONV) ||
+      !(flags & NTA_RELU) ||
+      (flags & (NTA_CONV_VALID | NTA_CONV_CIRCULAR | NTA_ACT_ERF |
+                NTA_ACT_GELU)))
+    return set_error(NTA_EUNSUPPORTED, "nta_stage_keep_pre: needs a 3x3 "
+                     "SAME Conv + Relu stage on 8x8, 16x16 or 32x32 (S=%d, "
+                     "flags=%d)", S, flags);
+  if (n1 == 0) return NTA_OK;
+  const bool ntk = flags & NTA_NTK;
+  if (!in_nngp || !out_nngp || !pre_nngp || !q1 || (n2 > 0 && !q2) ||
+      (ntk && (!in_ntk || !out_ntk || !pre_ntk)))
+    return set_error(NTA_EINVAL, "nta_stage_keep_pre: null pointer for a "
+                     "requested operand");
+  return launch_stage_keep_pre(in_nngp, in_ntk, n1, n2, S, ntk, w_var, b_var,
+                               q1, q2, out_nngp, out_ntk, pre_nngp, pre_ntk,
+                               (hipStream_t)stream);
+}
 
 // Conv(3x3 SAME) -> ReLU -> Conv(3x3 SAME) -> ReLU -> AvgPool(2x2 / 2) on
 // 32x32 images with C <= 4 channels, as one kernel (k_first_pool32).

@@ -14,7 +14,9 @@ rules).  Here the reverse pass of each fused stage runs on the GPU:
   cotangent with the other image's patches, folded back to pixels.
 
 Per-stage states are recomputed from the checkpoints the forward engine
-keeps at stage (pool) boundaries, one pair tile at a time.  Cotangents of
+keeps at stage (pool) boundaries, one pair tile at a time; the pooled 3x3
+stages of the recomputation also return their inputs' stencil means
+(``nta_stage_keep_pre``), which the ReLU adjoint consumes.  Cotangents of
 the normalisers ``q`` are summed over all pairs of an image and pushed
 through that image's self chain (whose spatial diagonal defines ``q``).
 
@@ -173,8 +175,9 @@ def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=64):
       qa = [q[i0:i0 + n1] for q in q1s]
       qb = [q[j0:j0 + n2] for q in q2s]
       keep = []
+      pre = {}
       _engine._run_chain(plan, x1[i0:i0 + n1], xb_all[j0:j0 + n2], n1, n2,
-                         qa, qb, want_ntk, stream, keep=keep)
+                         qa, qb, want_ntk, stream, keep=keep, pre=pre)
       # head: K_nngp = a m + b, K_ntk = c m + d mt + e (m = mean)
       gn = g_nngp[i0:i0 + n1, j0:j0 + n2].reshape(T)
       gt = g_ntk[i0:i0 + n1, j0:j0 + n2].reshape(T)
@@ -188,7 +191,9 @@ def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=64):
         fh_s, fw_s, valid, w_var, b_var = s.conv
         S = s.hin
         pool = s.pool[0] if s.pool is not None else 1
-        if si > 0:
+        if si in pre:                  # kept by the recomputation
+          aN, aT = pre.pop(si)
+        elif si > 0:
           Nin, Tin = keep[si - 1]
           aN = _stencil(Nin, T, S, fh_s, fw_s, valid, stream)
           aT = _stencil(Tin, T, S, fh_s, fw_s, valid, stream)
@@ -246,9 +251,10 @@ def _self_chain_vjp(plan, x, gq, gx, gwv, gbv, stream, C, fh, fw,
     n = min(chunk, N - n0)
     xa = x[n0:n0 + n]
     keep = []
+    pre = {}
     collect = [None] * L
     _engine._run_chain(plan, xa, None, n, 0, None, None, False, stream,
-                       collect_q=collect, keep=keep)
+                       collect_q=collect, keep=keep, pre=pre)
     gR = None                                   # cotangent of stage output
     for si in range(L - 1, -1, -1):
       s = plan.stages[si]
@@ -256,7 +262,9 @@ def _self_chain_vjp(plan, x, gq, gx, gwv, gbv, stream, C, fh, fw,
       S = s.hin
       pool = s.pool[0] if s.pool is not None else 1
       q = collect[si]
-      if si > 0:
+      if si in pre:
+        aN = pre.pop(si)[0]
+      elif si > 0:
         aN = _stencil(keep[si - 1][0], n, S, fh_s, fw_s, valid, stream)
       else:
         aN = _first_gram(xa, None, n, 0, S, C, fh, fw, stream)
