@@ -258,17 +258,23 @@ __global__ __launch_bounds__(256) void k_relu_bwd_rows(
   }
 }
 
-// gqb[p][k] = sum over h of part[p][h][k] (k < S^2), in order of h.
+// gqb[p][k] = sum over h of part[p][h][k] (k < S^2), in order of h (the
+// S loads of a thread unrolled, so they are in flight together).
+template <int S>
 __global__ void k_bwd_sum_h(const float* __restrict__ part, long long T,
-                            int S, float* __restrict__ gqb) {
-  const long long S2 = (long long)S * S;
+                            float* __restrict__ gqb) {
+  constexpr long long S2 = (long long)S * S;
   const long long n = T * S2;
   for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < n;
        x += (long long)gridDim.x * blockDim.x) {
     const long long p = x / S2, k = x - p * S2;
     const float* src = part + p * S2 * S + k;
+    float t[S];
+#pragma unroll
+    for (int h = 0; h < S; ++h) t[h] = src[h * S2];
     float v = 0.0f;
-    for (int h = 0; h < S; ++h) v += src[h * S2];
+#pragma unroll
+    for (int h = 0; h < S; ++h) v += t[h];
     gqb[x] = v;
   }
 }
@@ -538,8 +544,16 @@ extern "C" int nta_relu_bwd(const float* aN, const float* aT, int n1, int n2,
 #undef NTA_BWD_ROWS_S
 #undef NTA_BWD_ROWS_G
 #undef NTA_BWD_ROWS
-    hipLaunchKernelGGL(k_bwd_sum_h, dim3(grid_for(T * S * S)), dim3(256), 0,
-                       st, part, T, S, gqb);
+    const dim3 sgrid(grid_for(T * S * S));
+    if (S == 32)
+      hipLaunchKernelGGL(k_bwd_sum_h<32>, sgrid, dim3(256), 0, st, part, T,
+                         gqb);
+    else if (S == 16)
+      hipLaunchKernelGGL(k_bwd_sum_h<16>, sgrid, dim3(256), 0, st, part, T,
+                         gqb);
+    else
+      hipLaunchKernelGGL(k_bwd_sum_h<8>, sgrid, dim3(256), 0, st, part, T,
+                         gqb);
   } else {
     const long long per = (long long)S * S * S * S;
     const long long g = relu_bwd_blocks(n1, n2, S);

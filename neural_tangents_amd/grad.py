@@ -126,10 +126,12 @@ def _check_plan(plan):
       raise NotImplementedError('kernel gradients need SAME convolutions.')
 
 
-def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=64):
+def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=512):
   """Cotangents of (nngp, ntk) [N1, N2] -> (gx1, gx2, g_wvar[stage],
   g_bvar[stage]).  ``x2=None`` means ``x2 = x1`` (gx2 is then None and the
-  x1 cotangent collects both arguments)."""
+  x1 cotangent collects both arguments).  Pair tiles hold up to
+  ``tile_pairs`` pairs (about 40 MiB each at 32x32), at most 2^30 state
+  elements (32-bit indexing in the kernels)."""
   _check_plan(plan)
   dev = x1.device
   stream = _lib.stream_handle(dev)
@@ -163,6 +165,8 @@ def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=64):
   nlast = last.hout ** 2 * last.wout ** 2
   if plan.head == 'flatten':
     raise NotImplementedError('kernel gradients support GlobalAvgPool.')
+  S0 = plan.stages[0].hin
+  tile_pairs = max(1, min(tile_pairs, (1 << 30) // S0 ** 4))
   ti = max(1, min(N1, tile_pairs // max(1, min(N2, tile_pairs))))
   tj = max(1, min(N2, tile_pairs))
   P_all1 = _patches(x1, fh, fw)
@@ -243,9 +247,10 @@ def _first_gram(xa, xb, n1, n2, S, C, fh, fw, stream):
 
 
 def _self_chain_vjp(plan, x, gq, gx, gwv, gbv, stream, C, fh, fw,
-                    chunk=32):
+                    chunk=128):
   N = x.shape[0]
   L = len(plan.stages)
+  chunk = max(1, min(chunk, (1 << 30) // plan.stages[0].hin ** 4))
   P_all = _patches(x, fh, fw)
   for n0 in range(0, N, chunk):
     n = min(chunk, N - n0)

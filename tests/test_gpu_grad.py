@@ -114,3 +114,30 @@ def test_kernel_vjp_cross_equals_finite_difference_of_kernel():
   h = 1e-3
   fd = (f(x2 + h * v) - f(x2 - h * v)) / (2 * h)
   assert abs(float(np.sum(g * v)) - fd) <= 1e-4 * np.linalg.norm(g)
+
+
+@pytest.mark.parametrize('sym', [False, True])
+def test_kernel_vjp_is_independent_of_the_pair_tiling(sym):
+  """Tiles of 1 x 2 pairs (ragged at the edges, and off the diagonal of
+  K(x, x) with a self offset) give the cotangents of one tile."""
+  xs, xt, _, _ = _setup(ns=5, nt=3)
+  dev = torch.device('cuda', 0)
+  plan = grad.myrtle_builder(DEPTHS)(1.3, 0.2).plan(xs.shape)
+  x1 = torch.tensor(xs, dtype=torch.float32, device=dev)
+  x2 = None if sym else torch.tensor(xt, dtype=torch.float32, device=dev)
+  n2 = 5 if sym else 3
+  rng = np.random.default_rng(8)
+  g_n = torch.tensor(rng.standard_normal((5, n2)), dtype=torch.float32,
+                     device=dev)
+  g_t = torch.tensor(rng.standard_normal((5, n2)), dtype=torch.float32,
+                     device=dev)
+  one = grad.kernel_vjp(plan, x1, x2, g_n, g_t)
+  tiled = grad.kernel_vjp(plan, x1, x2, g_n, g_t, tile_pairs=2)
+  torch.testing.assert_close(tiled[0], one[0], rtol=1e-5,
+                             atol=1e-5 * float(one[0].abs().max()))
+  if not sym:
+    torch.testing.assert_close(tiled[1], one[1], rtol=1e-5,
+                               atol=1e-5 * float(one[1].abs().max()))
+  for k in (2, 3):                       # d/dw_var, d/db_var per stage
+    np.testing.assert_allclose(tiled[k], one[k], rtol=1e-5,
+                               atol=1e-6 * max(abs(v) for v in one[k]))
