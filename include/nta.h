@@ -157,6 +157,25 @@ int nta_first_pool_stage(const float* x1, const float* x2, int n1, int n2,
                          void* stream);
 
 /*
+ * nta_first_pool_stage that also writes the stencil means of the first
+ * stage's outputs at every position, pre[T][32][32][32][32] = (1/9) sum of
+ * the nine taps -- the second stage's pre-activations before w_var2,
+ * b_var2, which its adjoint (nta_relu_bwd: aN, aT) consumes.
+ * Replaces: the same reference functions as nta_first_pool_stage; the
+ * saved means stand in for what jax.grad keeps of the forward pass of the
+ * second Conv.kernel_fn (linear.py:1387-1398) for its reverse pass.
+ */
+int nta_first_pool_stage_keep_pre(const float* x1, const float* x2, int n1,
+                                  int n2, int H, int W, int C, int flags,
+                                  float w_var1, float b_var1,
+                                  const float* qn1a, const float* qn2a,
+                                  float w_var2, float b_var2,
+                                  const float* qn1b, const float* qn2b,
+                                  float* out_nngp, float* out_ntk,
+                                  float* pre_nngp, float* pre_ntk,
+                                  void* stream);
+
+/*
  * The first three stages of a Myrtle-10-type network as one launch on a
  * tile of pairs: [Conv(3x3, SAME) -> Relu] x 3 -> AvgPool(2x2, stride 2) on
  * 32x32 images with 1..4 channels; neither the first nor the second

@@ -359,7 +359,8 @@ def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
   self state on the way).  Returns the final (nngp, ntk) state when the plan
   has no head, else writes the head output into ``head_out``.  ``keep``
   (a list) receives the output state of every stage (the checkpoints the
-  adjoint recomputes from).  ``qn`` = (first-ReLU normaliser tables of the
+  adjoint recomputes from; ``(None, None)`` for a stage inside a fused
+  launch, whose state is never formed).  ``qn`` = (first-ReLU normaliser tables of the
   tile's x1 and x2 images, then the second ReLU's), from ``nta_qnorm``,
   selects the fused launch of the first two stages (cross tiles only).
   ``init`` = (nngp, ntk) input state of the tile ([T, H, H, W, W], ntk may
@@ -395,6 +396,8 @@ def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
             _lib.ptr(qn[2]), _lib.ptr(qn[3]), s2.conv[3], s2.conv[4],
             _lib.ptr(qn[4]), _lib.ptr(qn[5]), _lib.ptr(out), _lib.ptr(out_t),
             stream)
+    if keep is not None:         # the first two stages' states never formed
+      keep.extend([(None, None), (None, None), (out, out_t)])
     nngp, ntk = out, out_t
     skip = 3
   elif qn is not None and n2 != 0:
@@ -407,12 +410,22 @@ def _run_chain(plan: Plan, xa, xb, n1, n2, qa, qb, want_ntk, stream,
     # written), the accounting the per-layer schedule is measured against.
     nt = 2 if fl_ntk else 1
     alg = T * nt * 4 * (2 * _state_elems(s0.hc, s0.wc) + out[0].numel())
-    _launch('nta_first_pool_stage', alg, T, _lib.ptr(xa), _lib.ptr(xb), n1,
-            n2, H, W, C,
+    args = (_lib.ptr(xa), _lib.ptr(xb), n1, n2, H, W, C,
             _lib.NTK if fl_ntk else 0, s0.conv[3], s0.conv[4],
             _lib.ptr(qn[0]), _lib.ptr(qn[1]), s1.conv[3], s1.conv[4],
-            _lib.ptr(qn[2]), _lib.ptr(qn[3]), _lib.ptr(out),
-            _lib.ptr(out_t), stream)
+            _lib.ptr(qn[2]), _lib.ptr(qn[3]), _lib.ptr(out), _lib.ptr(out_t))
+    if pre is not None:
+      # the second stage's stencil means, [T, 32, 32, 32, 32] per tensor
+      shp = (T, s1.hin, s1.hin, s1.win, s1.win)
+      pN = torch.empty(shp, dtype=torch.float32, device=dev)
+      pT = torch.empty_like(pN) if fl_ntk else None
+      _launch('nta_first_pool_stage_keep_pre', alg + T * nt * 4 * pN[0].numel(),
+              T, *args, _lib.ptr(pN), _lib.ptr(pT), stream)
+      pre[1] = (pN, pT)
+    else:
+      _launch('nta_first_pool_stage', alg, T, *args, stream)
+    if keep is not None:         # the first stage's state is never formed
+      keep.extend([(None, None), (out, out_t)])
     nngp, ntk = out, out_t
     skip = 2
   for si, s in enumerate(plan.stages):

@@ -59,6 +59,9 @@ SIGNATURES = {
                        _I, _D, _D, _D, _D, _D, _P, _P, _LL, _P],
     'nta_first_pool_stage': [_P, _P, _I, _I, _I, _I, _I, _I, _F, _F, _P, _P,
                              _F, _F, _P, _P, _P, _P, _P],
+    'nta_first_pool_stage_keep_pre': [_P, _P, _I, _I, _I, _I, _I, _I, _F, _F,
+                                      _P, _P, _F, _F, _P, _P, _P, _P, _P, _P,
+                                      _P],
     'nta_first3_pool_stage': [_P, _P, _I, _I, _I, _I, _I, _I, _F, _F, _P,
                               _P, _F, _F, _P, _P, _F, _F, _P, _P, _P, _P,
                               _P],

@@ -917,7 +917,11 @@ __device__ __forceinline__ int f12_diag(int slot, int sw) {
   return f12_diag8(g, sw);
 }
 
-template <bool NTK>
+// KP: the stencil means of the first stage's outputs at every position,
+// (1/9) sum of the nine taps -- the second stage's pre-activations before
+// its w_var, b_var affine map -- also go to pre_n, pre_t
+// ([T][32][32][32][32]), for the adjoint of the second stage.
+template <bool NTK, bool KP = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
                     int n1, int n2, int C, int nblk, float scale1, float b1,
@@ -925,9 +929,11 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
                     const float2* __restrict__ qa2, float wv9, float b2,
                     const float2* __restrict__ qb1,
                     const float2* __restrict__ qb2, float* __restrict__ out_n,
-                    float* __restrict__ out_t) {
+                    float* __restrict__ out_t, float* __restrict__ pre_n,
+                    float* __restrict__ pre_t) {
   constexpr int S = 32, H2 = 16, SO = 16;
   constexpr long long PER_OUT = (long long)SO * SO * SO * SO;
+  constexpr long long PER_IN = (long long)S * S * S * S;
   constexpr int NT = NTK ? 2 : 1;
   // Images i, j as FP16 hi/lo pairs: xh[img][hl][1 + h][1 + w][c] (zero
   // padded, channels padded to 4).
@@ -1300,6 +1306,25 @@ void k_first_pool32(const float* __restrict__ x1, const float* __restrict__ x2,
     };
     advance(vn, 0);
     if constexpr (NTK) advance(vt, NT - 1);
+    if constexpr (KP) {
+      // plane pl at rows (hr, hc) of (i, j), element e at w = 2W + e / 2,
+      // w' = 2Wp + e % 2; the first stage's values are 2 pi too large
+      const int hr[4] = {2 * H, 2 * H + 1, 2 * H + 1, 2 * H};
+      const int hc[4] = {2 * Hp, 2 * Hp + 1, 2 * Hp, 2 * Hp + 1};
+      constexpr float kMean = kInv2Pi / 9.0f;
+#pragma unroll
+      for (int tt = 0; tt < NT; ++tt) {
+        float* dst = (tt ? pre_t : pre_n) + p * PER_IN;
+#pragma unroll
+        for (int pl = 0; pl < 4; ++pl) {
+          float* row = dst + ((hr[pl] * S + hc[pl]) * S + 2 * W) * S + 2 * Wp;
+          *reinterpret_cast<float2*>(row) =
+              make_float2(pre[tt][pl][0] * kMean, pre[tt][pl][1] * kMean);
+          *reinterpret_cast<float2*>(row + S) =
+              make_float2(pre[tt][pl][2] * kMean, pre[tt][pl][3] * kMean);
+        }
+      }
+    }
     // The next step's matrix work overlaps this step's second-stage ReLU
     // (past the sweep end it computes planes that are never used).
     gram6(H + 1, out.acc);
@@ -2111,28 +2136,31 @@ extern "C" int nta_stage_keep_pre(const float* in_nngp, const float* in_ntk,
 }
 
 // Conv(3x3 SAME) -> ReLU -> Conv(3x3 SAME) -> ReLU -> AvgPool(2x2 / 2) on
-// 32x32 images with C <= 4 channels, as one kernel (k_first_pool32).
-extern "C" int nta_first_pool_stage(
-    const float* x1, const float* x2, int n1, int n2, int H, int W, int C,
-    int flags, float w_var1, float b_var1, const float* qn1a,
-    const float* qn2a, float w_var2, float b_var2, const float* qn1b,
-    const float* qn2b, float* out_nngp, float* out_ntk, void* stream) {
+// 32x32 images with C <= 4 channels, as one kernel (k_first_pool32); with
+// pre_nngp given, the second stage's stencil means go there too.
+static int first_pool_stage(const char* name, const float* x1,
+                            const float* x2, int n1, int n2, int H, int W,
+                            int C, int flags, float w_var1, float b_var1,
+                            const float* qn1a, const float* qn2a,
+                            float w_var2, float b_var2, const float* qn1b,
+                            const float* qn2b, float* out_nngp,
+                            float* out_ntk, float* pre_nngp, float* pre_ntk,
+                            bool keep_pre, void* stream) {
   if (n1 < 0 || n2 < 0)
-    return set_error(NTA_EINVAL, "nta_first_pool_stage: bad n1=%d n2=%d", n1,
-                     n2);
+    return set_error(NTA_EINVAL, "%s: bad n1=%d n2=%d", name, n1, n2);
   if (H != 32 || W != 32 || C < 1 || C > 4)
-    return set_error(NTA_EUNSUPPORTED, "nta_first_pool_stage: needs 32x32 "
-                     "images with 1..4 channels (got %dx%dx%d)", H, W, C);
+    return set_error(NTA_EUNSUPPORTED, "%s: needs 32x32 images with 1..4 "
+                     "channels (got %dx%dx%d)", name, H, W, C);
   if (n1 == 0) return NTA_OK;
   const bool ntk = flags & NTA_NTK;
   if (!x1 || (n2 > 0 && !x2) || !qn1a || !qn1b ||
-      (n2 > 0 && (!qn2a || !qn2b)) || !out_nngp || (ntk && !out_ntk))
-    return set_error(NTA_EINVAL, "nta_first_pool_stage: null pointer for a "
-                     "requested operand");
+      (n2 > 0 && (!qn2a || !qn2b)) || !out_nngp || (ntk && !out_ntk) ||
+      (keep_pre && (!pre_nngp || (ntk && !pre_ntk))))
+    return set_error(NTA_EINVAL, "%s: null pointer for a requested operand",
+                     name);
   const long long T = n2 == 0 ? (long long)n1 : (long long)n1 * n2;
   if (T * kF12Slots > 0x7ffffff0LL)
-    return set_error(NTA_EINVAL, "nta_first_pool_stage: tile too large "
-                     "(%lld pairs)", T);
+    return set_error(NTA_EINVAL, "%s: tile too large (%lld pairs)", name, T);
   const int nblk = (int)((T * kF12Slots + 7) / 8 * 8);
   const float scale1 = w_var1 / (9.0f * (float)C);
   const float wv9 = w_var2 / 9.0f;
@@ -2140,15 +2168,44 @@ extern "C" int nta_first_pool_stage(
   const float2* a2 = reinterpret_cast<const float2*>(qn2a);
   const float2* c1 = reinterpret_cast<const float2*>(qn1b);
   const float2* c2 = reinterpret_cast<const float2*>(qn2b);
-  if (ntk)
-    hipLaunchKernelGGL(k_first_pool32<true>, dim3(nblk), dim3(256), 0,
-                       (hipStream_t)stream, x1, x2, n1, n2, C, nblk, scale1,
-                       b_var1, a1, a2, wv9, b_var2, c1, c2, out_nngp, out_ntk);
-  else
-    hipLaunchKernelGGL(k_first_pool32<false>, dim3(nblk), dim3(256), 0,
-                       (hipStream_t)stream, x1, x2, n1, n2, C, nblk, scale1,
-                       b_var1, a1, a2, wv9, b_var2, c1, c2, out_nngp, out_ntk);
-  return check_launch("nta_first_pool_stage");
+  const hipStream_t st = (hipStream_t)stream;
+#define NTA_F12_LAUNCH(NTK_, KP_)                                           \
+  hipLaunchKernelGGL((k_first_pool32<NTK_, KP_>), dim3(nblk), dim3(256), 0, \
+                     st, x1, x2, n1, n2, C, nblk, scale1, b_var1, a1, a2,   \
+                     wv9, b_var2, c1, c2, out_nngp, out_ntk, pre_nngp,      \
+                     pre_ntk)
+  if (keep_pre) {
+    if (ntk) NTA_F12_LAUNCH(true, true);
+    else NTA_F12_LAUNCH(false, true);
+  } else {
+    if (ntk) NTA_F12_LAUNCH(true, false);
+    else NTA_F12_LAUNCH(false, false);
+  }
+#undef NTA_F12_LAUNCH
+  return check_launch(name);
+}
+
+extern "C" int nta_first_pool_stage(
+    const float* x1, const float* x2, int n1, int n2, int H, int W, int C,
+    int flags, float w_var1, float b_var1, const float* qn1a,
+    const float* qn2a, float w_var2, float b_var2, const float* qn1b,
+    const float* qn2b, float* out_nngp, float* out_ntk, void* stream) {
+  return first_pool_stage("nta_first_pool_stage", x1, x2, n1, n2, H, W, C,
+                          flags, w_var1, b_var1, qn1a, qn2a, w_var2, b_var2,
+                          qn1b, qn2b, out_nngp, out_ntk, nullptr, nullptr,
+                          false, stream);
+}
+
+extern "C" int nta_first_pool_stage_keep_pre(
+    const float* x1, const float* x2, int n1, int n2, int H, int W, int C,
+    int flags, float w_var1, float b_var1, const float* qn1a,
+    const float* qn2a, float w_var2, float b_var2, const float* qn1b,
+    const float* qn2b, float* out_nngp, float* out_ntk, float* pre_nngp,
+    float* pre_ntk, void* stream) {
+  return first_pool_stage("nta_first_pool_stage_keep_pre", x1, x2, n1, n2, H,
+                          W, C, flags, w_var1, b_var1, qn1a, qn2a, w_var2,
+                          b_var2, qn1b, qn2b, out_nngp, out_ntk, pre_nngp,
+                          pre_ntk, true, stream);
 }
 
 // Conv(3x3 SAME) -> ReLU, three times, -> AvgPool(2x2 / 2) on 32x32 images

@@ -171,6 +171,12 @@ def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=512):
   tj = max(1, min(N2, tile_pairs))
   P_all1 = _patches(x1, fh, fw)
   P_all2 = P_all1 if sym else _patches(x2, fh, fw)
+  # Myrtle-5/7 at 32x32: the first two stages recomputed by one launch
+  # (nta_first_pool_stage_keep_pre), from the normalisers' (sqrt q, 1/sqrt q)
+  fused = _engine._fused_stages(plan, x1.shape) == 2
+  if fused:
+    qn1 = [_engine._qnorm(q1s[k], stream) for k in range(2)]
+    qn2 = qn1 if sym else [_engine._qnorm(q2s[k], stream) for k in range(2)]
   for i0 in range(0, N1, ti):
     n1 = min(ti, N1 - i0)
     for j0 in range(0, N2, tj):
@@ -180,8 +186,12 @@ def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=512):
       qb = [q[j0:j0 + n2] for q in q2s]
       keep = []
       pre = {}
+      qn = None
+      if fused:
+        qn = (qn1[0][i0:i0 + n1], qn2[0][j0:j0 + n2], qn1[1][i0:i0 + n1],
+              qn2[1][j0:j0 + n2])
       _engine._run_chain(plan, x1[i0:i0 + n1], xb_all[j0:j0 + n2], n1, n2,
-                         qa, qb, want_ntk, stream, keep=keep, pre=pre)
+                         qa, qb, want_ntk, stream, keep=keep, pre=pre, qn=qn)
       # head: K_nngp = a m + b, K_ntk = c m + d mt + e (m = mean)
       gn = g_nngp[i0:i0 + n1, j0:j0 + n2].reshape(T)
       gt = g_ntk[i0:i0 + n1, j0:j0 + n2].reshape(T)

@@ -66,3 +66,29 @@ def test_first_pool_stage_rejects_other_shapes_without_gpu_work():
                                 None)
   assert rc == _lib.NTA_EINVAL
   assert lib.nta_qnorm(None, -1, None, None) == _lib.NTA_EINVAL
+
+
+def test_keep_pre_entry_points_check_shapes_without_gpu_work():
+  lib = _lib.load()
+  rc = lib.nta_first_pool_stage_keep_pre(None, None, 1, 1, 16, 16, 3, 0, 2.0,
+                                         0.0, None, None, 2.0, 0.0, None,
+                                         None, None, None, None, None, None)
+  assert rc == _lib.NTA_EUNSUPPORTED
+  assert b'nta_first_pool_stage_keep_pre' in lib.nta_last_error()
+  rc = lib.nta_first_pool_stage_keep_pre(None, None, 2, 1, 32, 32, 3, 0, 2.0,
+                                         0.0, None, None, 2.0, 0.0, None,
+                                         None, None, None, None, None, None)
+  assert rc == _lib.NTA_EINVAL
+  relu = _lib.CONV | _lib.RELU
+  assert lib.nta_stage_keep_pre(None, None, 1, 0, 12, relu, 2.0, 0.0, None,
+                                None, None, None, None, None,
+                                None) == _lib.NTA_EUNSUPPORTED
+  assert lib.nta_stage_keep_pre(None, None, 1, 0, 16, _lib.CONV, 2.0, 0.0,
+                                None, None, None, None, None, None,
+                                None) == _lib.NTA_EUNSUPPORTED
+  assert lib.nta_stage_keep_pre(None, None, -1, 0, 16, relu, 2.0, 0.0, None,
+                                None, None, None, None, None,
+                                None) == _lib.NTA_EINVAL
+  assert lib.nta_stage_keep_pre(None, None, 1, 0, 16, relu, 2.0, 0.0, None,
+                                None, None, None, None, None,
+                                None) == _lib.NTA_EINVAL

@@ -1,7 +1,8 @@
-"""nta_stage_keep_pre: the pooled 3x3 stage that also returns its inputs'
-stencil means (the pre-activations the stage's adjoint consumes).  Its
-pooled outputs must match nta_stage's, and its means the stencil kernel's,
-on the states of a Myrtle-5 network (needs a GPU)."""
+"""nta_stage_keep_pre and nta_first_pool_stage_keep_pre: the pooled 3x3
+stages that also return their inputs' stencil means (the pre-activations the
+stage's adjoint consumes).  Their pooled outputs must match the launches
+without the means, and their means the stencil kernel's, on the states of a
+Myrtle-5 network (needs a GPU)."""
 import pytest
 import torch
 
@@ -67,6 +68,36 @@ def test_keep_pre_matches_stage_and_stencil(side, n1, n2, depths):
                                  atol=2e-6 * float(want_t.abs().max()))
     else:
       assert pT is None
+
+
+def test_fused_first_two_stages_keep_the_second_stage_means():
+  """Myrtle-5 32x32 cross tiles (3 and 4 x 5 pairs): the fused launch's
+  second-stage means and every later stage match the per-layer path's (the
+  fused first stage's split-FP16 Gram is FP32-accurate)."""
+  plan, x = _plan_and_images(32, 9, [2, 1, 1])
+  assert _engine._fused_stages(plan, x.shape) == 2
+  stream = _lib.stream_handle(x.device)
+  L = len(plan.stages)
+  for n1, n2 in ((1, 3), (4, 5)):
+    xa, xb = x[:n1], x[n1:n1 + n2]
+    qa = _engine._self_chain(plan, xa, False, _engine.DEFAULT_TILE_BYTES,
+                             stream)[0]
+    qb = _engine._self_chain(plan, xb, False, _engine.DEFAULT_TILE_BYTES,
+                             stream)[0]
+    qn = tuple(t for k in range(2) for t in (_engine._qnorm(qa[k], stream),
+                                             _engine._qnorm(qb[k], stream)))
+    args = (plan, xa, xb, n1, n2, qa, qb, True, stream)
+    keep0, pre0, keep1, pre1 = [], {}, [], {}
+    _engine._run_chain(*args, keep=keep0, pre=pre0)
+    _engine._run_chain(*args, keep=keep1, pre=pre1, qn=qn)
+    assert sorted(pre1) == sorted(pre0) == [1, 2, 3]
+    assert keep1[0] == (None, None)
+    for si in range(1, L):
+      for a, b in zip(keep0[si], keep1[si]):
+        assert float((a - b).abs().max()) <= 1e-5 * float(a.abs().max())
+    for si in (1, 2, 3):
+      for a, b in zip(pre0[si], pre1[si]):
+        assert float((a - b).abs().max()) <= 1e-5 * float(a.abs().max())
 
 
 def test_keep_pre_rejects_other_stages():
