@@ -11,7 +11,7 @@ test images of that workload (same generator, same order) run end to end:
 * ``gradient_descent_mse_ensemble`` with the mixed solver (FP32 MFMA
   Cholesky + FP64 refinement) is checked against scipy's FP64
   ``cho_factor`` / ``cho_solve`` of the same kernels at 1e-6 (alpha and the
-  test predictions), the north star's bar for the precision form.
+  test predictions), the accuracy bar of the precision form.
 """
 import numpy as np
 import pytest
