@@ -365,13 +365,28 @@ __global__ void k_bwd_qsum(const float* __restrict__ ea,
 }
 
 // acc[0], acc[1] += the sums of the (d/dw_var, d/db_var) partial slots
-// gwb[2 s], gwb[2 s + 1], s < parts, in a fixed order (one workgroup).
+// gwb[2 s], gwb[2 s + 1], s < parts, in a fixed order (one workgroup; a
+// thread's slots t, t + 256, ... loaded eight at a time, added in order).
 __global__ __launch_bounds__(256) void k_bwd_acc_wb(
     const double* __restrict__ gwb, long long parts, double* __restrict__ acc) {
   __shared__ double red[2][256];
   const int t = threadIdx.x;
   double w = 0.0, b = 0.0;
-  for (long long q = t; q < parts; q += 256) {
+  long long q = t;
+  for (; q + 7 * 256 < parts; q += 8 * 256) {
+    double vw[8], vb[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      vw[u] = gwb[2 * (q + u * 256)];
+      vb[u] = gwb[2 * (q + u * 256) + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      w += vw[u];
+      b += vb[u];
+    }
+  }
+  for (; q < parts; q += 256) {
     w += gwb[2 * q];
     b += gwb[2 * q + 1];
   }
