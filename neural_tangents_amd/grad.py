@@ -126,6 +126,16 @@ def _check_plan(plan):
       raise NotImplementedError('kernel gradients need SAME convolutions.')
 
 
+def _pair_tile(n1, n2, tile_pairs, side):
+  """(rows, columns) of the backward's pair tiles: whole rows of the
+  N1 x N2 grid where they fit, at most ``tile_pairs`` pairs and at most
+  2^30 elements of a [pairs, side^4] state (32-bit indexing)."""
+  tile_pairs = max(1, min(tile_pairs, (1 << 30) // side ** 4))
+  tj = max(1, min(n2, tile_pairs))
+  ti = max(1, min(n1, tile_pairs // tj))
+  return ti, tj
+
+
 def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=512):
   """Cotangents of (nngp, ntk) [N1, N2] -> (gx1, gx2, g_wvar[stage],
   g_bvar[stage]).  ``x2=None`` means ``x2 = x1`` (gx2 is then None and the
@@ -165,10 +175,7 @@ def kernel_vjp(plan, x1, x2, g_nngp, g_ntk, tile_pairs=512):
   nlast = last.hout ** 2 * last.wout ** 2
   if plan.head == 'flatten':
     raise NotImplementedError('kernel gradients support GlobalAvgPool.')
-  S0 = plan.stages[0].hin
-  tile_pairs = max(1, min(tile_pairs, (1 << 30) // S0 ** 4))
-  ti = max(1, min(N1, tile_pairs // max(1, min(N2, tile_pairs))))
-  tj = max(1, min(N2, tile_pairs))
+  ti, tj = _pair_tile(N1, N2, tile_pairs, plan.stages[0].hin)
   P_all1 = _patches(x1, fh, fw)
   P_all2 = P_all1 if sym else _patches(x2, fh, fw)
   # Myrtle-5/7 at 32x32: the first two stages recomputed by one launch
